@@ -1,0 +1,237 @@
+"""Python host binding of libgphi, the MI355X-native tiled graph-traversal library.
+
+The product is the C ABI in ``include/gphi.h`` (HIP kernels for gfx950 behind
+``extern "C"`` entry points).  This module is a thin ctypes layer over it that
+mirrors the reference apps' roles:
+
+=====================  ==========================================================
+reference              here
+=====================  ==========================================================
+graph_input            :meth:`Context.load_text` (apps/bfs_simd/bfs.cpp:968)
+graph_prepare          :meth:`Graph.bfs` (apps/bfs_simd/bfs.cpp:753)
+sssp_weighted          :meth:`Graph.sssp` (apps/sssp_simd/sssp.cpp:969)
+page_rank              :meth:`Graph.pagerank` (apps/pageRank_simd/page_rank.cpp:362)
+bellman-ford page_rank :meth:`Graph.bellman_ford` (apps/bellman-ford_simd/bellman-ford.cpp:291)
+=====================  ==========================================================
+
+There is no CPU fallback: if ``libgphi.so`` is missing or no GPU is visible,
+every call raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+PKG_DIR = Path(__file__).resolve().parent
+LIB_PATH = PKG_DIR / "libgphi.so"
+MAX_STEPS = 512
+TILES_COL_STRIPE = 0
+TILES_ROW_MAJOR = 1
+
+_ERRORS = {0: "OK", -1: "E_IO", -2: "E_FORMAT", -3: "E_RANGE", -4: "E_OOM", -5: "E_HIP",
+           -6: "E_ARG", -7: "E_STATE"}
+
+
+class GphiError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{_ERRORS.get(code, code)}: {msg}")
+        self.code = code
+        self.msg = msg
+
+
+class Opts(C.Structure):
+    _fields_ = [("device", C.c_int), ("t_ratio", C.c_uint32)]
+
+
+class GraphDesc(C.Structure):
+    _fields_ = [("n", C.c_uint32), ("m", C.c_uint64), ("tile_width", C.c_uint32),
+                ("stripe_length", C.c_uint32), ("tile_order", C.c_int),
+                ("degrees", C.c_void_p), ("csr_edges", C.c_void_p), ("csr_weights", C.c_void_p),
+                ("tile_heads", C.c_void_p), ("tile_tails", C.c_void_p),
+                ("tile_weights", C.c_void_p), ("tile_offsets", C.c_void_p)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("steps", C.c_uint32), ("dir", C.c_uint32 * MAX_STEPS),
+                ("fsize", C.c_uint32 * MAX_STEPS), ("fdeg", C.c_uint32 * MAX_STEPS),
+                ("sweep_ms", C.c_float * MAX_STEPS), ("update_ms", C.c_float * MAX_STEPS),
+                ("init_ms", C.c_float), ("total_ms", C.c_float), ("reached", C.c_uint64),
+                ("edges_traversed", C.c_uint64)]
+
+    def trace(self) -> np.ndarray:
+        """(steps, 3) array of (dir, |F|, sum outdeg F) per expanded frontier."""
+        k = self.steps
+        return np.stack([np.ctypeslib.as_array(self.dir)[:k], np.ctypeslib.as_array(self.fsize)[:k],
+                         np.ctypeslib.as_array(self.fdeg)[:k]], axis=1).astype(np.uint32)
+
+    def sweep_times(self) -> np.ndarray:
+        return np.ctypeslib.as_array(self.sweep_ms)[: self.steps].copy()
+
+    def update_times(self) -> np.ndarray:
+        return np.ctypeslib.as_array(self.update_ms)[: self.steps].copy()
+
+
+_lib = None
+_P = C.c_void_p
+
+
+def _declare(lib):
+    u32, u64, i32 = C.c_uint32, C.c_uint64, C.c_int
+    sig = {
+        "gphi_version": (C.c_char_p, []),
+        "gphi_open": (i32, [C.POINTER(Opts), C.POINTER(_P)]),
+        "gphi_close": (None, [_P]),
+        "gphi_last_error": (C.c_char_p, [_P]),
+        "gphi_graph_from_desc": (i32, [_P, C.POINTER(GraphDesc), C.POINTER(_P)]),
+        "gphi_graph_load_text": (i32, [_P, C.c_char_p, u32, u32, i32, C.POINTER(_P)]),
+        "gphi_graph_load_text_rowmajor": (i32, [_P, C.c_char_p, u32, u32, C.POINTER(_P)]),
+        "gphi_graph_info": (i32, [_P, _P, _P, _P, _P, _P]),
+        "gphi_graph_device_bytes": (u64, [_P]),
+        "gphi_graph_free": (None, [_P]),
+        "gphi_bfs": (i32, [_P, u32, _P, _P, C.POINTER(Stats)]),
+        "gphi_sssp": (i32, [_P, u32, _P, C.POINTER(Stats)]),
+        "gphi_pagerank": (i32, [_P, u32, C.c_float, C.c_float, _P, _P]),
+        "gphi_bellman_ford": (i32, [_P, u32, u32, _P, _P, _P]),
+        "gphi_bfs_alg_bytes": (i32, [_P, _P, C.POINTER(Stats), _P]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+def load_library(path: Path | str | None = None):
+    """Load libgphi.so (built in-tree by ``python -m bfs_simd_benchmarks_amd.build``)."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = Path(path) if path else LIB_PATH
+    if not p.exists():
+        raise GphiError(-7, f"{p} is not built; run python -m bfs_simd_benchmarks_amd.build")
+    lib = _declare(C.CDLL(str(p)))
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def _ptr(a: np.ndarray | None):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _u32(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.uint32)
+
+
+class Context:
+    """One HIP device + stream (gphi_open)."""
+
+    def __init__(self, device: int = 0, t_ratio: int = 20):
+        self.lib = load_library()
+        h = _P()
+        rc = self.lib.gphi_open(C.byref(Opts(device, t_ratio)), C.byref(h))
+        if rc:
+            raise GphiError(rc, self.lib.gphi_last_error(None).decode())
+        self.h = h
+        self.device = device
+
+    def check(self, rc: int):
+        if rc:
+            raise GphiError(rc, self.lib.gphi_last_error(self.h).decode())
+
+    def close(self):
+        if self.h:
+            self.lib.gphi_close(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def graph_from_arrays(self, n: int, degrees, *, W: int, S: int, csr_edges=None,
+                          csr_weights=None, tile_heads=None, tile_tails=None, tile_weights=None,
+                          tile_offsets=None, tile_order: int = TILES_COL_STRIPE) -> "Graph":
+        """Upload graph_input's arrays (apps/bfs_simd/bfs.cpp:968-1118)."""
+        keep = {k: (None if v is None else _u32(v)) for k, v in dict(
+            degrees=degrees, csr_edges=csr_edges, csr_weights=csr_weights, tile_heads=tile_heads,
+            tile_tails=tile_tails, tile_weights=tile_weights, tile_offsets=tile_offsets).items()}
+        m = len(keep["csr_edges"]) if keep["csr_edges"] is not None else len(keep["tile_heads"])
+        d = GraphDesc(n, m, W, S, tile_order, *(_ptr(keep[k]) for k in (
+            "degrees", "csr_edges", "csr_weights", "tile_heads", "tile_tails", "tile_weights",
+            "tile_offsets")))
+        h = _P()
+        self.check(self.lib.gphi_graph_from_desc(self.h, C.byref(d), C.byref(h)))
+        return Graph(self, h)
+
+    def load_text(self, prefix: str, W: int, S: int, weighted: bool = False) -> "Graph":
+        """prefix_col-S-coo-tiled-W-* + prefix_untiled-* (apps/bfs_simd/bfs.cpp:983-1113)."""
+        h = _P()
+        self.check(self.lib.gphi_graph_load_text(self.h, str(prefix).encode(), W, S, int(weighted),
+                                                 C.byref(h)))
+        return Graph(self, h)
+
+    def load_text_rowmajor(self, prefix: str, W: int, S: int) -> "Graph":
+        """prefix_coo-tiled-W-* (apps/pageRank_simd/page_rank.cpp:468-568)."""
+        h = _P()
+        self.check(self.lib.gphi_graph_load_text_rowmajor(self.h, str(prefix).encode(), W, S,
+                                                          C.byref(h)))
+        return Graph(self, h)
+
+
+class Graph:
+    def __init__(self, ctx: Context, h):
+        self.ctx, self.h = ctx, h
+        n, m, W, S, wt = C.c_uint32(), C.c_uint64(), C.c_uint32(), C.c_uint32(), C.c_int()
+        ctx.check(ctx.lib.gphi_graph_info(h, C.byref(n), C.byref(m), C.byref(W), C.byref(S),
+                                          C.byref(wt)))
+        self.n, self.m, self.W, self.S, self.weighted = n.value, m.value, W.value, S.value, bool(wt.value)
+
+    @property
+    def device_bytes(self) -> int:
+        return int(self.ctx.lib.gphi_graph_device_bytes(self.h))
+
+    def free(self):
+        if self.h:
+            self.ctx.lib.gphi_graph_free(self.h)
+            self.h = None
+
+    def bfs(self, src: int, parents: bool = True):
+        """Hybrid BFS (graph_prepare).  Returns (levels int32, parents int32|None, Stats)."""
+        lv = np.empty(self.n, np.int32)
+        pa = np.empty(self.n, np.int32) if parents else None
+        st = Stats()
+        self.ctx.check(self.ctx.lib.gphi_bfs(self.h, src, _ptr(lv), _ptr(pa), C.byref(st)))
+        return lv, pa, st
+
+    def sssp(self, src: int):
+        d = np.empty(self.n, np.uint32)
+        st = Stats()
+        self.ctx.check(self.ctx.lib.gphi_sssp(self.h, src, _ptr(d), C.byref(st)))
+        return d, st
+
+    def pagerank(self, iters: int = 1, damping: float = 0.85, init: float = 1.0):
+        r = np.empty(self.n, np.float32)
+        ms = C.c_float()
+        self.ctx.check(self.ctx.lib.gphi_pagerank(self.h, iters, damping, init, _ptr(r), C.byref(ms)))
+        return r, ms.value
+
+    def bellman_ford(self, src: int = 0, max_sweeps: int = 99):
+        d = np.empty(self.n, np.uint32)
+        run, ms = C.c_uint32(), C.c_float()
+        self.ctx.check(self.ctx.lib.gphi_bellman_ford(self.h, src, max_sweeps, _ptr(d), C.byref(run),
+                                                      C.byref(ms)))
+        return d, run.value, ms.value
+
+    def alg_bytes(self, levels: np.ndarray, st: Stats) -> np.ndarray:
+        out = np.zeros(max(st.steps, 1), np.float64)
+        lv = np.ascontiguousarray(levels, dtype=np.int32)
+        self.ctx.check(self.ctx.lib.gphi_bfs_alg_bytes(self.h, _ptr(lv), C.byref(st), _ptr(out)))
+        return out[: st.steps]
+
+
+__all__ = ["Context", "Graph", "GphiError", "Stats", "load_library", "TILES_COL_STRIPE",
+           "TILES_ROW_MAJOR", "LIB_PATH"]

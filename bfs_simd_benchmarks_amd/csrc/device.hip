@@ -1,0 +1,879 @@
+// device.hip -- gfx950 kernels and their host drivers for libgphi.
+//
+// Hot path (reference apps/bfs_simd/bfs.cpp):
+//   dense step  : tile sweep over the column-stripe COO (bfs_kernel_dense
+//                 :181-245 driven by scheduler_dense/BFS_dense :246-388)
+//   sparse step : CSR push from the frontier queue (BFS_kernel_sparse :510-711)
+//   update      : frontier bookkeeping + direction rule (update_dense :91-179,
+//                 update_sparse :467-509, to_dense :54-88, to_sparse :400-465,
+//                 heuristic :829-834) fused into one vertex pass
+// The same chunked tile sweep serves SSSP (apps/sssp_simd/sssp.cpp:344-445),
+// PageRank (apps/pageRank_simd/page_rank.cpp:128-199) and hop relaxation
+// (apps/bellman-ford_simd/bellman-ford.cpp:129-191).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <cstring>
+
+#include "internal.h"
+
+namespace gphi {
+
+// ---------------------------------------------------------------------------
+// wave helpers (wave64)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ uint32_t wave_sum(uint32_t x)
+{
+	for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+	return x;
+}
+
+// exclusive prefix sum across the wave; *total gets the wave sum
+__device__ __forceinline__ uint32_t wave_excl_scan(uint32_t x, uint32_t *total)
+{
+	uint32_t lane = lane_id();
+	uint32_t inc = x;
+	for (int o = 1; o < 64; o <<= 1) {
+		uint32_t y = __shfl_up(inc, o, 64);
+		if (lane >= (uint32_t)o) inc += y;
+	}
+	*total = __shfl(inc, 63, 64);
+	return inc - x;
+}
+
+__device__ __forceinline__ bool test_bit(const uint32_t *__restrict__ bm, uint32_t v)
+{
+	return (bm[v >> 5] >> (v & 31)) & 1u;
+}
+
+// ---------------------------------------------------------------------------
+// graph upload helpers
+// ---------------------------------------------------------------------------
+__global__ void fill_u32_kernel(uint32_t *p, uint64_t count, uint32_t value)
+{
+	uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+	uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+	for (; i < count; i += stride) p[i] = value;
+}
+
+// Chunk summary: range of head rows / tail columns of the real edges of each
+// kChunk-edge chunk.  One block per chunk (grid-stride).
+__global__ __launch_bounds__(kBlock) void chunk_meta_kernel(const uint32_t *__restrict__ heads,
+                                                            const uint32_t *__restrict__ tails,
+                                                            uint32_t n, uint32_t W, uint64_t nchunks,
+                                                            ChunkMeta *__restrict__ meta)
+{
+	__shared__ uint32_t s[4][kBlock / 64];
+	for (uint64_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
+		uint32_t rlo = UINT32_MAX, rhi = 0, clo = UINT32_MAX, chi = 0;
+		const uint32_t *H = heads + c * kChunk, *T = tails + c * kChunk;
+		for (uint32_t i = threadIdx.x; i < kChunk; i += kBlock) {
+			uint32_t h = H[i];
+			if (h >= n) continue;
+			uint32_t r = h / W, col = T[i] / W;
+			rlo = min(rlo, r);
+			rhi = max(rhi, r);
+			clo = min(clo, col);
+			chi = max(chi, col);
+		}
+		for (int o = 32; o > 0; o >>= 1) {
+			rlo = min(rlo, (uint32_t)__shfl_xor(rlo, o, 64));
+			rhi = max(rhi, (uint32_t)__shfl_xor(rhi, o, 64));
+			clo = min(clo, (uint32_t)__shfl_xor(clo, o, 64));
+			chi = max(chi, (uint32_t)__shfl_xor(chi, o, 64));
+		}
+		uint32_t w = threadIdx.x >> 6;
+		if (lane_id() == 0) {
+			s[0][w] = rlo;
+			s[1][w] = rhi;
+			s[2][w] = clo;
+			s[3][w] = chi;
+		}
+		__syncthreads();
+		if (threadIdx.x == 0) {
+			for (int k = 1; k < kBlock / 64; ++k) {
+				rlo = min(rlo, s[0][k]);
+				rhi = max(rhi, s[1][k]);
+				clo = min(clo, s[2][k]);
+				chi = max(chi, s[3][k]);
+			}
+			meta[c] = ChunkMeta{rlo, rhi, clo, chi};
+		}
+		__syncthreads();
+	}
+}
+
+static int dmalloc(gphi_ctx *ctx, void **p, size_t bytes, uint64_t *acc)
+{
+	hipError_t e = hipMalloc(p, bytes ? bytes : 4);
+	if (e != hipSuccess) {
+		*p = nullptr;
+		return set_error(ctx, GPHI_E_OOM, "hipMalloc(%zu bytes) failed: %s", bytes,
+		                 hipGetErrorString(e));
+	}
+	if (acc) *acc += bytes;
+	return GPHI_OK;
+}
+
+static unsigned grid_for(uint64_t items, unsigned per_block, unsigned cap)
+{
+	uint64_t g = (items + per_block - 1) / per_block;
+	if (g < 1) g = 1;
+	if (g > cap) g = cap;
+	return (unsigned)g;
+}
+
+int upload_graph(gphi_ctx *ctx, const gphi_graph_desc *d, gphi_graph *gr)
+{
+	DevGraph &G = gr->g;
+	hipStream_t s = ctx->stream;
+	G.n = d->n;
+	G.m = d->m;
+	G.W = d->tile_width;
+	G.S = d->stripe_length;
+	G.side = G.n % G.W ? G.n / G.W + 1 : G.n / G.W;
+	G.ntiles = (uint64_t)G.side * G.side;
+	G.tile_order = d->tile_order;
+	G.n_pad = (uint32_t)(((uint64_t)G.n + 1 + kBlock - 1) / kBlock * kBlock);
+	G.n_words = G.n_pad / 32;  // the update pass writes whole words up to n_pad
+	G.nchunks = (G.m + kChunk - 1) / kChunk;
+	G.m_pad = G.nchunks * kChunk;
+	G.has_csr = d->csr_edges != nullptr;
+	G.has_tiles = d->tile_heads != nullptr && d->tile_tails != nullptr;
+	G.weighted = (d->csr_weights != nullptr || !G.has_csr) &&
+	             (d->tile_weights != nullptr || !G.has_tiles) &&
+	             (d->csr_weights || d->tile_weights);
+	int rc;
+	uint64_t *acc = &G.device_bytes;
+
+	// degrees (padded) and CSR row offsets
+	if ((rc = dmalloc(ctx, (void **)&G.deg, (size_t)G.n_pad * 4, acc))) return rc;
+	GPHI_HIP(ctx, hipMemsetAsync(G.deg, 0, (size_t)G.n_pad * 4, s));
+	GPHI_HIP(ctx, hipMemcpyAsync(G.deg, d->degrees, (size_t)G.n * 4, hipMemcpyHostToDevice, s));
+	if (G.has_csr) {
+		std::vector<uint32_t> off((size_t)G.n + 1);
+		uint64_t a = 0;
+		for (uint32_t v = 0; v < G.n; ++v) {
+			off[v] = (uint32_t)a;
+			a += d->degrees[v];
+		}
+		off[G.n] = (uint32_t)a;
+		if (a != G.m)
+			return set_error(ctx, GPHI_E_FORMAT, "sum of degrees %llu != M %llu",
+			                 (unsigned long long)a, (unsigned long long)G.m);
+		if ((rc = dmalloc(ctx, (void **)&G.row_off, off.size() * 4, acc))) return rc;
+		GPHI_HIP(ctx, hipMemcpy(G.row_off, off.data(), off.size() * 4, hipMemcpyHostToDevice));
+		if ((rc = dmalloc(ctx, (void **)&G.cols, (size_t)G.m * 4, acc))) return rc;
+		GPHI_HIP(ctx, hipMemcpyAsync(G.cols, d->csr_edges, (size_t)G.m * 4,
+		                             hipMemcpyHostToDevice, s));
+		if (d->csr_weights) {
+			if ((rc = dmalloc(ctx, (void **)&G.csr_w, (size_t)G.m * 4, acc))) return rc;
+			GPHI_HIP(ctx, hipMemcpyAsync(G.csr_w, d->csr_weights, (size_t)G.m * 4,
+			                             hipMemcpyHostToDevice, s));
+		}
+	}
+	if (G.has_tiles) {
+		size_t pb = (size_t)G.m_pad * 4;
+		if ((rc = dmalloc(ctx, (void **)&G.heads, pb, acc))) return rc;
+		if ((rc = dmalloc(ctx, (void **)&G.tails, pb, acc))) return rc;
+		GPHI_HIP(ctx, hipMemcpyAsync(G.heads, d->tile_heads, (size_t)G.m * 4,
+		                             hipMemcpyHostToDevice, s));
+		GPHI_HIP(ctx, hipMemcpyAsync(G.tails, d->tile_tails, (size_t)G.m * 4,
+		                             hipMemcpyHostToDevice, s));
+		uint64_t padn = G.m_pad - G.m;
+		if (padn) {
+			fill_u32_kernel<<<grid_for(padn, kBlock, 1024), kBlock, 0, s>>>(G.heads + G.m, padn, G.n);
+			fill_u32_kernel<<<grid_for(padn, kBlock, 1024), kBlock, 0, s>>>(G.tails + G.m, padn, G.n);
+		}
+		if (d->tile_weights) {
+			if ((rc = dmalloc(ctx, (void **)&G.coo_w, pb, acc))) return rc;
+			GPHI_HIP(ctx, hipMemcpyAsync(G.coo_w, d->tile_weights, (size_t)G.m * 4,
+			                             hipMemcpyHostToDevice, s));
+			if (padn)
+				fill_u32_kernel<<<grid_for(padn, kBlock, 1024), kBlock, 0, s>>>(G.coo_w + G.m, padn, 0);
+		}
+		if ((rc = dmalloc(ctx, (void **)&G.meta, (size_t)G.nchunks * sizeof(ChunkMeta) + 16, acc)))
+			return rc;
+		if (G.nchunks)
+			chunk_meta_kernel<<<grid_for(G.nchunks, 1, 65536), kBlock, 0, s>>>(
+			    G.heads, G.tails, G.n, G.W, G.nchunks, G.meta);
+		GPHI_HIP(ctx, hipGetLastError());
+		if (d->tile_offsets) G.h_tile_off.assign(d->tile_offsets, d->tile_offsets + G.ntiles);
+	}
+	GPHI_HIP(ctx, hipStreamSynchronize(s));
+	return GPHI_OK;
+}
+
+void free_graph_device(gphi_graph *gr)
+{
+	DevGraph &G = gr->g;
+	void *ps[] = {G.row_off, G.cols, G.csr_w, G.deg, G.heads, G.tails, G.coo_w, G.meta,
+	              gr->level, gr->parent, gr->dist, gr->flag, gr->visited, gr->cur,
+	              gr->row_stamp, gr->col_unvis, gr->queue, gr->qoff, gr->ctrl, gr->rank,
+	              gr->sum, gr->contrib, gr->tile_lo};
+	for (void *p : ps)
+		if (p) (void)hipFree(p);
+}
+
+// ---------------------------------------------------------------------------
+// traversal state
+// ---------------------------------------------------------------------------
+static int ensure_state(gphi_graph *gr)
+{
+	gphi_ctx *ctx = gr->ctx;
+	DevGraph &G = gr->g;
+	if (gr->level) return GPHI_OK;
+	int rc;
+	size_t nb = (size_t)G.n_pad * 4;
+	if ((rc = dmalloc(ctx, (void **)&gr->level, nb, nullptr))) return rc;
+	if ((rc = dmalloc(ctx, (void **)&gr->parent, nb, nullptr))) return rc;
+	if ((rc = dmalloc(ctx, (void **)&gr->flag, G.n_pad, nullptr))) return rc;
+	if ((rc = dmalloc(ctx, (void **)&gr->visited, (size_t)G.n_words * 4, nullptr))) return rc;
+	if ((rc = dmalloc(ctx, (void **)&gr->cur, (size_t)G.n_words * 4, nullptr))) return rc;
+	if ((rc = dmalloc(ctx, (void **)&gr->row_stamp, ((size_t)G.side + 1) * 4, nullptr))) return rc;
+	if ((rc = dmalloc(ctx, (void **)&gr->col_unvis, ((size_t)G.side + 1) * 4, nullptr))) return rc;
+	if ((rc = dmalloc(ctx, (void **)&gr->queue, nb, nullptr))) return rc;
+	if ((rc = dmalloc(ctx, (void **)&gr->qoff, nb, nullptr))) return rc;
+	if ((rc = dmalloc(ctx, (void **)&gr->tile_lo, (G.m / kSparseTile + 2) * 4, nullptr))) return rc;
+	if ((rc = dmalloc(ctx, (void **)&gr->ctrl, sizeof(StepCtrl), nullptr))) return rc;
+	GPHI_HIP(ctx, hipMemsetAsync(gr->flag, 0, G.n_pad, ctx->stream));
+	GPHI_HIP(ctx, hipMemsetAsync(gr->ctrl, 0, sizeof(StepCtrl), ctx->stream));
+	return GPHI_OK;
+}
+
+// col_unvis[c] = vertices of tail block c; row_stamp[r] = none
+__global__ void init_blocks_kernel(uint32_t n, uint32_t W, uint32_t side, uint32_t *row_stamp,
+                                   uint32_t *col_unvis)
+{
+	uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+	if (c < side) {
+		row_stamp[c] = UINT32_MAX;
+		col_unvis[c] = (c + 1 == side) ? n - c * W : W;
+	}
+}
+
+// Source set-up, one block: level/parent/dist of src, its bits, the first
+// queue (the reference's first step is always sparse, bfs.cpp:795-817).
+template <bool kBfs>
+__global__ void init_source_kernel(uint32_t src, uint32_t W, const uint32_t *__restrict__ deg,
+                                   int32_t *level, int32_t *parent, uint32_t *dist,
+                                   uint32_t *visited, uint32_t *cur, uint32_t *row_stamp,
+                                   uint32_t *col_unvis, uint32_t *queue, uint32_t *qoff,
+                                   uint32_t *tile_lo, StepCtrl *ctrl)
+{
+	uint32_t d = deg[src];
+	uint32_t ntile = (d + kSparseTile - 1) / kSparseTile;
+	for (uint32_t k = threadIdx.x; k < ntile + 1; k += blockDim.x) tile_lo[k] = 0;
+	if (threadIdx.x == 0) {
+		if (kBfs) {
+			level[src] = 0;
+			parent[src] = (int32_t)src;
+			visited[src >> 5] |= 1u << (src & 31);
+			col_unvis[src / W] -= 1;
+		} else {
+			dist[src] = 0;
+		}
+		cur[src >> 5] |= 1u << (src & 31);
+		row_stamp[src / W] = 0;
+		queue[0] = src;
+		qoff[0] = 0;
+		ctrl->fsize = 1;
+		ctrl->fdeg = d;
+		ctrl->qsize = d ? 1 : 0;
+		ctrl->dir = 0;
+		ctrl->done = 0;
+		ctrl->blocks_done = 0;
+		ctrl->facc = 0;
+		ctrl->qpack = 0;
+		ctrl->changed = 0;
+	}
+}
+
+// ---------------------------------------------------------------------------
+// dense step: chunked tile sweep
+// ---------------------------------------------------------------------------
+// Whole-chunk skip (uniform per wave): no head row of the chunk holds a
+// frontier vertex (the reference's is_active_side test, bfs.cpp:275), or every
+// tail column of the chunk is fully visited (BFS only).
+template <bool kColSkip>
+__device__ __forceinline__ bool chunk_live(const ChunkMeta &md, const uint32_t *__restrict__ row_stamp,
+                                           const uint32_t *__restrict__ col_unvis, uint32_t level)
+{
+	if (md.row_lo > md.row_hi) return false;  // sentinel-only chunk
+	uint32_t lane = lane_id();
+	bool any_row;
+	uint32_t nr = md.row_hi - md.row_lo + 1;
+	if (nr > 64) {
+		any_row = true;
+	} else {
+		bool a = lane < nr && row_stamp[md.row_lo + lane] == level;
+		any_row = __ballot(a) != 0;
+	}
+	if (!any_row) return false;
+	if (kColSkip) {
+		uint32_t nc = md.col_hi - md.col_lo + 1;
+		if (nc <= 64) {
+			bool a = lane < nc && col_unvis[md.col_lo + lane] != 0;
+			if (__ballot(a) == 0) return false;
+		}
+	}
+	return true;
+}
+
+// XCD-aware chunk walk: blocks of one XCD (blockIdx % 8) sweep one contiguous
+// eighth of the chunks so that XCD's L2 keeps the bitmap windows of its stripes.
+#define GPHI_FOR_CHUNKS(c, nchunks)                                              \
+	const uint32_t xcd_ = blockIdx.x % kNumXcd, slot_ = blockIdx.x / kNumXcd;   \
+	const uint32_t per_ = gridDim.x / kNumXcd;                                 \
+	const uint64_t cpx_ = ((nchunks) + kNumXcd - 1) / kNumXcd;                 \
+	const uint64_t cend_ = min((uint64_t)(nchunks), (xcd_ + 1) * cpx_);        \
+	for (uint64_t c = xcd_ * cpx_ + slot_; c < cend_; c += per_)
+
+__device__ __forceinline__ void bfs_visit(uint32_t h, uint32_t t, const uint32_t *__restrict__ cur,
+                                          const uint32_t *__restrict__ visited,
+                                          uint8_t *__restrict__ flag, int32_t *__restrict__ parent)
+{
+	if (test_bit(cur, h) && !test_bit(visited, t) && !flag[t]) {
+		parent[t] = (int32_t)h;
+		flag[t] = 1;
+	}
+}
+
+__global__ __launch_bounds__(kBlock) void bfs_dense_kernel(
+    const uint32_t *__restrict__ heads, const uint32_t *__restrict__ tails,
+    const ChunkMeta *__restrict__ meta, uint64_t nchunks, const uint32_t *__restrict__ cur,
+    const uint32_t *__restrict__ visited, const uint32_t *__restrict__ row_stamp,
+    const uint32_t *__restrict__ col_unvis, uint8_t *__restrict__ flag,
+    int32_t *__restrict__ parent, uint32_t level)
+{
+	GPHI_FOR_CHUNKS(c, nchunks)
+	{
+		ChunkMeta md = meta[c];
+		if (!chunk_live<true>(md, row_stamp, col_unvis, level)) continue;
+		const uint4 *H = reinterpret_cast<const uint4 *>(heads + c * kChunk);
+		const uint4 *T = reinterpret_cast<const uint4 *>(tails + c * kChunk);
+		uint4 h0 = H[threadIdx.x], h1 = H[threadIdx.x + kBlock];
+		uint4 t0 = T[threadIdx.x], t1 = T[threadIdx.x + kBlock];
+		bfs_visit(h0.x, t0.x, cur, visited, flag, parent);
+		bfs_visit(h0.y, t0.y, cur, visited, flag, parent);
+		bfs_visit(h0.z, t0.z, cur, visited, flag, parent);
+		bfs_visit(h0.w, t0.w, cur, visited, flag, parent);
+		bfs_visit(h1.x, t1.x, cur, visited, flag, parent);
+		bfs_visit(h1.y, t1.y, cur, visited, flag, parent);
+		bfs_visit(h1.z, t1.z, cur, visited, flag, parent);
+		bfs_visit(h1.w, t1.w, cur, visited, flag, parent);
+	}
+}
+static_assert(kChunk == 2 * 4 * kBlock, "dense sweep loads two uint4 per array per thread");
+
+// SSSP relax (apps/sssp_simd/sssp.cpp:344-445): if mask[h] and dist[h]+w <
+// dist[t] then lower dist[t] and mark t for the next frontier.
+__device__ __forceinline__ void sssp_relax(uint32_t h, uint32_t t, uint32_t w,
+                                           const uint32_t *__restrict__ cur, uint32_t *dist,
+                                           uint8_t *__restrict__ flag)
+{
+	if (!test_bit(cur, h)) return;
+	uint32_t nd = dist[h] + w;
+	if (nd < dist[t]) {
+		uint32_t old = atomicMin(dist + t, nd);
+		if (nd < old && !flag[t]) flag[t] = 1;
+	}
+}
+
+__global__ __launch_bounds__(kBlock) void sssp_dense_kernel(
+    const uint32_t *__restrict__ heads, const uint32_t *__restrict__ tails,
+    const uint32_t *__restrict__ wts, const ChunkMeta *__restrict__ meta, uint64_t nchunks,
+    const uint32_t *__restrict__ cur, const uint32_t *__restrict__ row_stamp, uint32_t *dist,
+    uint8_t *__restrict__ flag, uint32_t level)
+{
+	GPHI_FOR_CHUNKS(c, nchunks)
+	{
+		ChunkMeta md = meta[c];
+		if (!chunk_live<false>(md, row_stamp, nullptr, level)) continue;
+		const uint4 *H = reinterpret_cast<const uint4 *>(heads + c * kChunk);
+		const uint4 *T = reinterpret_cast<const uint4 *>(tails + c * kChunk);
+		const uint4 *Wt = reinterpret_cast<const uint4 *>(wts + c * kChunk);
+		uint4 h0 = H[threadIdx.x], h1 = H[threadIdx.x + kBlock];
+		uint4 t0 = T[threadIdx.x], t1 = T[threadIdx.x + kBlock];
+		uint4 w0 = Wt[threadIdx.x], w1 = Wt[threadIdx.x + kBlock];
+		sssp_relax(h0.x, t0.x, w0.x, cur, dist, flag);
+		sssp_relax(h0.y, t0.y, w0.y, cur, dist, flag);
+		sssp_relax(h0.z, t0.z, w0.z, cur, dist, flag);
+		sssp_relax(h0.w, t0.w, w0.w, cur, dist, flag);
+		sssp_relax(h1.x, t1.x, w1.x, cur, dist, flag);
+		sssp_relax(h1.y, t1.y, w1.y, cur, dist, flag);
+		sssp_relax(h1.z, t1.z, w1.z, cur, dist, flag);
+		sssp_relax(h1.w, t1.w, w1.w, cur, dist, flag);
+	}
+}
+
+// ---------------------------------------------------------------------------
+// sparse step: CSR push, edge-balanced over kSparseTile-edge tiles whose first
+// queue entry (tile_lo) was written by the previous frontier update.
+// ---------------------------------------------------------------------------
+template <bool kBfs>
+__global__ __launch_bounds__(kBlock) void sparse_push_kernel(
+    const uint32_t *__restrict__ queue, const uint32_t *__restrict__ qoff,
+    const uint32_t *__restrict__ tile_lo, uint32_t qsize, uint32_t qdeg,
+    const uint32_t *__restrict__ row_off, const uint32_t *__restrict__ cols,
+    const uint32_t *__restrict__ csr_w, const uint32_t *__restrict__ visited,
+    uint8_t *__restrict__ flag, int32_t *__restrict__ parent, uint32_t *dist)
+{
+	__shared__ uint32_t s_off[kSparseTile + 2];
+	__shared__ uint32_t s_u[kSparseTile + 2];
+	__shared__ uint32_t s_base[kSparseTile + 2];
+	uint32_t ntile = (qdeg + kSparseTile - 1) / kSparseTile;
+	for (uint32_t k = blockIdx.x; k < ntile; k += gridDim.x) {
+		uint32_t lo = tile_lo[k];
+		uint32_t hi = (k + 1 < ntile) ? tile_lo[k + 1] : qsize - 1;
+		if (hi > qsize - 1) hi = qsize - 1;
+		uint32_t cnt = hi - lo + 1;
+		for (uint32_t j = threadIdx.x; j < cnt; j += kBlock) {
+			uint32_t u = queue[lo + j];
+			s_off[j] = qoff[lo + j];
+			s_u[j] = u;
+			s_base[j] = row_off[u];
+		}
+		__syncthreads();
+		uint32_t e0 = k * kSparseTile;
+		uint32_t e1 = min(e0 + kSparseTile, qdeg);
+		for (uint32_t e = e0 + threadIdx.x; e < e1; e += kBlock) {
+			// largest j with s_off[j] <= e
+			uint32_t a = 0, b = cnt;
+			while (b - a > 1) {
+				uint32_t mid = (a + b) >> 1;
+				if (s_off[mid] <= e)
+					a = mid;
+				else
+					b = mid;
+			}
+			uint32_t u = s_u[a];
+			uint32_t idx = s_base[a] + (e - s_off[a]);
+			uint32_t v = cols[idx];
+			if (kBfs) {
+				if (!test_bit(visited, v) && !flag[v]) {
+					parent[v] = (int32_t)u;
+					flag[v] = 1;
+				}
+			} else {
+				uint32_t nd = dist[u] + csr_w[idx];
+				if (nd < dist[v]) {
+					uint32_t old = atomicMin(dist + v, nd);
+					if (nd < old && !flag[v]) flag[v] = 1;
+				}
+			}
+		}
+		__syncthreads();
+	}
+}
+
+// ---------------------------------------------------------------------------
+// frontier update: one lane per vertex.  Turns the flags of the step into the
+// next frontier bitmap, the sparse queue with its degree prefix and tile
+// partition, the row stamps, and the |F| / sum(outdeg) counters; the last
+// block applies the direction rule |F|+sum > M/T_RATIO (bfs.cpp:829-834).
+// ---------------------------------------------------------------------------
+template <bool kBfs>
+__global__ __launch_bounds__(kBlock) void frontier_update_kernel(
+    uint32_t n, uint32_t W, uint32_t threshold, uint32_t next_level,
+    uint8_t *__restrict__ flag, const uint32_t *__restrict__ deg, int32_t *__restrict__ level,
+    uint32_t *__restrict__ visited, uint32_t *__restrict__ cur, uint32_t *__restrict__ row_stamp,
+    uint32_t *__restrict__ col_unvis, uint32_t *__restrict__ queue, uint32_t *__restrict__ qoff,
+    uint32_t *__restrict__ tile_lo, StepCtrl *ctrl)
+{
+	const uint32_t v = blockIdx.x * kBlock + threadIdx.x;  // grid covers n_pad exactly
+	const uint32_t lane = lane_id();
+	const bool f = v < n && flag[v] != 0;
+	const unsigned long long mask = __ballot(f);
+	if (lane == 0 || lane == 32) {
+		uint32_t bits = lane == 0 ? (uint32_t)mask : (uint32_t)(mask >> 32);
+		cur[v >> 5] = bits;
+		if (kBfs && bits) visited[v >> 5] |= bits;
+	}
+	uint32_t d = 0;
+	if (f) {
+		flag[v] = 0;
+		d = deg[v];
+		if (kBfs) level[v] = (int32_t)next_level;
+		row_stamp[v / W] = next_level;
+	}
+	if (kBfs && mask) {
+		uint32_t vfirst = v - lane;
+		uint32_t c0 = vfirst / W, c1 = (vfirst + 63) / W;
+		if (c0 == c1) {
+			if (lane == 0) atomicSub(col_unvis + c0, (uint32_t)__popcll(mask));
+		} else if (f) {
+			atomicSub(col_unvis + v / W, 1u);
+		}
+	}
+	// queue append: frontier vertices with outdeg > 0
+	const bool q = f && d > 0;
+	const unsigned long long qmask = __ballot(q);
+	uint32_t dsum;
+	uint32_t dpre = wave_excl_scan(q ? d : 0u, &dsum);
+	if (qmask) {
+		uint32_t rank = (uint32_t)__popcll(qmask & ((1ull << lane) - 1ull));
+		unsigned long long packed = 0;
+		if (lane == 0) {
+			packed = atomicAdd(&ctrl->qpack, ((unsigned long long)__popcll(qmask) << 32) | dsum);
+			atomicAdd(&ctrl->facc, (uint32_t)__popcll(mask));
+			__threadfence();
+		}
+		packed = __shfl(packed, 0, 64);
+		if (q) {
+			uint32_t pos = (uint32_t)(packed >> 32) + rank;
+			uint32_t off = (uint32_t)packed + dpre;
+			queue[pos] = v;
+			qoff[pos] = off;
+			// tiles of the next push whose first edge falls in [off, off+d)
+			uint32_t k = (off + kSparseTile - 1) / kSparseTile;
+			for (; (uint64_t)k * kSparseTile < (uint64_t)off + d; ++k) tile_lo[k] = pos;
+		}
+	} else if (mask && lane == 0) {
+		atomicAdd(&ctrl->facc, (uint32_t)__popcll(mask));
+		__threadfence();
+	}
+	__shared__ bool s_last;
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		__threadfence();
+		uint32_t prev = atomicAdd(&ctrl->blocks_done, 1u);
+		s_last = (prev == gridDim.x - 1);
+	}
+	__syncthreads();
+	if (s_last && threadIdx.x == 0) {
+		__threadfence();
+		uint32_t fs = atomicAdd(&ctrl->facc, 0u);
+		unsigned long long qp = atomicAdd(&ctrl->qpack, 0ull);
+		uint32_t fd = (uint32_t)qp;
+		ctrl->fsize = fs;
+		ctrl->fdeg = fd;
+		ctrl->qsize = (uint32_t)(qp >> 32);
+		ctrl->dir = (uint32_t)(fs + fd) > threshold ? 1u : 0u;
+		ctrl->done = fs == 0;
+		ctrl->facc = 0;
+		ctrl->qpack = 0;
+		ctrl->blocks_done = 0;
+		__threadfence();
+	}
+}
+
+// ---------------------------------------------------------------------------
+// host drivers
+// ---------------------------------------------------------------------------
+static constexpr unsigned kDenseGrid = 4096;   // multiple of kNumXcd
+static constexpr unsigned kSparseGrid = 4096;
+
+static float ev_ms(hipEvent_t a, hipEvent_t b)
+{
+	float ms = 0.f;
+	if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return 0.f;
+	return ms;
+}
+
+template <bool kBfs>
+static int frontier_loop(gphi_graph *gr, uint32_t src, gphi_stats *st)
+{
+	gphi_ctx *ctx = gr->ctx;
+	DevGraph &G = gr->g;
+	hipStream_t s = ctx->stream;
+	const uint32_t threshold = (uint32_t)G.m / ctx->t_ratio;
+	StepCtrl *hc = ctx->h_ctrl;
+	hipEvent_t *ev = ctx->ev;
+	const bool timing = st != nullptr && ctx->ev_ok;
+	if (timing) GPHI_HIP(ctx, hipEventRecord(ev[0], s));
+
+	// state initialisation (the reference's untimed set-up, bfs.cpp:771-797)
+	size_t nb = (size_t)G.n_pad * 4;
+	if (kBfs) {
+		GPHI_HIP(ctx, hipMemsetAsync(gr->level, 0xFF, nb, s));
+		GPHI_HIP(ctx, hipMemsetAsync(gr->parent, 0xFF, nb, s));
+		GPHI_HIP(ctx, hipMemsetAsync(gr->visited, 0, (size_t)G.n_words * 4, s));
+	} else {
+		fill_u32_kernel<<<grid_for(G.n_pad, kBlock, 4096), kBlock, 0, s>>>(gr->dist, G.n_pad, INT_MAX);
+	}
+	GPHI_HIP(ctx, hipMemsetAsync(gr->cur, 0, (size_t)G.n_words * 4, s));
+	init_blocks_kernel<<<grid_for(G.side, kBlock, 65535), kBlock, 0, s>>>(G.n, G.W, G.side,
+	                                                                      gr->row_stamp, gr->col_unvis);
+	init_source_kernel<kBfs><<<1, kBlock, 0, s>>>(src, G.W, G.deg, gr->level, gr->parent, gr->dist,
+	                                               gr->visited, gr->cur, gr->row_stamp, gr->col_unvis,
+	                                               gr->queue, gr->qoff, gr->tile_lo, gr->ctrl);
+	GPHI_HIP(ctx, hipGetLastError());
+	if (timing) GPHI_HIP(ctx, hipEventRecord(ev[1], s));
+	GPHI_HIP(ctx, hipMemcpyAsync(hc, gr->ctrl, sizeof(StepCtrl), hipMemcpyDeviceToHost, s));
+	GPHI_HIP(ctx, hipStreamSynchronize(s));
+
+	uint32_t steps = 0;
+	uint64_t reached = 0, trav = 0;
+	const unsigned ugrid = G.n_pad / kBlock;
+	while (!hc->done) {
+		const uint32_t L = steps;
+		const uint32_t dir = (steps == 0) ? 0u : hc->dir;  // first step sparse (bfs.cpp:807)
+		if (st && steps < GPHI_MAX_STEPS) {
+			st->dir[steps] = dir;
+			st->fsize[steps] = hc->fsize;
+			st->fdeg[steps] = hc->fdeg;
+		}
+		reached += hc->fsize;
+		trav += hc->fdeg;
+		const bool ev_step = timing && steps < GPHI_MAX_STEPS - 1;
+		if (dir == 0) {
+			if (!G.has_csr) return set_error(ctx, GPHI_E_STATE, "graph has no CSR arrays");
+			if (hc->fdeg)
+				sparse_push_kernel<kBfs><<<grid_for(hc->fdeg, kSparseTile, kSparseGrid), kBlock, 0, s>>>(
+				    gr->queue, gr->qoff, gr->tile_lo, hc->qsize, hc->fdeg, G.row_off, G.cols, G.csr_w,
+				    gr->visited, gr->flag, gr->parent, gr->dist);
+		} else {
+			if (!G.has_tiles) return set_error(ctx, GPHI_E_STATE, "graph has no tile arrays");
+			unsigned grid = (unsigned)std::min<uint64_t>(kDenseGrid, (G.nchunks + kNumXcd - 1) / kNumXcd * kNumXcd);
+			grid = std::max<unsigned>(grid / kNumXcd * kNumXcd, kNumXcd);
+			if (kBfs)
+				bfs_dense_kernel<<<grid, kBlock, 0, s>>>(G.heads, G.tails, G.meta, G.nchunks, gr->cur,
+				                                       gr->visited, gr->row_stamp, gr->col_unvis,
+				                                       gr->flag, gr->parent, L);
+			else
+				sssp_dense_kernel<<<grid, kBlock, 0, s>>>(G.heads, G.tails, G.coo_w, G.meta, G.nchunks,
+				                                        gr->cur, gr->row_stamp, gr->dist, gr->flag, L);
+		}
+		GPHI_HIP(ctx, hipGetLastError());
+		if (ev_step) GPHI_HIP(ctx, hipEventRecord(ev[2 + 2 * steps], s));
+		frontier_update_kernel<kBfs><<<ugrid, kBlock, 0, s>>>(
+		    G.n, G.W, threshold, L + 1, gr->flag, G.deg, gr->level, gr->visited, gr->cur,
+		    gr->row_stamp, gr->col_unvis, gr->queue, gr->qoff, gr->tile_lo, gr->ctrl);
+		GPHI_HIP(ctx, hipGetLastError());
+		if (ev_step) GPHI_HIP(ctx, hipEventRecord(ev[3 + 2 * steps], s));
+		GPHI_HIP(ctx, hipMemcpyAsync(hc, gr->ctrl, sizeof(StepCtrl), hipMemcpyDeviceToHost, s));
+		GPHI_HIP(ctx, hipStreamSynchronize(s));
+		++steps;
+		if (steps > G.n + 2) return set_error(ctx, GPHI_E_STATE, "frontier loop did not terminate");
+	}
+	if (st) {
+		st->steps = std::min<uint32_t>(steps, GPHI_MAX_STEPS);
+		st->reached = reached;
+		st->edges_traversed = trav;
+		if (timing) {
+			st->init_ms = ev_ms(ev[0], ev[1]);
+			hipEvent_t prev = ev[1];
+			uint32_t ns = std::min<uint32_t>(steps, GPHI_MAX_STEPS - 1);
+			for (uint32_t k = 0; k < ns; ++k) {
+				st->sweep_ms[k] = ev_ms(prev, ev[2 + 2 * k]);
+				st->update_ms[k] = ev_ms(ev[2 + 2 * k], ev[3 + 2 * k]);
+				prev = ev[3 + 2 * k];
+			}
+			st->total_ms = ns ? ev_ms(ev[0], ev[1 + 2 * ns]) : st->init_ms;
+		}
+	}
+	return GPHI_OK;
+}
+
+int run_bfs(gphi_graph *gr, uint32_t src, int32_t *levels, int32_t *parents, gphi_stats *st)
+{
+	gphi_ctx *ctx = gr->ctx;
+	DevGraph &G = gr->g;
+	if (src >= G.n) return set_error(ctx, GPHI_E_RANGE, "source %u >= N %u", src, G.n);
+	int rc = ensure_state(gr);
+	if (rc) return rc;
+	if (st) memset(st, 0, sizeof(*st));
+	if ((rc = frontier_loop<true>(gr, src, st))) return rc;
+	hipStream_t s = ctx->stream;
+	if (levels) GPHI_HIP(ctx, hipMemcpyAsync(levels, gr->level, (size_t)G.n * 4, hipMemcpyDeviceToHost, s));
+	if (parents) GPHI_HIP(ctx, hipMemcpyAsync(parents, gr->parent, (size_t)G.n * 4, hipMemcpyDeviceToHost, s));
+	GPHI_HIP(ctx, hipStreamSynchronize(s));
+	return GPHI_OK;
+}
+
+int run_sssp(gphi_graph *gr, uint32_t src, uint32_t *dist, gphi_stats *st)
+{
+	gphi_ctx *ctx = gr->ctx;
+	DevGraph &G = gr->g;
+	if (src >= G.n) return set_error(ctx, GPHI_E_RANGE, "source %u >= N %u", src, G.n);
+	if (!G.weighted) return set_error(ctx, GPHI_E_STATE, "graph carries no weights");
+	int rc = ensure_state(gr);
+	if (rc) return rc;
+	if (!gr->dist && (rc = dmalloc(ctx, (void **)&gr->dist, (size_t)G.n_pad * 4, nullptr))) return rc;
+	if (st) memset(st, 0, sizeof(*st));
+	if ((rc = frontier_loop<false>(gr, src, st))) return rc;
+	if (dist)
+		GPHI_HIP(ctx, hipMemcpyAsync(dist, gr->dist, (size_t)G.n * 4, hipMemcpyDeviceToHost, ctx->stream));
+	GPHI_HIP(ctx, hipStreamSynchronize(ctx->stream));
+	if (st) {
+		// reached = vertices with a finite distance (a vertex may be expanded
+		// several times by a label-correcting sweep, so the trace overcounts)
+		st->reached = 0;
+		st->edges_traversed = 0;
+	}
+	return GPHI_OK;
+}
+
+// ---------------------------------------------------------------------------
+// PageRank (FP32): contrib = rank/deg; sum[t] += contrib[h] over the tiles;
+// rank = (1-d)/N + d*sum   (apps/pageRank_simd/page_rank.cpp:128-199,428-446)
+// ---------------------------------------------------------------------------
+__global__ void pr_prepare_kernel(uint32_t n, const float *__restrict__ rank,
+                                  const uint32_t *__restrict__ deg, float *__restrict__ contrib,
+                                  float *__restrict__ sum)
+{
+	uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+	if (v < n) {
+		uint32_t d = deg[v];
+		contrib[v] = d ? rank[v] / (float)d : 0.f;
+		sum[v] = 0.f;
+	}
+}
+
+__device__ __forceinline__ void pr_edge(uint32_t h, uint32_t t, uint32_t n,
+                                        const float *__restrict__ contrib, float *sum)
+{
+	if (h < n) atomicAdd(sum + t, contrib[h]);
+}
+
+__global__ __launch_bounds__(kBlock) void pr_sweep_kernel(const uint32_t *__restrict__ heads,
+                                                          const uint32_t *__restrict__ tails,
+                                                          uint64_t nchunks, uint32_t n,
+                                                          const float *__restrict__ contrib,
+                                                          float *sum)
+{
+	GPHI_FOR_CHUNKS(c, nchunks)
+	{
+		const uint4 *H = reinterpret_cast<const uint4 *>(heads + c * kChunk);
+		const uint4 *T = reinterpret_cast<const uint4 *>(tails + c * kChunk);
+		uint4 h0 = H[threadIdx.x], h1 = H[threadIdx.x + kBlock];
+		uint4 t0 = T[threadIdx.x], t1 = T[threadIdx.x + kBlock];
+		pr_edge(h0.x, t0.x, n, contrib, sum);
+		pr_edge(h0.y, t0.y, n, contrib, sum);
+		pr_edge(h0.z, t0.z, n, contrib, sum);
+		pr_edge(h0.w, t0.w, n, contrib, sum);
+		pr_edge(h1.x, t1.x, n, contrib, sum);
+		pr_edge(h1.y, t1.y, n, contrib, sum);
+		pr_edge(h1.z, t1.z, n, contrib, sum);
+		pr_edge(h1.w, t1.w, n, contrib, sum);
+	}
+}
+
+__global__ void pr_apply_kernel(uint32_t n, float first, float damping, const float *__restrict__ sum,
+                                float *__restrict__ rank)
+{
+	uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+	if (v < n) rank[v] = first + damping * sum[v];
+}
+
+int run_pagerank(gphi_graph *gr, uint32_t iters, float damping, float init, float *rank, float *ms)
+{
+	gphi_ctx *ctx = gr->ctx;
+	DevGraph &G = gr->g;
+	if (!G.has_tiles) return set_error(ctx, GPHI_E_STATE, "graph has no tile arrays");
+	hipStream_t s = ctx->stream;
+	int rc;
+	if (!gr->rank) {
+		if ((rc = dmalloc(ctx, (void **)&gr->rank, (size_t)G.n_pad * 4, nullptr))) return rc;
+		if ((rc = dmalloc(ctx, (void **)&gr->sum, (size_t)G.n_pad * 4, nullptr))) return rc;
+		if ((rc = dmalloc(ctx, (void **)&gr->contrib, (size_t)G.n_pad * 4, nullptr))) return rc;
+	}
+	uint32_t bits;
+	memcpy(&bits, &init, 4);
+	fill_u32_kernel<<<grid_for(G.n, kBlock, 4096), kBlock, 0, s>>>((uint32_t *)gr->rank, G.n, bits);
+	const float first = (1.0f - damping) / (float)G.n;
+	unsigned vgrid = grid_for(G.n, kBlock, 1u << 30);
+	unsigned grid = (unsigned)std::min<uint64_t>(kDenseGrid, (G.nchunks + kNumXcd - 1) / kNumXcd * kNumXcd);
+	grid = std::max<unsigned>(grid / kNumXcd * kNumXcd, kNumXcd);
+	if (ctx->ev_ok) GPHI_HIP(ctx, hipEventRecord(ctx->ev[0], s));
+	for (uint32_t it = 0; it < iters; ++it) {
+		pr_prepare_kernel<<<vgrid, kBlock, 0, s>>>(G.n, gr->rank, G.deg, gr->contrib, gr->sum);
+		pr_sweep_kernel<<<grid, kBlock, 0, s>>>(G.heads, G.tails, G.nchunks, G.n, gr->contrib, gr->sum);
+		pr_apply_kernel<<<vgrid, kBlock, 0, s>>>(G.n, first, damping, gr->sum, gr->rank);
+	}
+	GPHI_HIP(ctx, hipGetLastError());
+	if (ctx->ev_ok) GPHI_HIP(ctx, hipEventRecord(ctx->ev[1], s));
+	if (rank) GPHI_HIP(ctx, hipMemcpyAsync(rank, gr->rank, (size_t)G.n * 4, hipMemcpyDeviceToHost, s));
+	GPHI_HIP(ctx, hipStreamSynchronize(s));
+	if (ms) *ms = ctx->ev_ok ? ev_ms(ctx->ev[0], ctx->ev[1]) : 0.f;
+	return GPHI_OK;
+}
+
+// ---------------------------------------------------------------------------
+// hop relaxation (apps/bellman-ford_simd/bellman-ford.cpp:129-191): full
+// sweeps of dist[t] = min(dist[t], dist[h]+1); a sweep that lowers nothing is
+// the fixpoint, where further sweeps would change nothing.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void hop_sweep_kernel(const uint32_t *__restrict__ heads,
+                                                           const uint32_t *__restrict__ tails,
+                                                           uint64_t nchunks, uint32_t n, uint32_t *dist,
+                                                           uint32_t *changed)
+{
+	uint32_t any = 0;
+	GPHI_FOR_CHUNKS(c, nchunks)
+	{
+		const uint4 *H = reinterpret_cast<const uint4 *>(heads + c * kChunk);
+		const uint4 *T = reinterpret_cast<const uint4 *>(tails + c * kChunk);
+		uint4 hv[2] = {H[threadIdx.x], H[threadIdx.x + kBlock]};
+		uint4 tv[2] = {T[threadIdx.x], T[threadIdx.x + kBlock]};
+#pragma unroll
+		for (int j = 0; j < 2; ++j) {
+			uint32_t hh[4] = {hv[j].x, hv[j].y, hv[j].z, hv[j].w};
+			uint32_t tt[4] = {tv[j].x, tv[j].y, tv[j].z, tv[j].w};
+#pragma unroll
+			for (int k = 0; k < 4; ++k) {
+				if (hh[k] >= n) continue;
+				uint32_t nd = dist[hh[k]] + 1;
+				if ((int32_t)nd < (int32_t)dist[tt[k]]) {
+					uint32_t old = atomicMin(dist + tt[k], nd);
+					if (nd < old) any = 1;
+				}
+			}
+		}
+	}
+	if (__ballot(any) && lane_id() == 0) atomicAdd(changed, 1u);
+}
+
+int run_bellman_ford(gphi_graph *gr, uint32_t src, uint32_t max_sweeps, uint32_t *dist,
+                     uint32_t *sweeps_run, float *ms)
+{
+	gphi_ctx *ctx = gr->ctx;
+	DevGraph &G = gr->g;
+	if (src >= G.n) return set_error(ctx, GPHI_E_RANGE, "source %u >= N %u", src, G.n);
+	if (!G.has_tiles) return set_error(ctx, GPHI_E_STATE, "graph has no tile arrays");
+	hipStream_t s = ctx->stream;
+	int rc;
+	if (!gr->dist && (rc = dmalloc(ctx, (void **)&gr->dist, (size_t)G.n_pad * 4, nullptr))) return rc;
+	if (!gr->ctrl && (rc = dmalloc(ctx, (void **)&gr->ctrl, sizeof(StepCtrl), nullptr))) return rc;
+	const uint32_t batch = 4;
+	uint32_t *counts = nullptr;
+	if ((rc = dmalloc(ctx, (void **)&counts, (max_sweeps + batch) * 4, nullptr))) return rc;
+	if (ctx->ev_ok) GPHI_HIP(ctx, hipEventRecord(ctx->ev[0], s));
+	fill_u32_kernel<<<grid_for(G.n_pad, kBlock, 4096), kBlock, 0, s>>>(gr->dist, G.n_pad, INT_MAX - 2);
+	uint32_t zero = 0;
+	GPHI_HIP(ctx, hipMemcpyAsync(gr->dist + src, &zero, 4, hipMemcpyHostToDevice, s));
+	GPHI_HIP(ctx, hipMemsetAsync(counts, 0, (max_sweeps + batch) * 4, s));
+	unsigned grid = (unsigned)std::min<uint64_t>(kDenseGrid, (G.nchunks + kNumXcd - 1) / kNumXcd * kNumXcd);
+	grid = std::max<unsigned>(grid / kNumXcd * kNumXcd, kNumXcd);
+	uint32_t done = 0, run = 0;
+	std::vector<uint32_t> hcnt(batch);
+	while (run < max_sweeps && !done) {
+		uint32_t nb = std::min(batch, max_sweeps - run);
+		for (uint32_t k = 0; k < nb; ++k)
+			hop_sweep_kernel<<<grid, kBlock, 0, s>>>(G.heads, G.tails, G.nchunks, G.n, gr->dist,
+			                                        counts + run + k);
+		GPHI_HIP(ctx, hipGetLastError());
+		GPHI_HIP(ctx, hipMemcpyAsync(hcnt.data(), counts + run, nb * 4, hipMemcpyDeviceToHost, s));
+		GPHI_HIP(ctx, hipStreamSynchronize(s));
+		for (uint32_t k = 0; k < nb; ++k) {
+			++run;
+			if (hcnt[k] == 0) {
+				done = 1;
+				break;
+			}
+		}
+	}
+	if (ctx->ev_ok) GPHI_HIP(ctx, hipEventRecord(ctx->ev[1], s));
+	if (dist) GPHI_HIP(ctx, hipMemcpyAsync(dist, gr->dist, (size_t)G.n * 4, hipMemcpyDeviceToHost, s));
+	GPHI_HIP(ctx, hipStreamSynchronize(s));
+	(void)hipFree(counts);
+	if (ms) *ms = ctx->ev_ok ? ev_ms(ctx->ev[0], ctx->ev[1]) : 0.f;
+	if (sweeps_run) *sweeps_run = run;
+	return GPHI_OK;
+}
+
+}  // namespace gphi

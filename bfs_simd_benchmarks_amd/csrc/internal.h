@@ -1,0 +1,127 @@
+// internal.h -- shared declarations of libgphi (host C++ and HIP translation units).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/gphi.h"
+
+namespace gphi {
+
+// Edges of one sweep chunk.  The tiled COO arrays are padded to a multiple of
+// this with sentinel edges (head = tail = n) whose frontier bit is never set.
+constexpr uint32_t kChunk = 2048;
+constexpr uint32_t kBlock = 256;
+constexpr uint32_t kSparseTile = 1024;  // edges per block iteration of the CSR push
+constexpr int kNumXcd = 8;              // MI355X: 8 XCDs, blocks dispatched round-robin
+
+struct Error {
+	int code;
+	std::string msg;
+};
+
+// Per-chunk summary used to skip whole chunks of the tile sweep: the range of
+// head tile-rows and tail tile-columns its (non-sentinel) edges touch.
+struct ChunkMeta {
+	uint32_t row_lo, row_hi, col_lo, col_hi;
+};
+
+// Control block of one frontier traversal, read back by the host after each
+// step (64 bytes).
+struct StepCtrl {
+	uint32_t fsize;         // |F| of the frontier just produced
+	uint32_t fdeg;          // sum of out-degrees of that frontier
+	uint32_t qsize;         // queue entries (frontier vertices with outdeg > 0)
+	uint32_t dir;           // direction chosen for expanding it
+	uint32_t done;          // frontier empty
+	uint32_t blocks_done;   // last-block detection
+	uint32_t facc;          // accumulator: |F|
+	uint32_t pad0;
+	unsigned long long qpack;  // accumulator: (queue count << 32) | degree sum
+	uint32_t changed;       // hop relaxation: edges that lowered a distance
+	uint32_t pad[5];
+};
+
+struct DevGraph {
+	uint32_t n = 0;
+	uint64_t m = 0;
+	uint32_t W = 0, S = 0, side = 0;
+	uint64_t ntiles = 0;
+	int tile_order = GPHI_TILES_COL_STRIPE;
+	uint32_t n_words = 0;      // 32-bit words of a vertex bitmap (covers id n)
+	uint32_t n_pad = 0;        // padded vertex count (multiple of kBlock, > n)
+	uint64_t m_pad = 0;        // tile arrays padded to whole chunks
+	uint64_t nchunks = 0;
+	bool has_csr = false, has_tiles = false, weighted = false;
+	// CSR (the _untiled files)
+	uint32_t *row_off = nullptr;  // n+1
+	uint32_t *cols = nullptr;     // m
+	uint32_t *csr_w = nullptr;    // m or null
+	uint32_t *deg = nullptr;      // n_pad
+	// tiled COO (padded)
+	uint32_t *heads = nullptr, *tails = nullptr, *coo_w = nullptr;
+	ChunkMeta *meta = nullptr;
+	// host copy of tile sizes for the byte model (SIDE^2 entries, may be empty)
+	std::vector<uint32_t> h_tile_off;
+	uint64_t device_bytes = 0;
+};
+
+}  // namespace gphi
+
+struct gphi_ctx {
+	int device = 0;
+	uint32_t t_ratio = 20;
+	hipStream_t stream = nullptr;
+	std::mutex mu;
+	gphi::Error err{0, ""};
+	std::string last;
+	// pinned host control block for per-step readback
+	gphi::StepCtrl *h_ctrl = nullptr;
+	hipEvent_t ev[2 * GPHI_MAX_STEPS + 4];
+	bool ev_ok = false;
+};
+
+struct gphi_graph {
+	gphi_ctx *ctx = nullptr;
+	gphi::DevGraph g;
+	// per-graph traversal state (allocated lazily, reused across calls)
+	int32_t *level = nullptr;  // levels (BFS) ; dist (SSSP/BF) aliases via uint32
+	int32_t *parent = nullptr;
+	uint32_t *dist = nullptr;
+	uint8_t *flag = nullptr;
+	uint32_t *visited = nullptr, *cur = nullptr;
+	uint32_t *row_stamp = nullptr, *col_unvis = nullptr;
+	uint32_t *queue = nullptr, *qoff = nullptr, *tile_lo = nullptr;
+	gphi::StepCtrl *ctrl = nullptr;
+	float *rank = nullptr, *sum = nullptr, *contrib = nullptr;
+};
+
+namespace gphi {
+int set_error(gphi_ctx *ctx, int code, const char *fmt, ...);
+int hip_fail(gphi_ctx *ctx, hipError_t e, const char *what, const char *file, int line);
+
+// device-side entry points (device.hip)
+int upload_graph(gphi_ctx *ctx, const gphi_graph_desc *d, gphi_graph *gr);
+void free_graph_device(gphi_graph *gr);
+int run_bfs(gphi_graph *gr, uint32_t src, int32_t *levels, int32_t *parents, gphi_stats *st);
+int run_sssp(gphi_graph *gr, uint32_t src, uint32_t *dist, gphi_stats *st);
+int run_pagerank(gphi_graph *gr, uint32_t iters, float damping, float init, float *rank,
+                 float *ms);
+int run_bellman_ford(gphi_graph *gr, uint32_t src, uint32_t max_sweeps, uint32_t *dist,
+                     uint32_t *sweeps_run, float *ms);
+
+// host text formats (formats.cpp)
+int read_text_graph(gphi_ctx *ctx, const char *prefix, uint32_t W, uint32_t S, int weighted,
+                    bool row_major, gphi_graph_desc *d, std::vector<uint32_t> *store);
+}  // namespace gphi
+
+#define GPHI_HIP(ctx, call)                                                    \
+	do {                                                                       \
+		hipError_t e_ = (call);                                                \
+		if (e_ != hipSuccess) return gphi::hip_fail((ctx), e_, #call, __FILE__, __LINE__); \
+	} while (0)

@@ -1,0 +1,133 @@
+/*
+ * gphi.h -- C ABI of libgphi, the MI355X-native tiled graph-traversal library.
+ *
+ * The reference (johnpzh/bfs_simd_benchmarks) has no library API: its boundary
+ * is the process CLI, the on-disk 64-way text formats and the C-style
+ * signatures inside each app's single translation unit, with sizes passed
+ * through globals.  Each entry point below names the reference function whose
+ * role it takes over (paths relative to the reference repository root).
+ *
+ * Conventions
+ *   - every function returns int: GPHI_OK (0) or a negative GPHI_E_* code;
+ *     gphi_last_error() gives the text.  The library never calls exit().
+ *   - all vertex ids are 0-based uint32 (the files are 1-based, as the
+ *     reference's readers subtract one: apps/bfs_simd/bfs.cpp:1049-1050).
+ *   - a gphi_ctx owns one HIP device and one HIP stream; calls on different
+ *     contexts may run concurrently; calls on one context are serialised.
+ *   - device memory is owned by the library; host buffers are the caller's.
+ */
+#ifndef GPHI_H
+#define GPHI_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GPHI_OK 0
+#define GPHI_E_IO -1      /* missing or unreadable file */
+#define GPHI_E_FORMAT -2  /* truncated / malformed file, header mismatch */
+#define GPHI_E_RANGE -3   /* id or size out of range (e.g. S > SIDE, M >= 2^32) */
+#define GPHI_E_OOM -4     /* host or device allocation failed */
+#define GPHI_E_HIP -5     /* HIP runtime error */
+#define GPHI_E_ARG -6     /* invalid argument */
+#define GPHI_E_STATE -7   /* wrong object state (e.g. graph lacks weights) */
+
+typedef struct gphi_ctx gphi_ctx;
+typedef struct gphi_graph gphi_graph;
+
+typedef struct gphi_opts {
+	int device;          /* HIP device ordinal */
+	uint32_t t_ratio;    /* direction rule |F|+sum(outdeg) > M/t_ratio; 0 -> 20
+	                        (apps/bfs_simd/bfs.cpp:1171) */
+} gphi_opts;
+
+/* Tile order of the COO arrays. */
+#define GPHI_TILES_COL_STRIPE 0 /* _col-S-coo-tiled-W (tools/column_major_tile/kcore.cpp:378-415) */
+#define GPHI_TILES_ROW_MAJOR 1  /* _coo-tiled-W (tools/coo_tiling/page_rank.cpp:430) */
+
+/* Host arrays of one loaded graph: exactly what graph_input hands to
+ * graph_prepare (apps/bfs_simd/bfs.cpp:968-1118, :753-761), without globals.
+ * Any of csr_edges / tile_heads+tile_tails may be NULL when the caller only
+ * needs the other half (e.g. page_rank reads row-major tiles only). */
+typedef struct gphi_graph_desc {
+	uint32_t n;                   /* NNODES */
+	uint64_t m;                   /* NEDGES (< 2^32, as the formats are %u) */
+	uint32_t tile_width;          /* W */
+	uint32_t stripe_length;       /* S */
+	int tile_order;               /* GPHI_TILES_* */
+	const uint32_t *degrees;      /* n out-degrees (the -nneibor file) */
+	const uint32_t *csr_edges;    /* m tails in _untiled order (graph_edges) */
+	const uint32_t *csr_weights;  /* m weights in _untiled order, or NULL */
+	const uint32_t *tile_heads;   /* m heads in tile order (graph_heads) */
+	const uint32_t *tile_tails;   /* m tails in tile order (graph_tails) */
+	const uint32_t *tile_weights; /* m weights in tile order, or NULL */
+	const uint32_t *tile_offsets; /* SIDE^2 tile offsets (tile_offsets), or NULL */
+} gphi_graph_desc;
+
+/* Per-step statistics of one traversal.  A step expands one frontier. */
+#define GPHI_MAX_STEPS 512
+typedef struct gphi_stats {
+	uint32_t steps;                      /* frontiers expanded */
+	uint32_t dir[GPHI_MAX_STEPS];        /* 0 = sparse (top-down CSR), 1 = dense (tiles) */
+	uint32_t fsize[GPHI_MAX_STEPS];      /* |F| expanded by the step */
+	uint32_t fdeg[GPHI_MAX_STEPS];       /* sum of out-degrees of F */
+	float sweep_ms[GPHI_MAX_STEPS];      /* sweep kernel time (HIP events) */
+	float update_ms[GPHI_MAX_STEPS];     /* frontier update kernel time */
+	float init_ms;                       /* state initialisation (device) */
+	float total_ms;                      /* init + all steps (device events) */
+	uint64_t reached;                    /* vertices with a finite result */
+	uint64_t edges_traversed;            /* sum of outdeg over reached vertices */
+} gphi_stats;
+
+/* ---- context: replaces the globals NNODES..T_RATIO (apps/bfs_simd/bfs.cpp:17-26) ---- */
+int gphi_open(const gphi_opts *opts, gphi_ctx **out);
+void gphi_close(gphi_ctx *ctx);
+const char *gphi_last_error(gphi_ctx *ctx);
+const char *gphi_version(void);
+
+/* ---- graphs ---- */
+/* Upload host arrays (graph_input's outputs).  apps/bfs_simd/bfs.cpp:968-1118 */
+int gphi_graph_from_desc(gphi_ctx *ctx, const gphi_graph_desc *desc, gphi_graph **out);
+/* Read the reference's 64-way text files and upload:
+ *   prefix_col-S-coo-tiled-W-{0..63},-offsets  and  prefix_untiled-{0..63},-nneibor
+ * (apps/bfs_simd/bfs.cpp:983-1113; weighted: apps/sssp_simd/sssp.cpp:41-189).
+ * `prefix` is the full name before the format suffix, e.g. "rmat16_reorder". */
+int gphi_graph_load_text(gphi_ctx *ctx, const char *prefix, uint32_t W, uint32_t S,
+                         int weighted, gphi_graph **out);
+/* Read prefix_coo-tiled-W-{0..63},-offsets,-nneibor (row-major tiles only), as
+ * apps/pageRank_simd/page_rank.cpp:468-568 and apps/bellman-ford_simd do. */
+int gphi_graph_load_text_rowmajor(gphi_ctx *ctx, const char *prefix, uint32_t W,
+                                  uint32_t S, gphi_graph **out);
+int gphi_graph_info(const gphi_graph *g, uint32_t *n, uint64_t *m, uint32_t *W, uint32_t *S,
+                    int *weighted);
+uint64_t gphi_graph_device_bytes(const gphi_graph *g);
+void gphi_graph_free(gphi_graph *g);
+
+/* ---- traversals (device-resident graph, results copied to host buffers) ---- */
+/* Hybrid BFS: graph_prepare, apps/bfs_simd/bfs.cpp:753-963.
+ * levels[n]: hop level (-1 unreached, the h_cost array); parents[n]: BFS-tree
+ * parent (-1 unreached, parent[src]=src).  Either output may be NULL. */
+int gphi_bfs(gphi_graph *g, uint32_t src, int32_t *levels, int32_t *parents, gphi_stats *st);
+/* Weighted hybrid SSSP: sssp_weighted, apps/sssp_simd/sssp.cpp:969-1112.
+ * dist[n]: shortest distance, 2147483647 when unreached. */
+int gphi_sssp(gphi_graph *g, uint32_t src, uint32_t *dist, gphi_stats *st);
+/* PageRank sweeps: page_rank, apps/pageRank_simd/page_rank.cpp:362-456, one
+ * sweep per iteration; rank starts at `init` everywhere, sum is cleared per
+ * iteration; rank = (1-damping)/N + damping*sum in FP32. */
+int gphi_pagerank(gphi_graph *g, uint32_t iters, float damping, float init, float *rank,
+                  float *ms);
+/* Hop relaxation: apps/bellman-ford_simd/bellman-ford.cpp:291-392 (init
+ * INT_MAX-2, dist[src]=0, at most max_sweeps sweeps, stops at the fixpoint). */
+int gphi_bellman_ford(gphi_graph *g, uint32_t src, uint32_t max_sweeps, uint32_t *dist,
+                      uint32_t *sweeps_run, float *ms);
+
+/* Algorithmic HBM bytes of a finished BFS (per step, the byte model of the
+ * design doc) from its levels and stats; bytes_per_step has st->steps entries. */
+int gphi_bfs_alg_bytes(const gphi_graph *g, const int32_t *levels, const gphi_stats *st,
+                       double *bytes_per_step);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,139 @@
+"""GPU parity of the hot path against the oracle (bit-exact levels/distances,
+valid parents, identical direction trace; PageRank within the stated tolerance).
+
+Every call goes through the C ABI of libgphi.so (HIP kernels for gfx950)."""
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def upload(ctx, g, weighted=False, csr=True, tiles=True):
+    return ctx.graph_from_arrays(
+        g.n, g.deg, W=g.W, S=g.S,
+        csr_edges=g.cols if csr else None,
+        csr_weights=g.cw if (weighted and csr) else None,
+        tile_heads=g.sh if tiles else None, tile_tails=g.st if tiles else None,
+        tile_weights=g.sw if (weighted and tiles) else None,
+        tile_offsets=g.soff if tiles else None)
+
+
+def check_bfs(ctx, g, src, t_ratio=20):
+    gg = upload(ctx, g)
+    try:
+        lv, pa, st = gg.bfs(src)
+    finally:
+        gg.free()
+    ref = O.bfs_levels(g.n, g.deg, g.cols, src)
+    assert np.array_equal(lv, ref), f"levels differ at {np.flatnonzero(lv != ref)[:10]}"
+    bad = O.check_parents(g.n, g.deg, g.cols, src, lv, pa)
+    assert bad == 0, f"invalid parent at vertex {bad - 1}"
+    _, _, tr = O.bfs_hybrid(g, src, t_ratio)
+    assert np.array_equal(st.trace(), tr), (st.trace(), tr)
+    assert st.reached == int((ref >= 0).sum())
+    assert st.edges_traversed == int(g.deg[ref >= 0].astype(np.uint64).sum())
+    return lv, st
+
+
+def test_bfs_c1_source0(ctx, c1):
+    print("case: bfs c1 src=0", flush=True)
+    check_bfs(ctx, c1, 0)
+
+
+@pytest.mark.parametrize("src", [1, 7, 100, 4097, 65535])
+def test_bfs_c1_sources(ctx, c1, src):
+    print(f"case: bfs c1 src={src}", flush=True)
+    check_bfs(ctx, c1, src)
+
+
+@pytest.mark.parametrize("W,S", [(256, 4), (64, 1), (4096, 16), (1000, 3), (65536, 1)])
+def test_bfs_c1_tilings(ctx, W, S):
+    print(f"case: bfs c1 W={W} S={S}", flush=True)
+    n, h, t = O.config_edges("c1")
+    g = O.build_from_edgelist(n, h, t, W, S)
+    check_bfs(ctx, g, 0)
+
+
+def test_bfs_tiny_graphs(ctx):
+    # path 0->1->2->3, an isolated vertex 4, a back edge 3->0 and a duplicate
+    for n, edges in [(5, [(0, 1), (1, 2), (2, 3), (3, 0), (1, 2)]),
+                     (1, []),
+                     (3, [(2, 1)]),
+                     (130, [(i, i + 1) for i in range(129)])]:
+        h = np.array([e[0] for e in edges], np.uint32)
+        t = np.array([e[1] for e in edges], np.uint32)
+        g = O.tile_edgelist(n, h, t, 64, 2) if n >= 64 else O.tile_edgelist(n, h, t, 2, 1)
+        for src in range(min(n, 4)):
+            print(f"case: tiny n={n} src={src}", flush=True)
+            check_bfs(ctx, g, src)
+
+
+def test_bfs_source_without_out_edges(ctx, c1):
+    src = int(np.flatnonzero(c1.deg == 0)[0])
+    print(f"case: bfs c1 isolated src={src}", flush=True)
+    lv, st = check_bfs(ctx, c1, src)
+    assert (lv >= 0).sum() == 1
+
+
+def test_bfs_repeated_calls_reuse_state(ctx, c1):
+    gg = upload(ctx, c1)
+    try:
+        for src in (0, 5, 0):
+            lv, pa, _ = gg.bfs(src)
+            assert np.array_equal(lv, O.bfs_levels(c1.n, c1.deg, c1.cols, src))
+    finally:
+        gg.free()
+
+
+def test_bfs_bad_source_raises(ctx, c1):
+    import bfs_simd_benchmarks_amd as G
+    gg = upload(ctx, c1)
+    try:
+        with pytest.raises(G.GphiError) as ei:
+            gg.bfs(c1.n)
+        assert ei.value.code == -3
+    finally:
+        gg.free()
+
+
+@pytest.mark.parametrize("src", [0, 3, 999, 20000])
+def test_sssp_c3(ctx, c3, src):
+    print(f"case: sssp c3 src={src}", flush=True)
+    gg = upload(ctx, c3, weighted=True)
+    try:
+        d, st = gg.sssp(src)
+    finally:
+        gg.free()
+    ref = O.sssp_dijkstra(c3.n, c3.deg, c3.cols, c3.cw, src)
+    assert np.array_equal(d, ref), f"dist differ at {np.flatnonzero(d != ref)[:10]}"
+
+
+def test_pagerank_c1_vs_f64(ctx, c1):
+    print("case: pagerank c1", flush=True)
+    gg = ctx.graph_from_arrays(c1.n, c1.deg, W=c1.W, S=c1.S, tile_heads=c1.rh, tile_tails=c1.rt,
+                               tile_offsets=c1.roff, tile_order=1)
+    try:
+        r, ms = gg.pagerank(20, 0.85, 1.0 / c1.n)
+    finally:
+        gg.free()
+    ref = O.pagerank_f64(c1.n, c1.rh, c1.rt, c1.deg, 20, 0.85, 1.0 / c1.n)
+    l1 = float(np.abs(r.astype(np.float64) - ref).sum())
+    print(f"pagerank L1 vs fp64 = {l1:.3e}")
+    assert l1 < 1e-6
+
+
+def test_bellman_ford_c1(ctx, c1):
+    print("case: bellman-ford c1", flush=True)
+    gg = ctx.graph_from_arrays(c1.n, c1.deg, W=c1.W, S=c1.S, tile_heads=c1.rh, tile_tails=c1.rt,
+                               tile_offsets=c1.roff, tile_order=1)
+    try:
+        d, run, ms = gg.bellman_ford(0, 99)
+    finally:
+        gg.free()
+    ref = O.bellman_ford_hops(c1.n, c1.rh, c1.rt, 0, 99)
+    assert np.array_equal(d, ref)
+    lv = O.bfs_levels(c1.n, c1.deg, c1.cols, 0)
+    assert np.array_equal(d, np.where(lv >= 0, lv, O.INT_MAX - 2).astype(np.uint32))
+    assert run < 99
