@@ -53,6 +53,20 @@ class GraphDesc(C.Structure):
                 ("tile_weights", C.c_void_p), ("tile_offsets", C.c_void_p)]
 
 
+class RmatSpec(C.Structure):
+    _fields_ = [("scale", C.c_uint32), ("nverts", C.c_uint32), ("seed", C.c_uint64),
+                ("samples", C.c_uint64), ("target_unique", C.c_uint64), ("a", C.c_double),
+                ("b", C.c_double), ("c", C.c_double), ("perm_seed", C.c_uint64)]
+
+
+class WeightSpec(C.Structure):
+    _fields_ = [("seed", C.c_uint64), ("lo", C.c_uint32), ("hi", C.c_uint32)]
+
+
+REORDER_NONE = 0
+REORDER_DEGREE = 1
+
+
 class Stats(C.Structure):
     _fields_ = [("steps", C.c_uint32), ("dir", C.c_uint32 * MAX_STEPS),
                 ("fsize", C.c_uint32 * MAX_STEPS), ("fdeg", C.c_uint32 * MAX_STEPS),
@@ -95,6 +109,10 @@ def _declare(lib):
         "gphi_pagerank": (i32, [_P, u32, C.c_float, C.c_float, _P, _P]),
         "gphi_bellman_ford": (i32, [_P, u32, u32, _P, _P, _P]),
         "gphi_bfs_alg_bytes": (i32, [_P, _P, C.POINTER(Stats), _P]),
+        "gphi_graph_generate": (i32, [_P, C.POINTER(RmatSpec), u32, u32, i32, C.POINTER(WeightSpec),
+                                      i32, C.POINTER(_P)]),
+        "gphi_graph_download": (i32, [_P, _P, _P, _P, _P, _P, _P]),
+        "gphi_graph_tile_offsets": (i32, [_P, _P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -167,6 +185,19 @@ class Context:
         self.check(self.lib.gphi_graph_from_desc(self.h, C.byref(d), C.byref(h)))
         return Graph(self, h)
 
+    def generate(self, *, scale: int, nverts: int, seed: int, samples: int, target_unique: int = 0,
+                 abc=(0.57, 0.19, 0.19), perm_seed: int = 0, W: int, S: int,
+                 reorder: int = REORDER_DEGREE, weight_seed: int = 0, wlo: int = 1, whi: int = 255,
+                 tile_order: int = TILES_COL_STRIPE) -> "Graph":
+        """Generate an RMAT edge list on the device and build the graph in HBM, as the
+        reference tools would lay it out (gphi_graph_generate)."""
+        spec = RmatSpec(scale, nverts, seed, samples, target_unique, *abc, perm_seed)
+        ws = WeightSpec(weight_seed, wlo, whi)
+        h = _P()
+        self.check(self.lib.gphi_graph_generate(self.h, C.byref(spec), W, S, reorder, C.byref(ws),
+                                                tile_order, C.byref(h)))
+        return Graph(self, h)
+
     def load_text(self, prefix: str, W: int, S: int, weighted: bool = False) -> "Graph":
         """prefix_col-S-coo-tiled-W-* + prefix_untiled-* (apps/bfs_simd/bfs.cpp:983-1113)."""
         h = _P()
@@ -199,9 +230,10 @@ class Graph:
             self.ctx.lib.gphi_graph_free(self.h)
             self.h = None
 
-    def bfs(self, src: int, parents: bool = True):
-        """Hybrid BFS (graph_prepare).  Returns (levels int32, parents int32|None, Stats)."""
-        lv = np.empty(self.n, np.int32)
+    def bfs(self, src: int, parents: bool = True, levels: bool = True):
+        """Hybrid BFS (graph_prepare).  Returns (levels int32|None, parents int32|None, Stats).
+        With levels=parents=False the results stay in HBM (no device-to-host copy)."""
+        lv = np.empty(self.n, np.int32) if levels else None
         pa = np.empty(self.n, np.int32) if parents else None
         st = Stats()
         self.ctx.check(self.ctx.lib.gphi_bfs(self.h, src, _ptr(lv), _ptr(pa), C.byref(st)))
@@ -225,6 +257,32 @@ class Graph:
         self.ctx.check(self.ctx.lib.gphi_bellman_ford(self.h, src, max_sweeps, _ptr(d), C.byref(run),
                                                       C.byref(ms)))
         return d, run.value, ms.value
+
+    def download(self, csr: bool = True, tiles: bool = True) -> dict:
+        """Host copies of the graph arrays (degrees, CSR, tile order; weights if any)."""
+        out = {"deg": np.empty(self.n, np.uint32)}
+        if csr:
+            out["cols"] = np.empty(self.m, np.uint32)
+            out["cw"] = np.empty(self.m, np.uint32) if self.weighted else None
+        if tiles:
+            out["heads"] = np.empty(self.m, np.uint32)
+            out["tails"] = np.empty(self.m, np.uint32)
+            out["tw"] = np.empty(self.m, np.uint32) if self.weighted else None
+        g = out.get
+        self.ctx.check(self.ctx.lib.gphi_graph_download(
+            self.h, _ptr(g("deg")), _ptr(g("cols")), _ptr(g("cw")), _ptr(g("heads")), _ptr(g("tails")),
+            _ptr(g("tw"))))
+        return out
+
+    @property
+    def side(self) -> int:
+        return self.n // self.W + (1 if self.n % self.W else 0)
+
+    def tile_offsets(self) -> np.ndarray:
+        """The -offsets array of the stored tile order (SIDE^2 entries)."""
+        out = np.empty(self.side * self.side, np.uint32)
+        self.ctx.check(self.ctx.lib.gphi_graph_tile_offsets(self.h, _ptr(out)))
+        return out
 
     def alg_bytes(self, levels: np.ndarray, st: Stats) -> np.ndarray:
         out = np.zeros(max(st.steps, 1), np.float64)

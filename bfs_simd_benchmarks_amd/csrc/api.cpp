@@ -247,52 +247,20 @@ int gphi_bfs_alg_bytes(const gphi_graph *gr, const int32_t *levels, const gphi_s
 {
 	if (!gr || !levels || !st || !out) return set_error(nullptr, GPHI_E_ARG, "NULL argument");
 	const gphi::DevGraph &G = gr->g;
-	if (G.h_tile_off.size() != G.ntiles)
-		return set_error(gr->ctx, GPHI_E_STATE, "graph was built without tile offsets");
 	const uint32_t steps = st->steps;
 	const double nbm = (double)((G.n + 7) / 8);
-	// per row: bitmask of levels (< 64) present; per column: max level (-1 -> inf)
-	std::vector<uint64_t> row_lv(G.side, 0);
-	std::vector<int64_t> col_max(G.side, -1);
-	for (uint32_t v = 0; v < G.n; ++v) {
-		int32_t l = levels[v];
-		uint32_t r = v / G.W;
-		if (l >= 0 && l < 64) row_lv[r] |= 1ull << l;
-		int64_t key = l < 0 ? INT64_MAX : (int64_t)l;
-		if (key > col_max[r]) col_max[r] = key;
-	}
 	std::vector<uint32_t> dense_steps;
 	for (uint32_t k = 0; k < steps; ++k)
 		if (st->dir[k]) dense_steps.push_back(k);
+	std::vector<double> e_d, t_d;
+	std::lock_guard<std::mutex> lk(gr->ctx->mu);
+	if (hipSetDevice(gr->ctx->device) != hipSuccess) return set_error(gr->ctx, GPHI_E_HIP, "hipSetDevice failed");
+	int rc = gphi::count_active_tiles(const_cast<gphi_graph *>(gr), levels, dense_steps, e_d, t_d);
+	if (rc) return rc;
 	std::vector<double> e_act(steps, 0.0), t_act(steps, 0.0);
-	if (!dense_steps.empty()) {
-		// walk tiles in their stored order, recovering (row, col)
-		const uint64_t side = G.side, S = G.S;
-		const uint64_t rem = side % S, bound = side - rem;
-		uint64_t idx = 0;
-		for (uint64_t sr = 0; sr < side;) {
-			uint64_t step = G.tile_order == GPHI_TILES_COL_STRIPE ? (sr < bound ? S : rem) : side;
-			for (uint64_t j = 0; j < step * side; ++j, ++idx) {
-				uint64_t row, col;
-				if (G.tile_order == GPHI_TILES_COL_STRIPE) {
-					col = j / step;
-					row = sr + j % step;
-				} else {
-					row = idx / side;
-					col = idx % side;
-				}
-				uint64_t b = G.h_tile_off[idx];
-				uint64_t e = idx + 1 < G.ntiles ? G.h_tile_off[idx + 1] : G.m;
-				if (b == e) continue;
-				for (uint32_t k : dense_steps) {
-					if (k < 64 && (row_lv[row] >> k & 1ull) && col_max[col] > (int64_t)k) {
-						e_act[k] += (double)(e - b);
-						t_act[k] += 1.0;
-					}
-				}
-			}
-			sr += (G.tile_order == GPHI_TILES_COL_STRIPE) ? step : side;
-		}
+	for (size_t j = 0; j < dense_steps.size(); ++j) {
+		e_act[dense_steps[j]] = e_d[j];
+		t_act[dense_steps[j]] = t_d[j];
 	}
 	for (uint32_t k = 0; k < steps; ++k) {
 		double f = st->fsize[k], dF = st->fdeg[k];
@@ -306,6 +274,45 @@ int gphi_bfs_alg_bytes(const gphi_graph *gr, const int32_t *levels, const gphi_s
 		out[k] = b;
 	}
 	return GPHI_OK;
+}
+
+int gphi_graph_generate(gphi_ctx *ctx, const gphi_rmat_spec *spec, uint32_t W, uint32_t S,
+                        int reorder, const gphi_weight_spec *weights, int tile_order,
+                        gphi_graph **out)
+{
+	if (!ctx || !spec || !out) return set_error(ctx, GPHI_E_ARG, "gphi_graph_generate: NULL argument");
+	std::lock_guard<std::mutex> lk(ctx->mu);
+	*out = nullptr;
+	if (tile_order != GPHI_TILES_COL_STRIPE && tile_order != GPHI_TILES_ROW_MAJOR)
+		return set_error(ctx, GPHI_E_ARG, "unknown tile order %d", tile_order);
+	if (weights && weights->seed && weights->lo > weights->hi)
+		return set_error(ctx, GPHI_E_ARG, "weight range [%u,%u] is empty", weights->lo, weights->hi);
+	if (hipSetDevice(ctx->device) != hipSuccess) return set_error(ctx, GPHI_E_HIP, "hipSetDevice failed");
+	gphi_graph *g = new (std::nothrow) gphi_graph();
+	if (!g) return set_error(ctx, GPHI_E_OOM, "out of host memory");
+	g->ctx = ctx;
+	int rc = gphi::generate_graph(ctx, spec, W, S, reorder, weights, tile_order, g);
+	if (rc) {
+		gphi::free_graph_device(g);
+		delete g;
+		return rc;
+	}
+	*out = g;
+	return GPHI_OK;
+}
+
+int gphi_graph_download(gphi_graph *g, uint32_t *degrees, uint32_t *csr_edges, uint32_t *csr_weights,
+                        uint32_t *tile_heads, uint32_t *tile_tails, uint32_t *tile_weights)
+{
+	GPHI_ENTER(g);
+	return gphi::download_graph(g, degrees, csr_edges, csr_weights, tile_heads, tile_tails, tile_weights);
+}
+
+int gphi_graph_tile_offsets(gphi_graph *g, uint32_t *out)
+{
+	GPHI_ENTER(g);
+	if (!out) return set_error(g->ctx, GPHI_E_ARG, "out is NULL");
+	return gphi::tile_offsets(g, out);
 }
 
 }  // extern "C"

@@ -126,6 +126,139 @@ static unsigned grid_for(uint64_t items, unsigned per_block, unsigned cap)
 	return (unsigned)g;
 }
 
+// Pads the tile arrays to whole chunks with sentinel edges and builds the
+// chunk summaries.
+int finish_tiles(gphi_ctx *ctx, gphi_graph *gr)
+{
+	DevGraph &G = gr->g;
+	hipStream_t s = ctx->stream;
+	uint64_t padn = G.m_pad - G.m;
+	if (padn) {
+		fill_u32_kernel<<<grid_for(padn, kBlock, 1024), kBlock, 0, s>>>(G.heads + G.m, padn, G.n);
+		fill_u32_kernel<<<grid_for(padn, kBlock, 1024), kBlock, 0, s>>>(G.tails + G.m, padn, G.n);
+		if (G.coo_w) fill_u32_kernel<<<grid_for(padn, kBlock, 1024), kBlock, 0, s>>>(G.coo_w + G.m, padn, 0);
+	}
+	int rc;
+	if ((rc = dmalloc(ctx, (void **)&G.meta, (size_t)G.nchunks * sizeof(ChunkMeta) + 16, &G.device_bytes)))
+		return rc;
+	if (G.nchunks)
+		chunk_meta_kernel<<<grid_for(G.nchunks, 1, 65536), kBlock, 0, s>>>(G.heads, G.tails, G.n, G.W,
+		                                                                  G.nchunks, G.meta);
+	GPHI_HIP(ctx, hipGetLastError());
+	GPHI_HIP(ctx, hipStreamSynchronize(s));
+	return GPHI_OK;
+}
+
+int download_graph(gphi_graph *gr, uint32_t *deg, uint32_t *cols, uint32_t *cw, uint32_t *th,
+                   uint32_t *tt, uint32_t *tw)
+{
+	gphi_ctx *ctx = gr->ctx;
+	DevGraph &G = gr->g;
+	size_t mb = (size_t)G.m * 4;
+	if ((cols && !G.cols) || (cw && !G.csr_w) || (th && !G.heads) || (tw && !G.coo_w))
+		return set_error(ctx, GPHI_E_STATE, "graph lacks a requested array");
+	if (deg) GPHI_HIP(ctx, hipMemcpy(deg, G.deg, (size_t)G.n * 4, hipMemcpyDeviceToHost));
+	if (cols) GPHI_HIP(ctx, hipMemcpy(cols, G.cols, mb, hipMemcpyDeviceToHost));
+	if (cw) GPHI_HIP(ctx, hipMemcpy(cw, G.csr_w, mb, hipMemcpyDeviceToHost));
+	if (th) GPHI_HIP(ctx, hipMemcpy(th, G.heads, mb, hipMemcpyDeviceToHost));
+	if (tt) GPHI_HIP(ctx, hipMemcpy(tt, G.tails, mb, hipMemcpyDeviceToHost));
+	if (tw) GPHI_HIP(ctx, hipMemcpy(tw, G.coo_w, mb, hipMemcpyDeviceToHost));
+	return GPHI_OK;
+}
+
+// Byte-model counters: for each dense step k, edges (E) and distinct
+// non-empty tiles (T) whose head row holds a level-k vertex and whose tail
+// column still had an unvisited vertex when step k began.  Tiles are
+// contiguous in the stored order, so a tile starts where (row, col) changes.
+__global__ __launch_bounds__(kBlock) void active_tiles_kernel(
+    const uint32_t *__restrict__ heads, const uint32_t *__restrict__ tails, uint64_t m, uint32_t W,
+    const uint64_t *__restrict__ row_lv, const int64_t *__restrict__ col_max,
+    const uint32_t *__restrict__ steps, uint32_t nsteps, unsigned long long *e_out,
+    unsigned long long *t_out)
+{
+	__shared__ unsigned long long se[64], st_[64];
+	for (uint32_t k = threadIdx.x; k < 64; k += blockDim.x) se[k] = st_[k] = 0;
+	__syncthreads();
+	for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < m;
+	     e += (uint64_t)gridDim.x * blockDim.x) {
+		uint32_t r = heads[e] / W, c = tails[e] / W;
+		bool start = e == 0 || heads[e - 1] / W != r || tails[e - 1] / W != c;
+		uint64_t lv = row_lv[r];
+		int64_t cm = col_max[c];
+		for (uint32_t j = 0; j < nsteps; ++j) {
+			uint32_t k = steps[j];
+			if ((lv >> k & 1ull) && cm > (int64_t)k) {
+				atomicAdd(&se[j], 1ull);
+				if (start) atomicAdd(&st_[j], 1ull);
+			}
+		}
+	}
+	__syncthreads();
+	for (uint32_t j = threadIdx.x; j < nsteps; j += blockDim.x) {
+		if (se[j]) atomicAdd(e_out + j, se[j]);
+		if (st_[j]) atomicAdd(t_out + j, st_[j]);
+	}
+}
+
+int count_active_tiles(gphi_graph *gr, const int32_t *levels, const std::vector<uint32_t> &dense_steps,
+                       std::vector<double> &e_act, std::vector<double> &t_act)
+{
+	gphi_ctx *ctx = gr->ctx;
+	DevGraph &G = gr->g;
+	hipStream_t s = ctx->stream;
+	const uint32_t ns = (uint32_t)dense_steps.size();
+	e_act.assign(ns, 0.0);
+	t_act.assign(ns, 0.0);
+	if (!ns) return GPHI_OK;
+	if (!G.has_tiles) return set_error(ctx, GPHI_E_STATE, "graph has no tile arrays");
+	if (ns > 64) return set_error(ctx, GPHI_E_RANGE, "more than 64 dense steps");
+	std::vector<uint64_t> row_lv(G.side, 0);
+	std::vector<int64_t> col_max(G.side, -1);
+	for (uint32_t v = 0; v < G.n; ++v) {
+		int32_t l = levels[v];
+		uint32_t b = v / G.W;
+		if (l >= 0 && l < 64) row_lv[b] |= 1ull << l;
+		int64_t key = l < 0 ? INT64_MAX : (int64_t)l;
+		if (key > col_max[b]) col_max[b] = key;
+	}
+	uint64_t *d_lv = nullptr;
+	int64_t *d_cm = nullptr;
+	uint32_t *d_steps = nullptr;
+	unsigned long long *d_cnt = nullptr;
+	int rc;
+	if ((rc = dmalloc(ctx, (void **)&d_lv, (size_t)G.side * 8, nullptr)) ||
+	    (rc = dmalloc(ctx, (void **)&d_cm, (size_t)G.side * 8, nullptr)) ||
+	    (rc = dmalloc(ctx, (void **)&d_steps, 64 * 4, nullptr)) ||
+	    (rc = dmalloc(ctx, (void **)&d_cnt, 128 * 8, nullptr))) {
+		(void)hipFree(d_lv);
+		(void)hipFree(d_cm);
+		(void)hipFree(d_steps);
+		return rc;
+	}
+	hipError_t e = hipMemcpy(d_lv, row_lv.data(), (size_t)G.side * 8, hipMemcpyHostToDevice);
+	if (e == hipSuccess) e = hipMemcpy(d_cm, col_max.data(), (size_t)G.side * 8, hipMemcpyHostToDevice);
+	if (e == hipSuccess) e = hipMemcpy(d_steps, dense_steps.data(), ns * 4, hipMemcpyHostToDevice);
+	if (e == hipSuccess) e = hipMemsetAsync(d_cnt, 0, 128 * 8, s);
+	std::vector<unsigned long long> h(128, 0);
+	if (e == hipSuccess) {
+		active_tiles_kernel<<<grid_for(G.m, kBlock, 8192), kBlock, 0, s>>>(G.heads, G.tails, G.m, G.W, d_lv,
+		                                                               d_cm, d_steps, ns, d_cnt, d_cnt + 64);
+		e = hipGetLastError();
+	}
+	if (e == hipSuccess) e = hipMemcpyAsync(h.data(), d_cnt, 128 * 8, hipMemcpyDeviceToHost, s);
+	if (e == hipSuccess) e = hipStreamSynchronize(s);
+	(void)hipFree(d_lv);
+	(void)hipFree(d_cm);
+	(void)hipFree(d_steps);
+	(void)hipFree(d_cnt);
+	if (e != hipSuccess) return hip_fail(ctx, e, "active tile count", __FILE__, __LINE__);
+	for (uint32_t j = 0; j < ns; ++j) {
+		e_act[j] = (double)h[j];
+		t_act[j] = (double)h[64 + j];
+	}
+	return GPHI_OK;
+}
+
 int upload_graph(gphi_ctx *ctx, const gphi_graph_desc *d, gphi_graph *gr)
 {
 	DevGraph &G = gr->g;
@@ -183,25 +316,12 @@ int upload_graph(gphi_ctx *ctx, const gphi_graph_desc *d, gphi_graph *gr)
 		                             hipMemcpyHostToDevice, s));
 		GPHI_HIP(ctx, hipMemcpyAsync(G.tails, d->tile_tails, (size_t)G.m * 4,
 		                             hipMemcpyHostToDevice, s));
-		uint64_t padn = G.m_pad - G.m;
-		if (padn) {
-			fill_u32_kernel<<<grid_for(padn, kBlock, 1024), kBlock, 0, s>>>(G.heads + G.m, padn, G.n);
-			fill_u32_kernel<<<grid_for(padn, kBlock, 1024), kBlock, 0, s>>>(G.tails + G.m, padn, G.n);
-		}
 		if (d->tile_weights) {
 			if ((rc = dmalloc(ctx, (void **)&G.coo_w, pb, acc))) return rc;
 			GPHI_HIP(ctx, hipMemcpyAsync(G.coo_w, d->tile_weights, (size_t)G.m * 4,
 			                             hipMemcpyHostToDevice, s));
-			if (padn)
-				fill_u32_kernel<<<grid_for(padn, kBlock, 1024), kBlock, 0, s>>>(G.coo_w + G.m, padn, 0);
 		}
-		if ((rc = dmalloc(ctx, (void **)&G.meta, (size_t)G.nchunks * sizeof(ChunkMeta) + 16, acc)))
-			return rc;
-		if (G.nchunks)
-			chunk_meta_kernel<<<grid_for(G.nchunks, 1, 65536), kBlock, 0, s>>>(
-			    G.heads, G.tails, G.n, G.W, G.nchunks, G.meta);
-		GPHI_HIP(ctx, hipGetLastError());
-		if (d->tile_offsets) G.h_tile_off.assign(d->tile_offsets, d->tile_offsets + G.ntiles);
+		if ((rc = finish_tiles(ctx, gr))) return rc;
 	}
 	GPHI_HIP(ctx, hipStreamSynchronize(s));
 	return GPHI_OK;

@@ -1,0 +1,565 @@
+// gen.hip -- synthetic inputs and the reference tool pipeline, on the device.
+//
+// The reference takes its edge lists from external generators (PBBS rMat via
+// tools/adj2snap/adj2snap.cpp, Galois graph-convert) and turns them into its
+// formats with five CPU tools.  This file builds the same arrays in HBM:
+//
+//   edge list  --(csr_tiling, tools/csr_tiling/page_rank.cpp:513-570)-->  _untiled
+//   _untiled   --(vertex_id_reorder_to_degrees, tools/.../bfs.cpp:50-83)--> _reorder list
+//   _reorder   --(csr_tiling)--> CSR  --(coo_tiling, tools/coo_tiling/page_rank.cpp:396-496)
+//              --> row-major tiles --(column_major_tile, kcore.cpp:360-417)--> column-stripe tiles
+//
+// Every tool step is a stable bucketing, done here as a stable LSD radix sort.
+// RMAT spec (project-defined; the reference ships no generator): sample i of
+// seed s draws, per bit level from the MSB, one 32-bit uniform from a
+// splitmix64 stream started at (s << 40) ^ i; u < a -> (0,0), < a+b -> (0,1),
+// < a+b+c -> (1,0), else (1,1).  Self-loops, ids >= nverts and duplicates drop.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <climits>
+#include <cstring>
+#include <vector>
+
+#include "internal.h"
+
+namespace gphi {
+
+__host__ __device__ __forceinline__ uint64_t sm64_next(uint64_t &s)
+{
+	uint64_t z = (s += 0x9E3779B97F4A7C15ULL);
+	z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+	z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+	return z ^ (z >> 31);
+}
+
+__host__ __device__ __forceinline__ void rmat_sample(uint64_t seed, uint64_t i, uint32_t scale,
+                                                     uint32_t t0, uint32_t t1, uint32_t t2,
+                                                     uint32_t &h, uint32_t &t)
+{
+	uint64_t st = (seed << 40) ^ i;
+	uint32_t hh = 0, tt = 0;
+	uint64_t r = 0;
+	for (uint32_t l = 0; l < scale; ++l) {
+		uint32_t u;
+		if ((l & 1) == 0) {
+			r = sm64_next(st);
+			u = (uint32_t)r;
+		} else {
+			u = (uint32_t)(r >> 32);
+		}
+		uint32_t bit = 1u << (scale - 1 - l);
+		if (u < t0) {
+		} else if (u < t1) {
+			tt |= bit;
+		} else if (u < t2) {
+			hh |= bit;
+		} else {
+			hh |= bit;
+			tt |= bit;
+		}
+	}
+	h = hh;
+	t = tt;
+}
+
+void rmat_thresholds(double a, double b, double c, uint32_t thr[3])
+{
+	const double two32 = 4294967296.0;
+	double ab = a + b, abc = a + b + c;
+	thr[0] = (uint32_t)(a * two32);
+	thr[1] = ab >= 1.0 ? 0xFFFFFFFFu : (uint32_t)(ab * two32);
+	thr[2] = abc >= 1.0 ? 0xFFFFFFFFu : (uint32_t)(abc * two32);
+}
+
+namespace {
+
+constexpr unsigned kB = 256;
+
+unsigned grid1(uint64_t n) { return (unsigned)std::max<uint64_t>(1, std::min<uint64_t>((n + kB - 1) / kB, 1u << 20)); }
+
+#define GRID_STRIDE(i, n) \
+	for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < (n); i += (uint64_t)gridDim.x * blockDim.x)
+
+__global__ void sample_kernel(uint64_t seed, uint64_t first, uint64_t count, uint32_t scale,
+                              uint32_t nverts, uint32_t t0, uint32_t t1, uint32_t t2,
+                              uint64_t *keys, uint32_t *idx)
+{
+	GRID_STRIDE(k, count)
+	{
+		uint32_t h, t;
+		rmat_sample(seed, first + k, scale, t0, t1, t2, h, t);
+		keys[k] = (h >= nverts || t >= nverts || h == t) ? UINT64_MAX : ((uint64_t)h << 32) | t;
+		if (idx) idx[k] = (uint32_t)(first + k);
+	}
+}
+
+// 1 at the first element of every run of equal keys (and never for invalid keys)
+__global__ void run_heads_kernel(const uint64_t *keys, uint64_t n, uint8_t *flags)
+{
+	GRID_STRIDE(i, n)
+	{
+		uint64_t k = keys[i];
+		flags[i] = k != UINT64_MAX && (i == 0 || keys[i - 1] != k);
+	}
+}
+
+__global__ void split_keys_kernel(const uint64_t *keys, uint64_t m, uint32_t *h, uint32_t *t)
+{
+	GRID_STRIDE(i, m)
+	{
+		uint64_t k = keys[i];
+		h[i] = (uint32_t)(k >> 32);
+		t[i] = (uint32_t)k;
+	}
+}
+
+__global__ void relabel_kernel(uint64_t m, const uint32_t *map, const uint32_t *h, const uint32_t *t,
+                               uint32_t *oh, uint32_t *ot)
+{
+	GRID_STRIDE(i, m)
+	{
+		oh[i] = map[h[i]];
+		ot[i] = map[t[i]];
+	}
+}
+
+__global__ void iota_kernel(uint64_t m, uint32_t *p)
+{
+	GRID_STRIDE(i, m) p[i] = (uint32_t)i;
+}
+
+__global__ void degree_kernel(uint64_t m, const uint32_t *h, uint32_t *deg)
+{
+	GRID_STRIDE(i, m) atomicAdd(deg + h[i], 1u);
+}
+
+// key(v) = (unsigned)((float)deg + float(rand())/RAND_MAX); RAND_MAX converts to
+// 2^31 so the quotient is an exact scaling (tools/vertex_id_reorder_to_degrees/bfs.cpp:61)
+__global__ void degree_key_kernel(uint32_t n, const uint32_t *deg, const int32_t *rnd, uint64_t *out)
+{
+	GRID_STRIDE(v, n)
+	{
+		float f = (float)rnd[v] * 4.656612873077392578125e-10f;  // * 2^-31, exact
+		float s = (float)deg[v] + f;
+		uint32_t key = (uint32_t)s;
+		out[v] = ((uint64_t)key << 32) | (uint32_t)v;
+	}
+}
+
+__global__ void rank_from_sorted_kernel(uint32_t n, const uint64_t *sorted, uint32_t *rank)
+{
+	GRID_STRIDE(r, n) rank[(uint32_t)sorted[r]] = (uint32_t)r;
+}
+
+__global__ void weights_kernel(uint64_t m, uint64_t seed, uint32_t lo, uint64_t span, uint32_t *w)
+{
+	GRID_STRIDE(e, m)
+	{
+		uint64_t st = (seed << 40) ^ e ^ 0x00A5A5A5A5ULL;
+		w[e] = lo + (uint32_t)(sm64_next(st) % span);
+	}
+}
+
+// position of tile (h/W, t/W) in the stored tile order
+__global__ void tile_key_kernel(uint64_t m, const uint32_t *h, const uint32_t *t, uint32_t W,
+                                uint32_t side, uint32_t S, int col_stripe, uint32_t *key)
+{
+	const uint32_t nfull = side / S, rem = side % S;
+	GRID_STRIDE(i, m)
+	{
+		uint32_t r = h[i] / W, c = t[i] / W;
+		uint32_t k;
+		if (!col_stripe) {
+			k = r * side + c;
+		} else {
+			uint32_t s = r / S;
+			if (s < nfull)
+				k = s * S * side + c * S + (r - s * S);
+			else
+				k = nfull * S * side + c * rem + (r - nfull * S);
+		}
+		key[i] = k;
+	}
+}
+
+__global__ void gather_kernel(uint64_t m, const uint32_t *perm, const uint32_t *src, uint32_t *dst)
+{
+	GRID_STRIDE(i, m) dst[i] = src[perm[i]];
+}
+
+struct Scratch {
+	gphi_ctx *ctx;
+	std::vector<void *> ptrs;
+	~Scratch()
+	{
+		for (void *p : ptrs) (void)hipFree(p);
+	}
+	template <class T>
+	T *alloc(size_t count)
+	{
+		void *p = nullptr;
+		if (hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T)) != hipSuccess) return nullptr;
+		ptrs.push_back(p);
+		return (T *)p;
+	}
+	void release(void *p)
+	{
+		auto it = std::find(ptrs.begin(), ptrs.end(), p);
+		if (it != ptrs.end()) {
+			(void)hipFree(p);
+			ptrs.erase(it);
+		}
+	}
+	void keep(void *p)  // ownership moves to the graph
+	{
+		auto it = std::find(ptrs.begin(), ptrs.end(), p);
+		if (it != ptrs.end()) ptrs.erase(it);
+	}
+};
+
+#define OOM_IF(p) \
+	if (!(p)) return set_error(ctx, GPHI_E_OOM, "device allocation failed in graph generation")
+
+// stable sort of m uint32 keys with an index payload -> perm (sorted order of indices)
+int stable_order(gphi_ctx *ctx, Scratch &sc, const uint32_t *keys, uint64_t m, int end_bit,
+                 uint32_t *perm_out)
+{
+	hipStream_t s = ctx->stream;
+	uint32_t *kout = sc.alloc<uint32_t>(m), *idx = sc.alloc<uint32_t>(m);
+	OOM_IF(kout && idx);
+	iota_kernel<<<grid1(m), kB, 0, s>>>(m, idx);
+	size_t tb = 0;
+	GPHI_HIP(ctx, hipcub::DeviceRadixSort::SortPairs(nullptr, tb, keys, kout, idx, perm_out, m, 0,
+	                                                 end_bit, s));
+	void *tmp = sc.alloc<uint8_t>(tb);
+	OOM_IF(tmp);
+	GPHI_HIP(ctx, hipcub::DeviceRadixSort::SortPairs(tmp, tb, keys, kout, idx, perm_out, m, 0,
+	                                                 end_bit, s));
+	GPHI_HIP(ctx, hipStreamSynchronize(s));
+	sc.release(tmp);
+	sc.release(kout);
+	sc.release(idx);
+	return GPHI_OK;
+}
+
+int bits_for(uint64_t maxval)
+{
+	int b = 1;
+	while (b < 64 && (maxval >> b)) ++b;
+	return b;
+}
+
+}  // namespace
+
+// glibc rand() (TYPE_3 additive feedback generator), the same published
+// algorithm the oracle restates; the reference tool never seeds it (seed 1).
+void glibc_rand_seq(uint32_t seed, uint64_t n, int32_t *out)
+{
+	int32_t r[34];
+	uint32_t ring[34];
+	if (seed == 0) seed = 1;
+	r[0] = (int32_t)seed;
+	for (int i = 1; i < 31; ++i) {
+		int32_t hi = r[i - 1] / 127773, lo = r[i - 1] % 127773;
+		int32_t word = 16807 * lo - 2836 * hi;
+		if (word < 0) word += 2147483647;
+		r[i] = word;
+	}
+	for (int i = 31; i < 34; ++i) r[i] = r[i - 31];
+	for (int i = 0; i < 34; ++i) ring[i] = (uint32_t)r[i];
+	uint64_t k = 34;
+	for (; k < 344; ++k) ring[k % 34] = ring[(k - 31) % 34] + ring[(k - 3) % 34];
+	for (uint64_t j = 0; j < n; ++j, ++k) {
+		uint32_t v = ring[(k - 31) % 34] + ring[(k - 3) % 34];
+		ring[k % 34] = v;
+		out[j] = (int32_t)(v >> 1);
+	}
+}
+
+void random_perm_host(uint32_t n, uint64_t seed, uint32_t *perm)
+{
+	uint64_t st = seed * 0x9E3779B97F4A7C15ULL + 0x632BE59BD9B4E019ULL;
+	for (uint32_t i = 0; i < n; ++i) perm[i] = i;
+	for (uint32_t i = n; i > 1; --i) {
+		uint32_t j = (uint32_t)(sm64_next(st) % i);
+		std::swap(perm[i - 1], perm[j]);
+	}
+}
+
+// Builds the device graph of a generated RMAT edge list.
+int generate_graph(gphi_ctx *ctx, const gphi_rmat_spec *sp, uint32_t W, uint32_t S, int reorder,
+                   const gphi_weight_spec *ws, int tile_order, gphi_graph *gr)
+{
+	hipStream_t s = ctx->stream;
+	Scratch sc{ctx, {}};
+	const uint32_t n = sp->nverts;
+	int rc = GPHI_OK;
+	if (n == 0 || sp->scale == 0 || sp->scale > 32 || W == 0 || S == 0)
+		return set_error(ctx, GPHI_E_ARG, "bad generator arguments");
+	uint32_t thr[3];
+	rmat_thresholds(sp->a, sp->b, sp->c, thr);
+
+	// ---- 1. samples -> unique sorted (h,t) keys -------------------------------
+	uint64_t m = 0;
+	uint64_t *ukeys = nullptr;
+	if (sp->target_unique == 0) {
+		const uint64_t ns = sp->samples;
+		uint64_t *k0 = sc.alloc<uint64_t>(ns), *k1 = sc.alloc<uint64_t>(ns);
+		OOM_IF(k0 && k1);
+		sample_kernel<<<grid1(ns), kB, 0, s>>>(sp->seed, 0, ns, sp->scale, n, thr[0], thr[1], thr[2], k0, nullptr);
+		size_t tb = 0;
+		int eb = 32 + bits_for(n);
+		GPHI_HIP(ctx, hipcub::DeviceRadixSort::SortKeys(nullptr, tb, k0, k1, ns, 0, 64, s));
+		void *tmp = sc.alloc<uint8_t>(tb);
+		OOM_IF(tmp);
+		GPHI_HIP(ctx, hipcub::DeviceRadixSort::SortKeys(tmp, tb, k0, k1, ns, 0, 64, s));
+		(void)eb;
+		sc.release(tmp);
+		uint8_t *fl = sc.alloc<uint8_t>(ns);
+		uint64_t *cnt = sc.alloc<uint64_t>(1);
+		OOM_IF(fl && cnt);
+		run_heads_kernel<<<grid1(ns), kB, 0, s>>>(k1, ns, fl);
+		tb = 0;
+		GPHI_HIP(ctx, hipcub::DeviceSelect::Flagged(nullptr, tb, k1, fl, k0, cnt, ns, s));
+		tmp = sc.alloc<uint8_t>(tb);
+		OOM_IF(tmp);
+		GPHI_HIP(ctx, hipcub::DeviceSelect::Flagged(tmp, tb, k1, fl, k0, cnt, ns, s));
+		GPHI_HIP(ctx, hipMemcpyAsync(&m, cnt, 8, hipMemcpyDeviceToHost, s));
+		GPHI_HIP(ctx, hipStreamSynchronize(s));
+		sc.release(tmp);
+		sc.release(fl);
+		sc.release(k1);
+		ukeys = k0;
+	} else {
+		// first `target_unique` distinct valid samples in sample order
+		uint64_t ns = std::min<uint64_t>(sp->samples, std::max<uint64_t>(sp->target_unique * 2, 1024));
+		for (;;) {
+			uint64_t *k0 = sc.alloc<uint64_t>(ns), *k1 = sc.alloc<uint64_t>(ns);
+			uint32_t *i0 = sc.alloc<uint32_t>(ns), *i1 = sc.alloc<uint32_t>(ns);
+			uint8_t *fl = sc.alloc<uint8_t>(ns);
+			uint64_t *cnt = sc.alloc<uint64_t>(1);
+			OOM_IF(k0 && k1 && i0 && i1 && fl && cnt);
+			sample_kernel<<<grid1(ns), kB, 0, s>>>(sp->seed, 0, ns, sp->scale, n, thr[0], thr[1], thr[2], k0, i0);
+			size_t tb = 0;
+			GPHI_HIP(ctx, hipcub::DeviceRadixSort::SortPairs(nullptr, tb, k0, k1, i0, i1, ns, 0, 64, s));
+			void *tmp = sc.alloc<uint8_t>(tb);
+			OOM_IF(tmp);
+			GPHI_HIP(ctx, hipcub::DeviceRadixSort::SortPairs(tmp, tb, k0, k1, i0, i1, ns, 0, 64, s));
+			sc.release(tmp);
+			run_heads_kernel<<<grid1(ns), kB, 0, s>>>(k1, ns, fl);
+			// keep (first index, key) of each run
+			tb = 0;
+			GPHI_HIP(ctx, hipcub::DeviceSelect::Flagged(nullptr, tb, i1, fl, i0, cnt, ns, s));
+			tmp = sc.alloc<uint8_t>(tb);
+			OOM_IF(tmp);
+			GPHI_HIP(ctx, hipcub::DeviceSelect::Flagged(tmp, tb, i1, fl, i0, cnt, ns, s));
+			GPHI_HIP(ctx, hipStreamSynchronize(s));
+			sc.release(tmp);
+			tb = 0;
+			GPHI_HIP(ctx, hipcub::DeviceSelect::Flagged(nullptr, tb, k1, fl, k0, cnt, ns, s));
+			tmp = sc.alloc<uint8_t>(tb);
+			OOM_IF(tmp);
+			GPHI_HIP(ctx, hipcub::DeviceSelect::Flagged(tmp, tb, k1, fl, k0, cnt, ns, s));
+			uint64_t u = 0;
+			GPHI_HIP(ctx, hipMemcpyAsync(&u, cnt, 8, hipMemcpyDeviceToHost, s));
+			GPHI_HIP(ctx, hipStreamSynchronize(s));
+			sc.release(tmp);
+			if (u >= sp->target_unique || ns >= sp->samples) {
+				// order the unique edges by first occurrence, keep the first target
+				tb = 0;
+				GPHI_HIP(ctx, hipcub::DeviceRadixSort::SortPairs(nullptr, tb, i0, i1, k0, k1, u, 0, 32, s));
+				tmp = sc.alloc<uint8_t>(tb);
+				OOM_IF(tmp);
+				GPHI_HIP(ctx, hipcub::DeviceRadixSort::SortPairs(tmp, tb, i0, i1, k0, k1, u, 0, 32, s));
+				sc.release(tmp);
+				m = std::min<uint64_t>(u, sp->target_unique);
+				tb = 0;
+				GPHI_HIP(ctx, hipcub::DeviceRadixSort::SortKeys(nullptr, tb, k1, k0, m, 0, 64, s));
+				tmp = sc.alloc<uint8_t>(tb);
+				OOM_IF(tmp);
+				GPHI_HIP(ctx, hipcub::DeviceRadixSort::SortKeys(tmp, tb, k1, k0, m, 0, 64, s));
+				GPHI_HIP(ctx, hipStreamSynchronize(s));
+				sc.release(tmp);
+				sc.release(k1);
+				sc.release(i0);
+				sc.release(i1);
+				sc.release(fl);
+				ukeys = k0;
+				break;
+			}
+			sc.release(k0);
+			sc.release(k1);
+			sc.release(i0);
+			sc.release(i1);
+			sc.release(fl);
+			sc.release(cnt);
+			ns = std::min<uint64_t>(ns * 2, sp->samples);
+		}
+	}
+	if (m >= (1ull << 32)) return set_error(ctx, GPHI_E_RANGE, "too many edges (%llu)", (unsigned long long)m);
+
+	// ---- 2. edge list (sorted by (h,t)), optional id permutation ------------
+	uint32_t *h0 = sc.alloc<uint32_t>(m), *t0 = sc.alloc<uint32_t>(m);
+	OOM_IF(h0 && t0);
+	split_keys_kernel<<<grid1(m), kB, 0, s>>>(ukeys, m, h0, t0);
+	GPHI_HIP(ctx, hipStreamSynchronize(s));
+	sc.release(ukeys);
+	uint32_t *h1 = sc.alloc<uint32_t>(m), *t1 = sc.alloc<uint32_t>(m), *perm = sc.alloc<uint32_t>(m);
+	OOM_IF(h1 && t1 && perm);
+	const int vbits = bits_for(n);
+	if (sp->perm_seed) {
+		std::vector<uint32_t> hp(n);
+		random_perm_host(n, sp->perm_seed, hp.data());
+		uint32_t *dmap = sc.alloc<uint32_t>(n);
+		OOM_IF(dmap);
+		GPHI_HIP(ctx, hipMemcpy(dmap, hp.data(), (size_t)n * 4, hipMemcpyHostToDevice));
+		relabel_kernel<<<grid1(m), kB, 0, s>>>(m, dmap, h0, t0, h1, t1);
+		// _untiled: stable bucketing by (new) head
+		if ((rc = stable_order(ctx, sc, h1, m, vbits, perm))) return rc;
+		gather_kernel<<<grid1(m), kB, 0, s>>>(m, perm, h1, h0);
+		gather_kernel<<<grid1(m), kB, 0, s>>>(m, perm, t1, t0);
+		sc.release(dmap);
+	}
+	// (h0, t0) is now the _untiled order of the edge list
+
+	// ---- 3. relabel (degree order or none) -> the `_reorder` edge list -------
+	uint32_t *deg = sc.alloc<uint32_t>(n);
+	OOM_IF(deg);
+	if (reorder == GPHI_REORDER_DEGREE) {
+		GPHI_HIP(ctx, hipMemsetAsync(deg, 0, (size_t)n * 4, s));
+		degree_kernel<<<grid1(m), kB, 0, s>>>(m, h0, deg);
+		std::vector<int32_t> rnd(n);
+		glibc_rand_seq(1, n, rnd.data());
+		int32_t *drnd = sc.alloc<int32_t>(n);
+		uint64_t *vk0 = sc.alloc<uint64_t>(n), *vk1 = sc.alloc<uint64_t>(n);
+		uint32_t *rank = sc.alloc<uint32_t>(n);
+		OOM_IF(drnd && vk0 && vk1 && rank);
+		GPHI_HIP(ctx, hipMemcpyAsync(drnd, rnd.data(), (size_t)n * 4, hipMemcpyHostToDevice, s));
+		degree_key_kernel<<<grid1(n), kB, 0, s>>>(n, deg, drnd, vk0);
+		size_t tb = 0;
+		GPHI_HIP(ctx, hipcub::DeviceRadixSort::SortKeysDescending(nullptr, tb, vk0, vk1, n, 0, 64, s));
+		void *tmp = sc.alloc<uint8_t>(tb);
+		OOM_IF(tmp);
+		GPHI_HIP(ctx, hipcub::DeviceRadixSort::SortKeysDescending(tmp, tb, vk0, vk1, n, 0, 64, s));
+		rank_from_sorted_kernel<<<grid1(n), kB, 0, s>>>(n, vk1, rank);
+		relabel_kernel<<<grid1(m), kB, 0, s>>>(m, rank, h0, t0, h1, t1);
+		GPHI_HIP(ctx, hipStreamSynchronize(s));
+		sc.release(tmp);
+		sc.release(drnd);
+		sc.release(vk0);
+		sc.release(vk1);
+		sc.release(rank);
+		std::swap(h0, h1);
+		std::swap(t0, t1);
+	} else if (reorder != GPHI_REORDER_NONE) {
+		return set_error(ctx, GPHI_E_ARG, "unsupported reorder kind %d for generated graphs", reorder);
+	}
+	// (h0, t0) = the `_reorder` edge list; weights attach to this order
+	uint32_t *w0 = nullptr, *w1 = nullptr;
+	if (ws && ws->seed) {
+		w0 = sc.alloc<uint32_t>(m);
+		w1 = sc.alloc<uint32_t>(m);
+		OOM_IF(w0 && w1);
+		weights_kernel<<<grid1(m), kB, 0, s>>>(m, ws->seed, ws->lo, (uint64_t)ws->hi - ws->lo + 1, w0);
+	}
+
+	// ---- 4. CSR (_untiled of the _reorder list) ------------------------------
+	if ((rc = stable_order(ctx, sc, h0, m, vbits, perm))) return rc;
+	gather_kernel<<<grid1(m), kB, 0, s>>>(m, perm, h0, h1);
+	gather_kernel<<<grid1(m), kB, 0, s>>>(m, perm, t0, t1);
+	if (w0) gather_kernel<<<grid1(m), kB, 0, s>>>(m, perm, w0, w1);
+	// (h1, t1, w1) in CSR order
+	GPHI_HIP(ctx, hipMemsetAsync(deg, 0, (size_t)n * 4, s));
+	degree_kernel<<<grid1(m), kB, 0, s>>>(m, h1, deg);
+
+	// ---- 5. tile order (stable by tile position, from CSR order) --------------
+	const uint32_t side = n % W ? n / W + 1 : n / W;
+	if ((uint64_t)side * side > (1ull << 32))
+		return set_error(ctx, GPHI_E_RANGE, "SIDE^2 = %llu tiles exceeds 2^32", (unsigned long long)side * side);
+	uint32_t *key = h0;  // reuse: h0/t0 become scratch
+	tile_key_kernel<<<grid1(m), kB, 0, s>>>(m, h1, t1, W, side, S, tile_order == GPHI_TILES_COL_STRIPE, key);
+	if ((rc = stable_order(ctx, sc, key, m, bits_for((uint64_t)side * side - 1), perm))) return rc;
+	// tiled arrays go straight into padded graph buffers
+	DevGraph &G = gr->g;
+	G.n = n;
+	G.m = m;
+	G.W = W;
+	G.S = S;
+	G.side = side;
+	G.ntiles = (uint64_t)side * side;
+	G.tile_order = tile_order;
+	G.n_pad = (uint32_t)(((uint64_t)n + 1 + kBlock - 1) / kBlock * kBlock);
+	G.n_words = G.n_pad / 32;
+	G.nchunks = (m + kChunk - 1) / kChunk;
+	G.m_pad = G.nchunks * kChunk;
+	G.has_csr = G.has_tiles = true;
+	G.weighted = w0 != nullptr;
+	uint64_t *acc = &G.device_bytes;
+	auto galloc = [&](void **p, size_t bytes) -> int {
+		if (hipMalloc(p, std::max<size_t>(bytes, 4)) != hipSuccess) {
+			*p = nullptr;
+			return set_error(ctx, GPHI_E_OOM, "hipMalloc(%zu) failed", bytes);
+		}
+		*acc += bytes;
+		return GPHI_OK;
+	};
+	if ((rc = galloc((void **)&G.heads, G.m_pad * 4))) return rc;
+	if ((rc = galloc((void **)&G.tails, G.m_pad * 4))) return rc;
+	gather_kernel<<<grid1(m), kB, 0, s>>>(m, perm, h1, G.heads);
+	gather_kernel<<<grid1(m), kB, 0, s>>>(m, perm, t1, G.tails);
+	if (w0) {
+		if ((rc = galloc((void **)&G.coo_w, G.m_pad * 4))) return rc;
+		gather_kernel<<<grid1(m), kB, 0, s>>>(m, perm, w1, G.coo_w);
+	}
+	// CSR arrays: t1 / w1 move into the graph
+	G.cols = t1;
+	sc.keep(t1);
+	if (w0) {
+		G.csr_w = w1;
+		sc.keep(w1);
+	}
+	*acc += m * 4 * (w0 ? 2 : 1);
+	if ((rc = galloc((void **)&G.deg, (size_t)G.n_pad * 4))) return rc;
+	GPHI_HIP(ctx, hipMemsetAsync(G.deg, 0, (size_t)G.n_pad * 4, s));
+	GPHI_HIP(ctx, hipMemcpyAsync(G.deg, deg, (size_t)n * 4, hipMemcpyDeviceToDevice, s));
+	if ((rc = galloc((void **)&G.row_off, ((size_t)n + 1) * 4))) return rc;
+	{
+		size_t tb = 0;
+		GPHI_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(nullptr, tb, G.deg, G.row_off, n + 1, s));
+		void *tmp = sc.alloc<uint8_t>(tb);
+		OOM_IF(tmp);
+		GPHI_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(tmp, tb, G.deg, G.row_off, n + 1, s));
+		GPHI_HIP(ctx, hipStreamSynchronize(s));
+	}
+	return finish_tiles(ctx, gr);
+}
+
+// -offsets of the stored tile order: histogram of tile positions + scan
+int tile_offsets(gphi_graph *gr, uint32_t *out)
+{
+	gphi_ctx *ctx = gr->ctx;
+	DevGraph &G = gr->g;
+	hipStream_t s = ctx->stream;
+	if (!G.has_tiles) return set_error(ctx, GPHI_E_STATE, "graph has no tile arrays");
+	if (G.ntiles > (1ull << 32)) return set_error(ctx, GPHI_E_RANGE, "too many tiles");
+	Scratch sc{ctx, {}};
+	const uint64_t nt = G.ntiles, m = G.m;
+	uint32_t *key = sc.alloc<uint32_t>(m), *cnt = sc.alloc<uint32_t>(nt + 1), *off = sc.alloc<uint32_t>(nt + 1);
+	OOM_IF(key && cnt && off);
+	tile_key_kernel<<<grid1(m), kB, 0, s>>>(m, G.heads, G.tails, G.W, G.side, G.S,
+	                                       G.tile_order == GPHI_TILES_COL_STRIPE, key);
+	GPHI_HIP(ctx, hipMemsetAsync(cnt, 0, (nt + 1) * 4, s));
+	degree_kernel<<<grid1(m), kB, 0, s>>>(m, key, cnt);
+	size_t tb = 0;
+	GPHI_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(nullptr, tb, cnt, off, nt, s));
+	void *tmp = sc.alloc<uint8_t>(tb);
+	OOM_IF(tmp);
+	GPHI_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(tmp, tb, cnt, off, nt, s));
+	GPHI_HIP(ctx, hipMemcpyAsync(out, off, nt * 4, hipMemcpyDeviceToHost, s));
+	GPHI_HIP(ctx, hipStreamSynchronize(s));
+	return GPHI_OK;
+}
+
+}  // namespace gphi

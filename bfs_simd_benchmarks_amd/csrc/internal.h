@@ -66,8 +66,6 @@ struct DevGraph {
 	// tiled COO (padded)
 	uint32_t *heads = nullptr, *tails = nullptr, *coo_w = nullptr;
 	ChunkMeta *meta = nullptr;
-	// host copy of tile sizes for the byte model (SIDE^2 entries, may be empty)
-	std::vector<uint32_t> h_tile_off;
 	uint64_t device_bytes = 0;
 };
 
@@ -114,6 +112,19 @@ int run_pagerank(gphi_graph *gr, uint32_t iters, float damping, float init, floa
                  float *ms);
 int run_bellman_ford(gphi_graph *gr, uint32_t src, uint32_t max_sweeps, uint32_t *dist,
                      uint32_t *sweeps_run, float *ms);
+
+int finish_tiles(gphi_ctx *ctx, gphi_graph *gr);
+int count_active_tiles(gphi_graph *gr, const int32_t *levels, const std::vector<uint32_t> &dense_steps,
+                       std::vector<double> &e_act, std::vector<double> &t_act);
+int tile_offsets(gphi_graph *gr, uint32_t *out);
+int download_graph(gphi_graph *gr, uint32_t *deg, uint32_t *cols, uint32_t *cw, uint32_t *th,
+                   uint32_t *tt, uint32_t *tw);
+
+// generator + tool pipeline on the device (gen.hip)
+int generate_graph(gphi_ctx *ctx, const gphi_rmat_spec *sp, uint32_t W, uint32_t S, int reorder,
+                   const gphi_weight_spec *ws, int tile_order, gphi_graph *gr);
+void glibc_rand_seq(uint32_t seed, uint64_t n, int32_t *out);
+void random_perm_host(uint32_t n, uint64_t seed, uint32_t *perm);
 
 // host text formats (formats.cpp)
 int read_text_graph(gphi_ctx *ctx, const char *prefix, uint32_t W, uint32_t S, int weighted,

@@ -122,6 +122,45 @@ int gphi_pagerank(gphi_graph *g, uint32_t iters, float damping, float init, floa
 int gphi_bellman_ford(gphi_graph *g, uint32_t src, uint32_t max_sweeps, uint32_t *dist,
                       uint32_t *sweeps_run, float *ms);
 
+/* ---- synthetic inputs and the tool pipeline on the device ----
+ * The reference's inputs came from external generators (PBBS rMat,
+ * tools/adj2snap/adj2snap.cpp; Galois graph-convert).  gphi_graph_generate
+ * draws an RMAT edge list (project spec: per-sample splitmix64 streams), drops
+ * self-loops / ids >= nverts / duplicates, optionally permutes ids, relabels
+ * by degree (tools/vertex_id_reorder_to_degrees/bfs.cpp:50-83, glibc rand()
+ * tie-break) and builds the CSR (tools/csr_tiling) and tile order
+ * (tools/coo_tiling, tools/column_major_tile) directly in HBM. */
+typedef struct gphi_rmat_spec {
+	uint32_t scale;          /* ids drawn from [0, 2^scale) */
+	uint32_t nverts;         /* N; samples with an id >= N are dropped */
+	uint64_t seed;
+	uint64_t samples;        /* samples drawn (upper bound when target_unique > 0) */
+	uint64_t target_unique;  /* 0: keep all unique edges; else the first this many */
+	double a, b, c;          /* quadrant probabilities (d = 1-a-b-c) */
+	uint64_t perm_seed;      /* 0: no id permutation before reordering */
+} gphi_rmat_spec;
+
+typedef struct gphi_weight_spec {
+	uint64_t seed;           /* 0: unweighted */
+	uint32_t lo, hi;         /* uniform integer weights in [lo, hi] */
+} gphi_weight_spec;
+
+#define GPHI_REORDER_NONE 0
+#define GPHI_REORDER_DEGREE 1 /* tools/vertex_id_reorder_to_degrees */
+
+int gphi_graph_generate(gphi_ctx *ctx, const gphi_rmat_spec *spec, uint32_t W, uint32_t S,
+                        int reorder, const gphi_weight_spec *weights, int tile_order,
+                        gphi_graph **out);
+/* Copy a graph's arrays to host buffers (any may be NULL): degrees[n],
+ * csr_edges[m], csr_weights[m], tile_heads[m], tile_tails[m], tile_weights[m]. */
+int gphi_graph_download(gphi_graph *g, uint32_t *degrees, uint32_t *csr_edges,
+                        uint32_t *csr_weights, uint32_t *tile_heads, uint32_t *tile_tails,
+                        uint32_t *tile_weights);
+
+/* Exclusive tile offsets in the stored tile order (the -offsets file);
+ * out has SIDE^2 entries. */
+int gphi_graph_tile_offsets(gphi_graph *g, uint32_t *out);
+
 /* Algorithmic HBM bytes of a finished BFS (per step, the byte model of the
  * design doc) from its levels and stats; bytes_per_step has st->steps entries. */
 int gphi_bfs_alg_bytes(const gphi_graph *g, const int32_t *levels, const gphi_stats *st,

@@ -309,18 +309,3 @@ def config_graph(name, W=None, S=None, weight_seed=None):
         w = weights(g.m, weight_seed)
         g = tile_edgelist(n, g.eh, g.et, g.W, g.S, w)
     return g
-
-
-def sources(n, deg, count, seed):
-    """First `count` distinct vertices with outdeg > 0 of a seeded stream."""
-    st = (seed * 0x9E3779B97F4A7C15 + 0x5DEECE66D) & (2**64 - 1)
-    out, seen = [], set()
-    i = 0
-    while len(out) < count and i < 100 * count + 1000:
-        h, _ = rmat_sample(st, i, 32, (0.25, 0.25, 0.25))
-        v = h % n
-        i += 1
-        if deg[v] > 0 and v not in seen:
-            seen.add(v)
-            out.append(v)
-    return np.array(out, np.uint32)

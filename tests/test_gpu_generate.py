@@ -1,0 +1,73 @@
+"""The device-side generator and tool pipeline (gphi_graph_generate) must lay
+out exactly the arrays the oracle's restatement of the reference tools
+produces: degrees, CSR order, tile order and weights, element for element."""
+import numpy as np
+import pytest
+
+import oracle as O
+from bfs_simd_benchmarks_amd.workloads import WORKLOADS
+
+pytestmark = pytest.mark.gpu
+
+
+def _cmp(a, b, what):
+    assert a.shape == b.shape, (what, a.shape, b.shape)
+    bad = np.flatnonzero(a != b)
+    assert bad.size == 0, f"{what}: {bad.size} mismatches, first at {bad[:5]}"
+
+
+def test_generate_c1_matches_tool_pipeline(ctx, c1):
+    print("case: generate c1", flush=True)
+    g = WORKLOADS["c1"].generate(ctx)
+    try:
+        assert (g.n, g.m) == (c1.n, c1.m)
+        d = g.download()
+    finally:
+        g.free()
+    _cmp(d["deg"], c1.deg, "deg")
+    _cmp(d["cols"], c1.cols, "csr")
+    _cmp(d["heads"], c1.sh, "tile heads")
+    _cmp(d["tails"], c1.st, "tile tails")
+
+
+def test_generate_c3_weights(ctx, c3):
+    print("case: generate c3 weighted", flush=True)
+    g = WORKLOADS["c3"].generate(ctx, weighted=True)
+    try:
+        assert g.weighted
+        d = g.download()
+    finally:
+        g.free()
+    _cmp(d["cols"], c3.cols, "csr")
+    _cmp(d["cw"], c3.cw, "csr weights")
+    _cmp(d["tw"], c3.sw, "tile weights")
+    assert d["cw"].min() >= 1 and d["cw"].max() <= 255
+
+
+@pytest.mark.parametrize("W,S,order", [(256, 4, 0), (1000, 3, 0), (1024, 16, 1), (64, 1, 0)])
+def test_generate_c1_other_tilings(ctx, W, S, order):
+    print(f"case: generate c1 W={W} S={S} order={order}", flush=True)
+    n, h, t = O.config_edges("c1")
+    ref = O.build_from_edgelist(n, h, t, W, S)
+    g = WORKLOADS["c1"].generate(ctx, W=W, S=S, tile_order=order)
+    try:
+        d = g.download(csr=False)
+    finally:
+        g.free()
+    _cmp(d["heads"], ref.rh if order else ref.sh, "tile heads")
+    _cmp(d["tails"], ref.rt if order else ref.st, "tile tails")
+
+
+def test_generate_c2_pokec_shape(ctx):
+    print("case: generate c2 (first 30,622,564 unique samples, permuted ids)", flush=True)
+    ref = O.config_graph("c2")
+    g = WORKLOADS["c2"].generate(ctx)
+    try:
+        assert (g.n, g.m) == (1_632_803, 30_622_564)
+        d = g.download()
+    finally:
+        g.free()
+    _cmp(d["deg"], ref.deg, "deg")
+    _cmp(d["cols"], ref.cols, "csr")
+    _cmp(d["heads"], ref.sh, "tile heads")
+    _cmp(d["tails"], ref.st, "tile tails")
