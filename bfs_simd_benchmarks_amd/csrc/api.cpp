@@ -97,6 +97,7 @@ static int check_desc(gphi_ctx *ctx, const gphi_graph_desc *d)
 {
 	if (!d) return set_error(ctx, GPHI_E_ARG, "graph descriptor is NULL");
 	if (d->n == 0) return set_error(ctx, GPHI_E_RANGE, "graph has no vertices");
+	if (d->n > 0xFFFFFFFFu - 2 * gphi::kPadVerts) return set_error(ctx, GPHI_E_RANGE, "N=%u too large", d->n);
 	if (d->m >= (1ull << 32)) return set_error(ctx, GPHI_E_RANGE, "M=%llu does not fit the %%u formats",
 	                                           (unsigned long long)d->m);
 	if (d->tile_width == 0) return set_error(ctx, GPHI_E_ARG, "tile width is 0");

@@ -270,7 +270,7 @@ int upload_graph(gphi_ctx *ctx, const gphi_graph_desc *d, gphi_graph *gr)
 	G.side = G.n % G.W ? G.n / G.W + 1 : G.n / G.W;
 	G.ntiles = (uint64_t)G.side * G.side;
 	G.tile_order = d->tile_order;
-	G.n_pad = (uint32_t)(((uint64_t)G.n + 1 + kBlock - 1) / kBlock * kBlock);
+	G.n_pad = (uint32_t)(((uint64_t)G.n + 1 + kPadVerts - 1) / kPadVerts * kPadVerts);
 	G.n_words = G.n_pad / 32;  // the update pass writes whole words up to n_pad
 	G.nchunks = (G.m + kChunk - 1) / kChunk;
 	G.m_pad = G.nchunks * kChunk;
@@ -333,7 +333,7 @@ void free_graph_device(gphi_graph *gr)
 	void *ps[] = {G.row_off, G.cols, G.csr_w, G.deg, G.heads, G.tails, G.coo_w, G.meta,
 	              gr->level, gr->parent, gr->dist, gr->flag, gr->visited, gr->cur,
 	              gr->row_stamp, gr->col_unvis, gr->queue, gr->qoff, gr->ctrl, gr->rank,
-	              gr->sum, gr->contrib, gr->tile_lo};
+	              gr->sum, gr->contrib, gr->tile_lo, gr->blk_cnt, gr->blk_base};
 	for (void *p : ps)
 		if (p) (void)hipFree(p);
 }
@@ -359,6 +359,9 @@ static int ensure_state(gphi_graph *gr)
 	if ((rc = dmalloc(ctx, (void **)&gr->qoff, nb, nullptr))) return rc;
 	if ((rc = dmalloc(ctx, (void **)&gr->tile_lo, (G.m / kSparseTile + 2) * 4, nullptr))) return rc;
 	if ((rc = dmalloc(ctx, (void **)&gr->ctrl, sizeof(StepCtrl), nullptr))) return rc;
+	const size_t nblk = G.n_pad / kPadVerts;
+	if ((rc = dmalloc(ctx, (void **)&gr->blk_cnt, nblk * sizeof(uint4), nullptr))) return rc;
+	if ((rc = dmalloc(ctx, (void **)&gr->blk_base, nblk * sizeof(uint2), nullptr))) return rc;
 	GPHI_HIP(ctx, hipMemsetAsync(gr->flag, 0, G.n_pad, ctx->stream));
 	GPHI_HIP(ctx, hipMemsetAsync(gr->ctrl, 0, sizeof(StepCtrl), ctx->stream));
 	return GPHI_OK;
@@ -591,93 +594,185 @@ __global__ __launch_bounds__(kBlock) void sparse_push_kernel(
 }
 
 // ---------------------------------------------------------------------------
-// frontier update: one lane per vertex.  Turns the flags of the step into the
-// next frontier bitmap, the sparse queue with its degree prefix and tile
-// partition, the row stamps, and the |F| / sum(outdeg) counters; the last
-// block applies the direction rule |F|+sum > M/T_RATIO (bfs.cpp:829-834).
+// frontier update (update_dense :91-179, update_sparse :467-509, to_dense
+// :54-88, to_sparse :400-465 and the rule :829-834 of apps/bfs_simd/bfs.cpp,
+// fused).  Three launches, no same-address atomics on the hot counters:
+//   mark : one lane per 32-vertex word: flags -> next-frontier word, visited,
+//          levels, row stamps, column counts; per-block (|F|, queue, sum deg)
+//   scan : one block: exclusive scan of the per-block counts, direction rule
+//   emit : one lane per word: the sparse queue in ascending vertex order with
+//          its degree prefix and the tile partition of the next CSR push
 // ---------------------------------------------------------------------------
+
 template <bool kBfs>
-__global__ __launch_bounds__(kBlock) void frontier_update_kernel(
-    uint32_t n, uint32_t W, uint32_t threshold, uint32_t next_level,
-    uint8_t *__restrict__ flag, const uint32_t *__restrict__ deg, int32_t *__restrict__ level,
-    uint32_t *__restrict__ visited, uint32_t *__restrict__ cur, uint32_t *__restrict__ row_stamp,
-    uint32_t *__restrict__ col_unvis, uint32_t *__restrict__ queue, uint32_t *__restrict__ qoff,
-    uint32_t *__restrict__ tile_lo, StepCtrl *ctrl)
+__global__ __launch_bounds__(kBlock) void frontier_mark_kernel(
+    uint32_t W, uint32_t next_level, uint8_t *__restrict__ flag, const uint32_t *__restrict__ deg,
+    int32_t *__restrict__ level, uint32_t *__restrict__ visited, uint32_t *__restrict__ cur,
+    uint32_t *__restrict__ row_stamp, uint32_t *__restrict__ col_unvis, uint4 *__restrict__ blk_cnt)
 {
-	const uint32_t v = blockIdx.x * kBlock + threadIdx.x;  // grid covers n_pad exactly
-	const uint32_t lane = lane_id();
-	const bool f = v < n && flag[v] != 0;
-	const unsigned long long mask = __ballot(f);
-	if (lane == 0 || lane == 32) {
-		uint32_t bits = lane == 0 ? (uint32_t)mask : (uint32_t)(mask >> 32);
-		cur[v >> 5] = bits;
-		if (kBfs && bits) visited[v >> 5] |= bits;
-	}
-	uint32_t d = 0;
-	if (f) {
-		flag[v] = 0;
-		d = deg[v];
-		if (kBfs) level[v] = (int32_t)next_level;
-		row_stamp[v / W] = next_level;
-	}
-	if (kBfs && mask) {
-		uint32_t vfirst = v - lane;
-		uint32_t c0 = vfirst / W, c1 = (vfirst + 63) / W;
-		if (c0 == c1) {
-			if (lane == 0) atomicSub(col_unvis + c0, (uint32_t)__popcll(mask));
-		} else if (f) {
-			atomicSub(col_unvis + v / W, 1u);
+	__shared__ uint32_t s_red[3][kBlock / 64];
+	const uint32_t w = blockIdx.x * kBlock + threadIdx.x;
+	const uint32_t v0 = w * 32;
+	uint4 *fp = reinterpret_cast<uint4 *>(flag + v0);
+	const uint4 fa = fp[0], fb = fp[1];
+	const uint32_t fw[8] = {fa.x, fa.y, fa.z, fa.w, fb.x, fb.y, fb.z, fb.w};
+	uint32_t bits = 0;
+#pragma unroll
+	for (int j = 0; j < 8; ++j)
+#pragma unroll
+		for (int k = 0; k < 4; ++k)
+			if ((fw[j] >> (8 * k)) & 0xFFu) bits |= 1u << (4 * j + k);
+	uint32_t F = 0, Q = 0, D = 0;
+	cur[w] = bits;
+	const bool row_uniform = (W & 31u) == 0;
+	if (bits) {
+		fp[0] = make_uint4(0, 0, 0, 0);
+		fp[1] = make_uint4(0, 0, 0, 0);
+		if (kBfs) visited[w] |= bits;
+		if (row_uniform) row_stamp[v0 / W] = next_level;
+		F = (uint32_t)__popc(bits);
+		for (uint32_t b = bits; b; b &= b - 1) {
+			const uint32_t v = v0 + (uint32_t)__ffs(b) - 1;
+			if (kBfs) level[v] = (int32_t)next_level;
+			if (!row_uniform) row_stamp[v / W] = next_level;
+			const uint32_t d = deg[v];
+			D += d;
+			Q += d > 0;
 		}
 	}
-	// queue append: frontier vertices with outdeg > 0
-	const bool q = f && d > 0;
-	const unsigned long long qmask = __ballot(q);
-	uint32_t dsum;
-	uint32_t dpre = wave_excl_scan(q ? d : 0u, &dsum);
-	if (qmask) {
-		uint32_t rank = (uint32_t)__popcll(qmask & ((1ull << lane) - 1ull));
-		unsigned long long packed = 0;
-		if (lane == 0) {
-			packed = atomicAdd(&ctrl->qpack, ((unsigned long long)__popcll(qmask) << 32) | dsum);
-			atomicAdd(&ctrl->facc, (uint32_t)__popcll(mask));
-			__threadfence();
+	if (kBfs) {
+		// column counts of unvisited vertices: one atomic per wave when the
+		// wave's 2048 vertices share a tail block, else one per word
+		const uint32_t wv0 = (w - lane_id()) * 32;
+		const bool uniform = (W & 31u) == 0 && wv0 / W == (wv0 + 64 * 32 - 1) / W;
+		if (uniform) {
+			const uint32_t tot = wave_sum(F);
+			if (lane_id() == 0 && tot) atomicSub(col_unvis + wv0 / W, tot);
+		} else if (bits) {
+			if (row_uniform) {
+				atomicSub(col_unvis + v0 / W, F);
+			} else {
+				for (uint32_t b = bits; b; b &= b - 1) atomicSub(col_unvis + (v0 + __ffs(b) - 1) / W, 1u);
+			}
 		}
-		packed = __shfl(packed, 0, 64);
-		if (q) {
-			uint32_t pos = (uint32_t)(packed >> 32) + rank;
-			uint32_t off = (uint32_t)packed + dpre;
-			queue[pos] = v;
-			qoff[pos] = off;
-			// tiles of the next push whose first edge falls in [off, off+d)
-			uint32_t k = (off + kSparseTile - 1) / kSparseTile;
-			for (; (uint64_t)k * kSparseTile < (uint64_t)off + d; ++k) tile_lo[k] = pos;
-		}
-	} else if (mask && lane == 0) {
-		atomicAdd(&ctrl->facc, (uint32_t)__popcll(mask));
-		__threadfence();
 	}
-	__shared__ bool s_last;
+	F = wave_sum(F);
+	Q = wave_sum(Q);
+	D = wave_sum(D);
+	const uint32_t wid = threadIdx.x >> 6;
+	if (lane_id() == 0) {
+		s_red[0][wid] = F;
+		s_red[1][wid] = Q;
+		s_red[2][wid] = D;
+	}
 	__syncthreads();
 	if (threadIdx.x == 0) {
-		__threadfence();
-		uint32_t prev = atomicAdd(&ctrl->blocks_done, 1u);
-		s_last = (prev == gridDim.x - 1);
+		uint4 r = make_uint4(0, 0, 0, 0);
+		for (int k = 0; k < (int)(kBlock / 64); ++k) {
+			r.x += s_red[0][k];
+			r.y += s_red[1][k];
+			r.z += s_red[2][k];
+		}
+		blk_cnt[blockIdx.x] = r;
+	}
+}
+
+// One block: exclusive scan of per-block (queue, degree) counts, totals, and
+// the direction rule |F| + sum(outdeg) > M / T_RATIO in uint32 arithmetic.
+constexpr uint32_t kScanThreads = 1024;
+__global__ __launch_bounds__(kScanThreads) void frontier_scan_kernel(uint32_t nblk, uint32_t threshold,
+                                                                     const uint4 *__restrict__ blk_cnt,
+                                                                     uint2 *__restrict__ blk_base,
+                                                                     StepCtrl *ctrl)
+{
+	__shared__ uint32_t sq[kScanThreads / 64], sd[kScanThreads / 64], sf[kScanThreads / 64];
+	const uint32_t per = (nblk + kScanThreads - 1) / kScanThreads;
+	const uint32_t b0 = min(nblk, threadIdx.x * per), b1 = min(nblk, b0 + per);
+	uint32_t q = 0, d = 0, f = 0;
+	for (uint32_t b = b0; b < b1; ++b) {
+		uint4 c = blk_cnt[b];
+		f += c.x;
+		q += c.y;
+		d += c.z;
+	}
+	uint32_t qt, dt;
+	uint32_t qx = wave_excl_scan(q, &qt), dx = wave_excl_scan(d, &dt);
+	uint32_t ft = wave_sum(f);
+	const uint32_t wid = threadIdx.x >> 6;
+	if (lane_id() == 0) {
+		sq[wid] = qt;
+		sd[wid] = dt;
+		sf[wid] = ft;
 	}
 	__syncthreads();
-	if (s_last && threadIdx.x == 0) {
-		__threadfence();
-		uint32_t fs = atomicAdd(&ctrl->facc, 0u);
-		unsigned long long qp = atomicAdd(&ctrl->qpack, 0ull);
-		uint32_t fd = (uint32_t)qp;
-		ctrl->fsize = fs;
-		ctrl->fdeg = fd;
-		ctrl->qsize = (uint32_t)(qp >> 32);
-		ctrl->dir = (uint32_t)(fs + fd) > threshold ? 1u : 0u;
-		ctrl->done = fs == 0;
-		ctrl->facc = 0;
-		ctrl->qpack = 0;
-		ctrl->blocks_done = 0;
-		__threadfence();
+	uint32_t qb = 0, db = 0, qall = 0, dall = 0, fall = 0;
+	for (uint32_t k = 0; k < kScanThreads / 64; ++k) {
+		if (k < wid) {
+			qb += sq[k];
+			db += sd[k];
+		}
+		qall += sq[k];
+		dall += sd[k];
+		fall += sf[k];
+	}
+	qx += qb;
+	dx += db;
+	for (uint32_t b = b0; b < b1; ++b) {
+		uint4 c = blk_cnt[b];
+		blk_base[b] = make_uint2(qx, dx);
+		qx += c.y;
+		dx += c.z;
+	}
+	if (threadIdx.x == 0) {
+		ctrl->fsize = fall;
+		ctrl->fdeg = dall;
+		ctrl->qsize = qall;
+		ctrl->dir = (uint32_t)(fall + dall) > threshold ? 1u : 0u;
+		ctrl->done = fall == 0;
+	}
+}
+
+__global__ __launch_bounds__(kBlock) void frontier_emit_kernel(
+    const uint32_t *__restrict__ deg, const uint32_t *__restrict__ cur,
+    const uint4 *__restrict__ blk_cnt, const uint2 *__restrict__ blk_base,
+    uint32_t *__restrict__ queue, uint32_t *__restrict__ qoff, uint32_t *__restrict__ tile_lo)
+{
+	__shared__ uint32_t sq[kBlock / 64], sd[kBlock / 64];
+	if (blk_cnt[blockIdx.x].y == 0) return;  // uniform: no queue entries here
+	const uint32_t w = blockIdx.x * kBlock + threadIdx.x;
+	const uint32_t v0 = w * 32;
+	const uint32_t bits = cur[w];
+	uint32_t q = 0, d = 0;
+	for (uint32_t b = bits; b; b &= b - 1) {
+		const uint32_t dv = deg[v0 + __ffs(b) - 1];
+		q += dv > 0;
+		d += dv;
+	}
+	uint32_t qt, dt;
+	uint32_t qx = wave_excl_scan(q, &qt), dx = wave_excl_scan(d, &dt);
+	const uint32_t wid = threadIdx.x >> 6;
+	if (lane_id() == 0) {
+		sq[wid] = qt;
+		sd[wid] = dt;
+	}
+	__syncthreads();
+	for (uint32_t k = 0; k < wid; ++k) {
+		qx += sq[k];
+		dx += sd[k];
+	}
+	const uint2 base = blk_base[blockIdx.x];
+	uint32_t pos = base.x + qx, off = base.y + dx;
+	for (uint32_t b = bits; b; b &= b - 1) {
+		const uint32_t v = v0 + __ffs(b) - 1;
+		const uint32_t dv = deg[v];
+		if (!dv) continue;
+		queue[pos] = v;
+		qoff[pos] = off;
+		// tiles of the next push whose first edge falls in [off, off+dv)
+		for (uint32_t k = (off + kSparseTile - 1) / kSparseTile; (uint64_t)k * kSparseTile < (uint64_t)off + dv; ++k)
+			tile_lo[k] = pos;
+		++pos;
+		off += dv;
 	}
 }
 
@@ -728,7 +823,7 @@ static int frontier_loop(gphi_graph *gr, uint32_t src, gphi_stats *st)
 
 	uint32_t steps = 0;
 	uint64_t reached = 0, trav = 0;
-	const unsigned ugrid = G.n_pad / kBlock;
+	const unsigned nblk = G.n_pad / kPadVerts;
 	while (!hc->done) {
 		const uint32_t L = steps;
 		const uint32_t dir = (steps == 0) ? 0u : hc->dir;  // first step sparse (bfs.cpp:807)
@@ -760,9 +855,11 @@ static int frontier_loop(gphi_graph *gr, uint32_t src, gphi_stats *st)
 		}
 		GPHI_HIP(ctx, hipGetLastError());
 		if (ev_step) GPHI_HIP(ctx, hipEventRecord(ev[2 + 2 * steps], s));
-		frontier_update_kernel<kBfs><<<ugrid, kBlock, 0, s>>>(
-		    G.n, G.W, threshold, L + 1, gr->flag, G.deg, gr->level, gr->visited, gr->cur,
-		    gr->row_stamp, gr->col_unvis, gr->queue, gr->qoff, gr->tile_lo, gr->ctrl);
+		frontier_mark_kernel<kBfs><<<nblk, kBlock, 0, s>>>(G.W, L + 1, gr->flag, G.deg, gr->level, gr->visited,
+		                                                  gr->cur, gr->row_stamp, gr->col_unvis, gr->blk_cnt);
+		frontier_scan_kernel<<<1, kScanThreads, 0, s>>>(nblk, threshold, gr->blk_cnt, gr->blk_base, gr->ctrl);
+		frontier_emit_kernel<<<nblk, kBlock, 0, s>>>(G.deg, gr->cur, gr->blk_cnt, gr->blk_base, gr->queue,
+		                                            gr->qoff, gr->tile_lo);
 		GPHI_HIP(ctx, hipGetLastError());
 		if (ev_step) GPHI_HIP(ctx, hipEventRecord(ev[3 + 2 * steps], s));
 		GPHI_HIP(ctx, hipMemcpyAsync(hc, gr->ctrl, sizeof(StepCtrl), hipMemcpyDeviceToHost, s));

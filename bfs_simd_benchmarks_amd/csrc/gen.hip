@@ -490,7 +490,7 @@ int generate_graph(gphi_ctx *ctx, const gphi_rmat_spec *sp, uint32_t W, uint32_t
 	G.side = side;
 	G.ntiles = (uint64_t)side * side;
 	G.tile_order = tile_order;
-	G.n_pad = (uint32_t)(((uint64_t)n + 1 + kBlock - 1) / kBlock * kBlock);
+	G.n_pad = (uint32_t)(((uint64_t)n + 1 + kPadVerts - 1) / kPadVerts * kPadVerts);
 	G.n_words = G.n_pad / 32;
 	G.nchunks = (m + kChunk - 1) / kChunk;
 	G.m_pad = G.nchunks * kChunk;

@@ -18,6 +18,7 @@ namespace gphi {
 constexpr uint32_t kChunk = 2048;
 constexpr uint32_t kBlock = 256;
 constexpr uint32_t kSparseTile = 1024;  // edges per block iteration of the CSR push
+constexpr uint32_t kPadVerts = kBlock * 32;  // vertices per frontier-update block
 constexpr int kNumXcd = 8;              // MI355X: 8 XCDs, blocks dispatched round-robin
 
 struct Error {
@@ -54,7 +55,7 @@ struct DevGraph {
 	uint64_t ntiles = 0;
 	int tile_order = GPHI_TILES_COL_STRIPE;
 	uint32_t n_words = 0;      // 32-bit words of a vertex bitmap (covers id n)
-	uint32_t n_pad = 0;        // padded vertex count (multiple of kBlock, > n)
+	uint32_t n_pad = 0;        // padded vertex count (multiple of 32*kBlock, > n)
 	uint64_t m_pad = 0;        // tile arrays padded to whole chunks
 	uint64_t nchunks = 0;
 	bool has_csr = false, has_tiles = false, weighted = false;
@@ -96,6 +97,8 @@ struct gphi_graph {
 	uint32_t *row_stamp = nullptr, *col_unvis = nullptr;
 	uint32_t *queue = nullptr, *qoff = nullptr, *tile_lo = nullptr;
 	gphi::StepCtrl *ctrl = nullptr;
+	uint4 *blk_cnt = nullptr;  // per update block: |F|, queue entries, degree sum
+	uint2 *blk_base = nullptr; // per update block: queue and degree prefix
 	float *rank = nullptr, *sum = nullptr, *contrib = nullptr;
 };
 
