@@ -333,7 +333,7 @@ void free_graph_device(gphi_graph *gr)
 	void *ps[] = {G.row_off, G.cols, G.csr_w, G.deg, G.heads, G.tails, G.coo_w, G.meta,
 	              gr->level, gr->parent, gr->dist, gr->flag, gr->visited, gr->cur,
 	              gr->row_stamp, gr->col_unvis, gr->queue, gr->qoff, gr->ctrl, gr->rank,
-	              gr->sum, gr->contrib, gr->tile_lo, gr->blk_cnt, gr->blk_base};
+	              gr->sum, gr->contrib, gr->tile_lo, gr->blk_cnt, gr->blk_base, gr->live, gr->live_cnt};
 	for (void *p : ps)
 		if (p) (void)hipFree(p);
 }
@@ -359,6 +359,9 @@ static int ensure_state(gphi_graph *gr)
 	if ((rc = dmalloc(ctx, (void **)&gr->qoff, nb, nullptr))) return rc;
 	if ((rc = dmalloc(ctx, (void **)&gr->tile_lo, (G.m / kSparseTile + 2) * 4, nullptr))) return rc;
 	if ((rc = dmalloc(ctx, (void **)&gr->ctrl, sizeof(StepCtrl), nullptr))) return rc;
+	const size_t list_cap = (G.nchunks + kNumXcd - 1) / kNumXcd;
+	if ((rc = dmalloc(ctx, (void **)&gr->live, list_cap * kNumXcd * 4 + 4, nullptr))) return rc;
+	if ((rc = dmalloc(ctx, (void **)&gr->live_cnt, kNumXcd * 4, nullptr))) return rc;
 	const size_t nblk = G.n_pad / kPadVerts;
 	if ((rc = dmalloc(ctx, (void **)&gr->blk_cnt, nblk * sizeof(uint4), nullptr))) return rc;
 	if ((rc = dmalloc(ctx, (void **)&gr->blk_base, nblk * sizeof(uint2), nullptr))) return rc;
@@ -418,79 +421,144 @@ __global__ void init_source_kernel(uint32_t src, uint32_t W, const uint32_t *__r
 // ---------------------------------------------------------------------------
 // dense step: chunked tile sweep
 // ---------------------------------------------------------------------------
-// Whole-chunk skip (uniform per wave): no head row of the chunk holds a
-// frontier vertex (the reference's is_active_side test, bfs.cpp:275), or every
-// tail column of the chunk is fully visited (BFS only).
+// Chunk selection (one lane per chunk): a chunk is live when a head row of
+// it holds a frontier vertex (the reference's is_active_side test,
+// bfs.cpp:275) and, for BFS, a tail column of it still has an unvisited
+// vertex.  Live chunks are appended to one list per XCD range of chunks, so
+// the sweep's blocks on one XCD walk one contiguous region of the tile arrays
+// and that XCD's L2 keeps the bitmap windows of its stripes.
 template <bool kColSkip>
-__device__ __forceinline__ bool chunk_live(const ChunkMeta &md, const uint32_t *__restrict__ row_stamp,
-                                           const uint32_t *__restrict__ col_unvis, uint32_t level)
+__global__ __launch_bounds__(kBlock) void chunk_select_kernel(
+    const ChunkMeta *__restrict__ meta, uint64_t nchunks, const uint32_t *__restrict__ row_stamp,
+    const uint32_t *__restrict__ col_unvis, uint32_t level, uint32_t *__restrict__ lists,
+    uint32_t list_cap, uint32_t *__restrict__ counts)
 {
-	if (md.row_lo > md.row_hi) return false;  // sentinel-only chunk
-	uint32_t lane = lane_id();
-	bool any_row;
-	uint32_t nr = md.row_hi - md.row_lo + 1;
-	if (nr > 64) {
-		any_row = true;
-	} else {
-		bool a = lane < nr && row_stamp[md.row_lo + lane] == level;
-		any_row = __ballot(a) != 0;
-	}
-	if (!any_row) return false;
-	if (kColSkip) {
-		uint32_t nc = md.col_hi - md.col_lo + 1;
-		if (nc <= 64) {
-			bool a = lane < nc && col_unvis[md.col_lo + lane] != 0;
-			if (__ballot(a) == 0) return false;
+	const uint64_t c = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+	bool live = false;
+	if (c < nchunks) {
+		const ChunkMeta md = meta[c];
+		if (md.row_lo <= md.row_hi) {
+			if (md.row_hi - md.row_lo >= 64) {
+				live = true;
+			} else {
+				for (uint32_t r = md.row_lo; r <= md.row_hi && !live; ++r) live = row_stamp[r] == level;
+			}
+			if (kColSkip && live && md.col_hi - md.col_lo < 64) {
+				bool any = false;
+				for (uint32_t k = md.col_lo; k <= md.col_hi && !any; ++k) any = col_unvis[k] != 0;
+				live = any;
+			}
 		}
 	}
-	return true;
+	const uint64_t cpx = (nchunks + kNumXcd - 1) / kNumXcd;
+	const uint32_t x = c < nchunks ? (uint32_t)(c / cpx) : 0;
+	for (uint32_t xx = 0; xx < (uint32_t)kNumXcd; ++xx) {
+		const bool mine = live && x == xx;
+		const unsigned long long m = __ballot(mine);
+		if (!m) continue;
+		uint32_t base = 0;
+		const uint32_t leader = (uint32_t)__ffsll((long long)m) - 1;
+		if (lane_id() == leader) base = atomicAdd(counts + xx, (uint32_t)__popcll(m));
+		base = __shfl(base, leader, 64);
+		if (mine) lists[(uint64_t)xx * list_cap + base + __popcll(m & ((1ull << lane_id()) - 1ull))] = (uint32_t)c;
+	}
 }
 
-// XCD-aware chunk walk: blocks of one XCD (blockIdx % 8) sweep one contiguous
-// eighth of the chunks so that XCD's L2 keeps the bitmap windows of its stripes.
+// Walk of this XCD's live-chunk list, two chunks per iteration so each lane
+// has 16 edges of loads in flight.
+#define GPHI_FOR_LIVE(ca, cb, has_b, lists, list_cap, counts)                      \
+	const uint32_t xcd_ = blockIdx.x % kNumXcd, slot_ = blockIdx.x / kNumXcd;        \
+	const uint32_t per_ = gridDim.x / kNumXcd;                                      \
+	const uint32_t cnt_ = counts[xcd_];                                             \
+	const uint32_t *lst_ = lists + (uint64_t)xcd_ * (list_cap);                     \
+	for (uint32_t i_ = slot_; i_ < cnt_; i_ += 2 * per_)                            \
+		if (const uint32_t ca = lst_[i_], has_b = i_ + per_ < cnt_, cb = has_b ? lst_[i_ + per_] : ca; true)
+
+typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
+// streaming (non-temporal) 16-byte load: the tile arrays are read once per
+// sweep and should not evict the bitmaps and state arrays from L2 / MALL
+__device__ __forceinline__ uint4 ld_stream(const uint4 *p)
+{
+	const u32x4_t v = __builtin_nontemporal_load(reinterpret_cast<const u32x4_t *>(p));
+	return make_uint4(v.x, v.y, v.z, v.w);
+}
+
+// 8 consecutive-in-memory edges of one lane: frontier bit of the head, then
+// visited bit and flag of the tail; the winner writes parent and flag.
+__device__ __forceinline__ void bfs_visit8(const uint32_t (&h)[8], const uint32_t (&t)[8],
+                                           const uint32_t *__restrict__ cur,
+                                           const uint32_t *__restrict__ visited,
+                                           uint8_t *__restrict__ flag, int32_t *__restrict__ parent)
+{
+	uint32_t cw[8];
+#pragma unroll
+	for (int j = 0; j < 8; ++j) cw[j] = cur[h[j] >> 5];
+	uint32_t hit = 0;
+#pragma unroll
+	for (int j = 0; j < 8; ++j) hit |= ((cw[j] >> (h[j] & 31)) & 1u) << j;
+	if (!hit) return;
+	uint32_t vw[8];
+#pragma unroll
+	for (int j = 0; j < 8; ++j) vw[j] = (hit >> j & 1u) ? visited[t[j] >> 5] : 0xFFFFFFFFu;
+	uint32_t cand = 0;
+#pragma unroll
+	for (int j = 0; j < 8; ++j) cand |= (((vw[j] >> (t[j] & 31)) & 1u) ^ 1u) << j;
+	cand &= hit;
+	if (!cand) return;
+	uint8_t fl[8];
+#pragma unroll
+	for (int j = 0; j < 8; ++j) fl[j] = (cand >> j & 1u) ? flag[t[j]] : 1;
+#pragma unroll
+	for (int j = 0; j < 8; ++j)
+		if (!fl[j]) {
+			parent[t[j]] = (int32_t)h[j];
+			flag[t[j]] = 1;
+		}
+}
+
+__global__ __launch_bounds__(kBlock) void bfs_dense_kernel(
+    const uint32_t *__restrict__ heads, const uint32_t *__restrict__ tails,
+    const uint32_t *__restrict__ lists, uint32_t list_cap, const uint32_t *__restrict__ counts,
+    const uint32_t *__restrict__ cur, const uint32_t *__restrict__ visited,
+    uint8_t *__restrict__ flag, int32_t *__restrict__ parent)
+{
+	GPHI_FOR_LIVE(ca, cb, has_b, lists, list_cap, counts)
+	{
+		const uint4 *Ha = reinterpret_cast<const uint4 *>(heads + (uint64_t)ca * kChunk);
+		const uint4 *Ta = reinterpret_cast<const uint4 *>(tails + (uint64_t)ca * kChunk);
+		const uint4 *Hb = reinterpret_cast<const uint4 *>(heads + (uint64_t)cb * kChunk);
+		const uint4 *Tb = reinterpret_cast<const uint4 *>(tails + (uint64_t)cb * kChunk);
+		const uint4 a0 = ld_stream(Ha + threadIdx.x), a1 = ld_stream(Ha + threadIdx.x + kBlock);
+		const uint4 b0 = ld_stream(Ta + threadIdx.x), b1 = ld_stream(Ta + threadIdx.x + kBlock);
+		uint4 c0 = make_uint4(0, 0, 0, 0), c1 = c0, d0 = c0, d1 = c0;
+		if (has_b) {
+			c0 = ld_stream(Hb + threadIdx.x);
+			c1 = ld_stream(Hb + threadIdx.x + kBlock);
+			d0 = ld_stream(Tb + threadIdx.x);
+			d1 = ld_stream(Tb + threadIdx.x + kBlock);
+		}
+		{
+			const uint32_t h[8] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w};
+			const uint32_t t[8] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w};
+			bfs_visit8(h, t, cur, visited, flag, parent);
+		}
+		if (has_b) {
+			const uint32_t h[8] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
+			const uint32_t t[8] = {d0.x, d0.y, d0.z, d0.w, d1.x, d1.y, d1.z, d1.w};
+			bfs_visit8(h, t, cur, visited, flag, parent);
+		}
+	}
+}
+static_assert(kChunk == 2 * 4 * kBlock, "dense sweep loads two uint4 per array per thread");
+
+// XCD-aware walk over all chunks (PageRank and hop relaxation sweep every edge):
+// blocks of one XCD (blockIdx % 8) sweep one contiguous eighth of the chunks.
 #define GPHI_FOR_CHUNKS(c, nchunks)                                              \
 	const uint32_t xcd_ = blockIdx.x % kNumXcd, slot_ = blockIdx.x / kNumXcd;   \
 	const uint32_t per_ = gridDim.x / kNumXcd;                                 \
 	const uint64_t cpx_ = ((nchunks) + kNumXcd - 1) / kNumXcd;                 \
 	const uint64_t cend_ = min((uint64_t)(nchunks), (xcd_ + 1) * cpx_);        \
 	for (uint64_t c = xcd_ * cpx_ + slot_; c < cend_; c += per_)
-
-__device__ __forceinline__ void bfs_visit(uint32_t h, uint32_t t, const uint32_t *__restrict__ cur,
-                                          const uint32_t *__restrict__ visited,
-                                          uint8_t *__restrict__ flag, int32_t *__restrict__ parent)
-{
-	if (test_bit(cur, h) && !test_bit(visited, t) && !flag[t]) {
-		parent[t] = (int32_t)h;
-		flag[t] = 1;
-	}
-}
-
-__global__ __launch_bounds__(kBlock) void bfs_dense_kernel(
-    const uint32_t *__restrict__ heads, const uint32_t *__restrict__ tails,
-    const ChunkMeta *__restrict__ meta, uint64_t nchunks, const uint32_t *__restrict__ cur,
-    const uint32_t *__restrict__ visited, const uint32_t *__restrict__ row_stamp,
-    const uint32_t *__restrict__ col_unvis, uint8_t *__restrict__ flag,
-    int32_t *__restrict__ parent, uint32_t level)
-{
-	GPHI_FOR_CHUNKS(c, nchunks)
-	{
-		ChunkMeta md = meta[c];
-		if (!chunk_live<true>(md, row_stamp, col_unvis, level)) continue;
-		const uint4 *H = reinterpret_cast<const uint4 *>(heads + c * kChunk);
-		const uint4 *T = reinterpret_cast<const uint4 *>(tails + c * kChunk);
-		uint4 h0 = H[threadIdx.x], h1 = H[threadIdx.x + kBlock];
-		uint4 t0 = T[threadIdx.x], t1 = T[threadIdx.x + kBlock];
-		bfs_visit(h0.x, t0.x, cur, visited, flag, parent);
-		bfs_visit(h0.y, t0.y, cur, visited, flag, parent);
-		bfs_visit(h0.z, t0.z, cur, visited, flag, parent);
-		bfs_visit(h0.w, t0.w, cur, visited, flag, parent);
-		bfs_visit(h1.x, t1.x, cur, visited, flag, parent);
-		bfs_visit(h1.y, t1.y, cur, visited, flag, parent);
-		bfs_visit(h1.z, t1.z, cur, visited, flag, parent);
-		bfs_visit(h1.w, t1.w, cur, visited, flag, parent);
-	}
-}
-static_assert(kChunk == 2 * 4 * kBlock, "dense sweep loads two uint4 per array per thread");
 
 // SSSP relax (apps/sssp_simd/sssp.cpp:344-445): if mask[h] and dist[h]+w <
 // dist[t] then lower dist[t] and mark t for the next frontier.
@@ -508,28 +576,30 @@ __device__ __forceinline__ void sssp_relax(uint32_t h, uint32_t t, uint32_t w,
 
 __global__ __launch_bounds__(kBlock) void sssp_dense_kernel(
     const uint32_t *__restrict__ heads, const uint32_t *__restrict__ tails,
-    const uint32_t *__restrict__ wts, const ChunkMeta *__restrict__ meta, uint64_t nchunks,
-    const uint32_t *__restrict__ cur, const uint32_t *__restrict__ row_stamp, uint32_t *dist,
-    uint8_t *__restrict__ flag, uint32_t level)
+    const uint32_t *__restrict__ wts, const uint32_t *__restrict__ lists, uint32_t list_cap,
+    const uint32_t *__restrict__ counts, const uint32_t *__restrict__ cur, uint32_t *dist,
+    uint8_t *__restrict__ flag)
 {
-	GPHI_FOR_CHUNKS(c, nchunks)
+	GPHI_FOR_LIVE(ca, cb, has_b, lists, list_cap, counts)
 	{
-		ChunkMeta md = meta[c];
-		if (!chunk_live<false>(md, row_stamp, nullptr, level)) continue;
-		const uint4 *H = reinterpret_cast<const uint4 *>(heads + c * kChunk);
-		const uint4 *T = reinterpret_cast<const uint4 *>(tails + c * kChunk);
-		const uint4 *Wt = reinterpret_cast<const uint4 *>(wts + c * kChunk);
-		uint4 h0 = H[threadIdx.x], h1 = H[threadIdx.x + kBlock];
-		uint4 t0 = T[threadIdx.x], t1 = T[threadIdx.x + kBlock];
-		uint4 w0 = Wt[threadIdx.x], w1 = Wt[threadIdx.x + kBlock];
-		sssp_relax(h0.x, t0.x, w0.x, cur, dist, flag);
-		sssp_relax(h0.y, t0.y, w0.y, cur, dist, flag);
-		sssp_relax(h0.z, t0.z, w0.z, cur, dist, flag);
-		sssp_relax(h0.w, t0.w, w0.w, cur, dist, flag);
-		sssp_relax(h1.x, t1.x, w1.x, cur, dist, flag);
-		sssp_relax(h1.y, t1.y, w1.y, cur, dist, flag);
-		sssp_relax(h1.z, t1.z, w1.z, cur, dist, flag);
-		sssp_relax(h1.w, t1.w, w1.w, cur, dist, flag);
+		for (int q = 0; q < 2; ++q) {
+			if (q == 1 && !has_b) break;
+			const uint64_t c = q ? cb : ca;
+			const uint4 *H = reinterpret_cast<const uint4 *>(heads + c * kChunk);
+			const uint4 *T = reinterpret_cast<const uint4 *>(tails + c * kChunk);
+			const uint4 *Wt = reinterpret_cast<const uint4 *>(wts + c * kChunk);
+			const uint4 h0 = ld_stream(H + threadIdx.x), h1 = ld_stream(H + threadIdx.x + kBlock);
+			const uint4 t0 = ld_stream(T + threadIdx.x), t1 = ld_stream(T + threadIdx.x + kBlock);
+			const uint4 w0 = ld_stream(Wt + threadIdx.x), w1 = ld_stream(Wt + threadIdx.x + kBlock);
+			sssp_relax(h0.x, t0.x, w0.x, cur, dist, flag);
+			sssp_relax(h0.y, t0.y, w0.y, cur, dist, flag);
+			sssp_relax(h0.z, t0.z, w0.z, cur, dist, flag);
+			sssp_relax(h0.w, t0.w, w0.w, cur, dist, flag);
+			sssp_relax(h1.x, t1.x, w1.x, cur, dist, flag);
+			sssp_relax(h1.y, t1.y, w1.y, cur, dist, flag);
+			sssp_relax(h1.z, t1.z, w1.z, cur, dist, flag);
+			sssp_relax(h1.w, t1.w, w1.w, cur, dist, flag);
+		}
 	}
 }
 
@@ -631,13 +701,24 @@ __global__ __launch_bounds__(kBlock) void frontier_mark_kernel(
 		if (kBfs) visited[w] |= bits;
 		if (row_uniform) row_stamp[v0 / W] = next_level;
 		F = (uint32_t)__popc(bits);
-		for (uint32_t b = bits; b; b &= b - 1) {
-			const uint32_t v = v0 + (uint32_t)__ffs(b) - 1;
-			if (kBfs) level[v] = (int32_t)next_level;
-			if (!row_uniform) row_stamp[v / W] = next_level;
-			const uint32_t d = deg[v];
-			D += d;
-			Q += d > 0;
+		// the word's 32 degrees in one 128-byte line (deg is padded to n_pad)
+		const uint4 *dp = reinterpret_cast<const uint4 *>(deg + v0);
+		uint4 dq[8];
+#pragma unroll
+		for (int j = 0; j < 8; ++j) dq[j] = dp[j];
+#pragma unroll
+		for (int j = 0; j < 8; ++j) {
+			const uint32_t dd[4] = {dq[j].x, dq[j].y, dq[j].z, dq[j].w};
+#pragma unroll
+			for (int k = 0; k < 4; ++k) {
+				const uint32_t i = 4 * j + k;
+				if (bits >> i & 1u) {
+					if (kBfs) level[v0 + i] = (int32_t)next_level;
+					if (!row_uniform) row_stamp[(v0 + i) / W] = next_level;
+					D += dd[k];
+					Q += dd[k] > 0;
+				}
+			}
 		}
 	}
 	if (kBfs) {
@@ -743,10 +824,23 @@ __global__ __launch_bounds__(kBlock) void frontier_emit_kernel(
 	const uint32_t v0 = w * 32;
 	const uint32_t bits = cur[w];
 	uint32_t q = 0, d = 0;
-	for (uint32_t b = bits; b; b &= b - 1) {
-		const uint32_t dv = deg[v0 + __ffs(b) - 1];
-		q += dv > 0;
-		d += dv;
+	uint32_t dd[32];
+	if (bits) {
+		const uint4 *dp = reinterpret_cast<const uint4 *>(deg + v0);
+#pragma unroll
+		for (int j = 0; j < 8; ++j) {
+			const uint4 x = dp[j];
+			dd[4 * j] = x.x;
+			dd[4 * j + 1] = x.y;
+			dd[4 * j + 2] = x.z;
+			dd[4 * j + 3] = x.w;
+		}
+#pragma unroll
+		for (int i = 0; i < 32; ++i) {
+			const uint32_t dv = (bits >> i & 1u) ? dd[i] : 0u;
+			q += dv > 0;
+			d += dv;
+		}
 	}
 	uint32_t qt, dt;
 	uint32_t qx = wave_excl_scan(q, &qt), dx = wave_excl_scan(d, &dt);
@@ -762,11 +856,12 @@ __global__ __launch_bounds__(kBlock) void frontier_emit_kernel(
 	}
 	const uint2 base = blk_base[blockIdx.x];
 	uint32_t pos = base.x + qx, off = base.y + dx;
-	for (uint32_t b = bits; b; b &= b - 1) {
-		const uint32_t v = v0 + __ffs(b) - 1;
-		const uint32_t dv = deg[v];
+#pragma unroll
+	for (int i = 0; i < 32; ++i) {
+		if (!(bits >> i & 1u)) continue;
+		const uint32_t dv = dd[i];
 		if (!dv) continue;
-		queue[pos] = v;
+		queue[pos] = v0 + i;
 		qoff[pos] = off;
 		// tiles of the next push whose first edge falls in [off, off+dv)
 		for (uint32_t k = (off + kSparseTile - 1) / kSparseTile; (uint64_t)k * kSparseTile < (uint64_t)off + dv; ++k)
@@ -843,15 +938,25 @@ static int frontier_loop(gphi_graph *gr, uint32_t src, gphi_stats *st)
 				    gr->visited, gr->flag, gr->parent, gr->dist);
 		} else {
 			if (!G.has_tiles) return set_error(ctx, GPHI_E_STATE, "graph has no tile arrays");
+			const uint32_t list_cap = (uint32_t)((G.nchunks + kNumXcd - 1) / kNumXcd);
+			GPHI_HIP(ctx, hipMemsetAsync(gr->live_cnt, 0, kNumXcd * 4, s));
+			const unsigned sgrid = (unsigned)std::max<uint64_t>(1, (G.nchunks + kBlock - 1) / kBlock);
+			if (kBfs)
+				chunk_select_kernel<true><<<sgrid, kBlock, 0, s>>>(G.meta, G.nchunks, gr->row_stamp,
+				                                                 gr->col_unvis, L, gr->live, list_cap,
+				                                                 gr->live_cnt);
+			else
+				chunk_select_kernel<false><<<sgrid, kBlock, 0, s>>>(G.meta, G.nchunks, gr->row_stamp,
+				                                                  gr->col_unvis, L, gr->live, list_cap,
+				                                                  gr->live_cnt);
 			unsigned grid = (unsigned)std::min<uint64_t>(kDenseGrid, (G.nchunks + kNumXcd - 1) / kNumXcd * kNumXcd);
 			grid = std::max<unsigned>(grid / kNumXcd * kNumXcd, kNumXcd);
 			if (kBfs)
-				bfs_dense_kernel<<<grid, kBlock, 0, s>>>(G.heads, G.tails, G.meta, G.nchunks, gr->cur,
-				                                       gr->visited, gr->row_stamp, gr->col_unvis,
-				                                       gr->flag, gr->parent, L);
+				bfs_dense_kernel<<<grid, kBlock, 0, s>>>(G.heads, G.tails, gr->live, list_cap, gr->live_cnt,
+				                                       gr->cur, gr->visited, gr->flag, gr->parent);
 			else
-				sssp_dense_kernel<<<grid, kBlock, 0, s>>>(G.heads, G.tails, G.coo_w, G.meta, G.nchunks,
-				                                        gr->cur, gr->row_stamp, gr->dist, gr->flag, L);
+				sssp_dense_kernel<<<grid, kBlock, 0, s>>>(G.heads, G.tails, G.coo_w, gr->live, list_cap,
+				                                        gr->live_cnt, gr->cur, gr->dist, gr->flag);
 		}
 		GPHI_HIP(ctx, hipGetLastError());
 		if (ev_step) GPHI_HIP(ctx, hipEventRecord(ev[2 + 2 * steps], s));
