@@ -46,7 +46,21 @@ def main() -> int:
     ap.add_argument("--S", type=int, default=None, help="stripe length")
     ap.add_argument("--cpu-seconds", type=float, default=20.0,
                     help="budget of the CPU baseline (0 disables it)")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB five times (calibrates the "
+                         "FETCH_SIZE / WRITE_SIZE counters against known bytes)")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+        n = args.copy_probe * (1 << 20) // 4
+        src = torch.arange(n, dtype=torch.int32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_read": n * 4, "copies": 5,
+                          "ok": bool(torch.equal(src, dst))}))
+        return 0
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:

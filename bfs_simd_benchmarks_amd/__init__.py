@@ -113,6 +113,11 @@ def _declare(lib):
                                       i32, C.POINTER(_P)]),
         "gphi_graph_download": (i32, [_P, _P, _P, _P, _P, _P, _P]),
         "gphi_graph_tile_offsets": (i32, [_P, _P]),
+        "gphi_host_read_text": (i32, [C.c_char_p, u32, u32, i32, i32, C.POINTER(_P)]),
+        "gphi_host_from_device": (i32, [_P, C.POINTER(_P)]),
+        "gphi_host_write_text": (i32, [_P, C.c_char_p]),
+        "gphi_host_desc": (i32, [_P, C.POINTER(GraphDesc)]),
+        "gphi_host_free": (None, [_P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -141,6 +146,56 @@ def _ptr(a: np.ndarray | None):
 
 def _u32(a) -> np.ndarray:
     return np.ascontiguousarray(a, dtype=np.uint32)
+
+
+def _host_check(lib, rc):
+    if rc:
+        raise GphiError(rc, lib.gphi_last_error(None).decode())
+
+
+class HostGraph:
+    """graph_input's arrays in host memory (the reference's text formats, no device)."""
+
+    def __init__(self, h):
+        self.lib = load_library()
+        self.h = h
+        self.desc = GraphDesc()
+        _host_check(self.lib, self.lib.gphi_host_desc(h, C.byref(self.desc)))
+
+    @classmethod
+    def read(cls, prefix, W: int, S: int, weighted: bool = False, row_major: bool = False):
+        lib = load_library()
+        h = _P()
+        _host_check(lib, lib.gphi_host_read_text(str(prefix).encode(), W, S, int(weighted),
+                                                  int(row_major), C.byref(h)))
+        return cls(h)
+
+    def write(self, prefix):
+        _host_check(self.lib, self.lib.gphi_host_write_text(self.h, str(prefix).encode()))
+
+    def arrays(self) -> dict:
+        d = self.desc
+        side = d.n // d.tile_width + (1 if d.n % d.tile_width else 0)
+
+        def arr(ptr, count):
+            if not ptr or count == 0:
+                return None
+            return np.ctypeslib.as_array(C.cast(ptr, C.POINTER(C.c_uint32)), shape=(count,)).copy()
+        return {"n": d.n, "m": d.m, "W": d.tile_width, "S": d.stripe_length, "order": d.tile_order,
+                "deg": arr(d.degrees, d.n), "cols": arr(d.csr_edges, d.m), "cw": arr(d.csr_weights, d.m),
+                "heads": arr(d.tile_heads, d.m), "tails": arr(d.tile_tails, d.m),
+                "tw": arr(d.tile_weights, d.m), "offsets": arr(d.tile_offsets, side * side)}
+
+    def free(self):
+        if self.h:
+            self.lib.gphi_host_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 class Context:
@@ -274,6 +329,12 @@ class Graph:
             _ptr(g("tw"))))
         return out
 
+    def to_host(self) -> HostGraph:
+        """Download into a HostGraph (writable as the reference's text files)."""
+        h = _P()
+        self.ctx.check(self.ctx.lib.gphi_host_from_device(self.h, C.byref(h)))
+        return HostGraph(h)
+
     @property
     def side(self) -> int:
         return self.n // self.W + (1 if self.n % self.W else 0)
@@ -291,5 +352,5 @@ class Graph:
         return out[: st.steps]
 
 
-__all__ = ["Context", "Graph", "GphiError", "Stats", "load_library", "TILES_COL_STRIPE",
+__all__ = ["Context", "Graph", "HostGraph", "GphiError", "Stats", "load_library", "TILES_COL_STRIPE",
            "TILES_ROW_MAJOR", "LIB_PATH"]

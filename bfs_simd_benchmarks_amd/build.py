@@ -96,7 +96,7 @@ def build(verbose: bool = False, jobs: int = 8) -> Path:
         path = CLI / src
         if not path.exists():
             continue
-        _run(["g++", "-O2", "-std=c++17", "-fopenmp", f"-I{PKG.parent / 'include'}", str(path),
+        _run(["g++", "-O2", "-std=c++17", "-fopenmp", f"-I{PKG.parent / 'include'}", f"-I{CLI}", str(path),
               "-o", str(BIN / name), f"-L{PKG}", "-lgphi", f"-Wl,-rpath,$ORIGIN/..", *LINK_FLAGS],
              verbose)
     return LIB

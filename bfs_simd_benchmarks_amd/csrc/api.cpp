@@ -309,6 +309,42 @@ int gphi_graph_download(gphi_graph *g, uint32_t *degrees, uint32_t *csr_edges, u
 	return gphi::download_graph(g, degrees, csr_edges, csr_weights, tile_heads, tile_tails, tile_weights);
 }
 
+int gphi_host_from_device(gphi_graph *g, gphi_host_graph **out)
+{
+	if (!out) return set_error(nullptr, GPHI_E_ARG, "out is NULL");
+	*out = nullptr;
+	gphi_graph_desc d;
+	std::vector<uint32_t> st[7];
+	{
+		GPHI_ENTER(g);
+		const gphi::DevGraph &G = g->g;
+		memset(&d, 0, sizeof d);
+		d.n = G.n;
+		d.m = G.m;
+		d.tile_width = G.W;
+		d.stripe_length = G.S;
+		d.tile_order = G.tile_order;
+		try {
+			st[0].resize(G.n);
+			if (G.has_csr) st[1].resize(G.m);
+			if (G.csr_w) st[2].resize(G.m);
+			if (G.has_tiles) {
+				st[3].resize(G.m);
+				st[4].resize(G.m);
+				st[6].resize(G.ntiles);
+			}
+			if (G.coo_w) st[5].resize(G.m);
+		} catch (const std::bad_alloc &) {
+			return set_error(g->ctx, GPHI_E_OOM, "host allocation failed");
+		}
+		auto ptr = [](std::vector<uint32_t> &v) { return v.empty() ? nullptr : v.data(); };
+		int rc = gphi::download_graph(g, ptr(st[0]), ptr(st[1]), ptr(st[2]), ptr(st[3]), ptr(st[4]), ptr(st[5]));
+		if (!rc && G.has_tiles) rc = gphi::tile_offsets(g, st[6].data());
+		if (rc) return rc;
+	}
+	return gphi::host_adopt(&d, st, out);
+}
+
 int gphi_graph_tile_offsets(gphi_graph *g, uint32_t *out)
 {
 	GPHI_ENTER(g);

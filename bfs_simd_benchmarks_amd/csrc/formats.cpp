@@ -7,7 +7,9 @@
 // (k+1)*floor(M/64)), part 63 the rest; ids are 1-based ("h t" or "h t w").
 // Unlike the reference readers (apps/bfs_simd/bfs.cpp:968-1118, which never
 // check fscanf) every count and id is validated.
+#include <algorithm>
 #include <cerrno>
+#include <new>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -230,4 +232,180 @@ int read_text_graph(gphi_ctx *ctx, const char *prefix_c, uint32_t W, uint32_t S,
 	return GPHI_OK;
 }
 
+// ---------------------------------------------------------------------------
+// writers: the exact bytes of the reference tools' fprintf("%u %u\n") lines
+// ---------------------------------------------------------------------------
+namespace {
+
+inline void put_u32(std::string &o, uint32_t x)
+{
+	char b[10];
+	int k = 0;
+	do {
+		b[k++] = (char)('0' + x % 10);
+		x /= 10;
+	} while (x);
+	while (k) o.push_back(b[--k]);
+}
+
+int write_file(const std::string &path, const std::string &data)
+{
+	FILE *f = fopen(path.c_str(), "wb");
+	if (!f) return set_error(nullptr, GPHI_E_IO, "cannot create %s (%s)", path.c_str(), strerror(errno));
+	size_t wr = fwrite(data.data(), 1, data.size(), f);
+	int rc = fclose(f);
+	if (wr != data.size() || rc != 0) return set_error(nullptr, GPHI_E_IO, "short write to %s", path.c_str());
+	return GPHI_OK;
+}
+
+// heads==null: heads are implied by the CSR degrees (the _untiled order)
+int write_parts(const std::string &prefix, uint32_t n, uint64_t m, const uint32_t *deg,
+                const uint32_t *heads, const uint32_t *tails, const uint32_t *wts)
+{
+	const uint64_t eb = m / kParts;
+	std::vector<uint64_t> row;  // CSR row starts, for implied heads
+	if (!heads) {
+		row.resize((size_t)n + 1);
+		row[0] = 0;
+		for (uint32_t v = 0; v < n; ++v) row[v + 1] = row[v] + deg[v];
+	}
+	std::vector<int> codes(kParts, GPHI_OK);
+#pragma omp parallel for schedule(dynamic, 1)
+	for (uint32_t k = 0; k < kParts; ++k) {
+		const uint64_t lo = (uint64_t)k * eb, hi = (k + 1 < kParts) ? lo + eb : m;
+		std::string o;
+		o.reserve((size_t)(hi - lo) * (wts ? 24 : 16) + 32);
+		if (k == 0) {
+			put_u32(o, n);
+			o.push_back(' ');
+			put_u32(o, (uint32_t)m);
+			o.push_back('\n');
+		}
+		uint32_t hv = 0;
+		if (!heads && hi > lo) hv = (uint32_t)(std::upper_bound(row.begin(), row.end(), lo) - row.begin() - 1);
+		for (uint64_t e = lo; e < hi; ++e) {
+			uint32_t h;
+			if (heads) {
+				h = heads[e];
+			} else {
+				while (row[hv + 1] <= e) ++hv;
+				h = hv;
+			}
+			put_u32(o, h + 1);
+			o.push_back(' ');
+			put_u32(o, tails[e] + 1);
+			if (wts) {
+				o.push_back(' ');
+				put_u32(o, wts[e]);
+			}
+			o.push_back('\n');
+		}
+		codes[k] = write_file(prefix + "-" + std::to_string(k), o);
+	}
+	for (int c : codes)
+		if (c) return c;
+	return GPHI_OK;
+}
+
+int write_lines(const std::string &path, const uint32_t *a, uint64_t count)
+{
+	std::string o;
+	o.reserve((size_t)count * 8);
+	for (uint64_t i = 0; i < count; ++i) {
+		put_u32(o, a[i]);
+		o.push_back('\n');
+	}
+	return write_file(path, o);
+}
+
+}  // namespace
+
 }  // namespace gphi
+
+struct gphi_host_graph {
+	gphi_graph_desc d;
+	std::vector<uint32_t> store[8];
+};
+
+namespace gphi {
+int host_adopt(const gphi_graph_desc *d, std::vector<uint32_t> *st, gphi_host_graph **out)
+{
+	gphi_host_graph *h = new (std::nothrow) gphi_host_graph();
+	if (!h) return set_error(nullptr, GPHI_E_OOM, "out of host memory");
+	h->d = *d;
+	for (int i = 0; i < 7; ++i) h->store[i].swap(st[i]);
+	auto ptr = [&](int i) -> const uint32_t * { return h->store[i].empty() ? nullptr : h->store[i].data(); };
+	h->d.degrees = ptr(0);
+	h->d.csr_edges = ptr(1);
+	h->d.csr_weights = ptr(2);
+	h->d.tile_heads = ptr(3);
+	h->d.tile_tails = ptr(4);
+	h->d.tile_weights = ptr(5);
+	h->d.tile_offsets = ptr(6);
+	if (d->m == 0) {  // empty edge arrays are still present
+		static const uint32_t none = 0;
+		if (d->csr_edges) h->d.csr_edges = &none;
+		if (d->tile_heads) h->d.tile_heads = h->d.tile_tails = &none;
+	}
+	*out = h;
+	return GPHI_OK;
+}
+}  // namespace gphi
+
+using gphi::set_error;
+
+extern "C" {
+
+int gphi_host_read_text(const char *prefix, uint32_t W, uint32_t S, int weighted, int row_major,
+                        gphi_host_graph **out)
+{
+	if (!prefix || !out) return set_error(nullptr, GPHI_E_ARG, "NULL argument");
+	*out = nullptr;
+	if (W == 0 || S == 0) return set_error(nullptr, GPHI_E_ARG, "tile width and stripe length must be > 0");
+	gphi_host_graph *h = new (std::nothrow) gphi_host_graph();
+	if (!h) return set_error(nullptr, GPHI_E_OOM, "out of host memory");
+	int rc = gphi::read_text_graph(nullptr, prefix, W, S, weighted, row_major != 0, &h->d, h->store);
+	if (rc) {
+		delete h;
+		return rc;
+	}
+	*out = h;
+	return GPHI_OK;
+}
+
+int gphi_host_desc(const gphi_host_graph *h, gphi_graph_desc *out)
+{
+	if (!h || !out) return set_error(nullptr, GPHI_E_ARG, "NULL argument");
+	*out = h->d;
+	return GPHI_OK;
+}
+
+void gphi_host_free(gphi_host_graph *h) { delete h; }
+
+int gphi_host_write_text(const gphi_host_graph *h, const char *prefix_c)
+{
+	if (!h || !prefix_c) return set_error(nullptr, GPHI_E_ARG, "NULL argument");
+	const gphi_graph_desc &d = h->d;
+	const std::string prefix(prefix_c);
+	int rc;
+	if (d.csr_edges) {
+		const std::string p = prefix + "_untiled";
+		if ((rc = gphi::write_parts(p, d.n, d.m, d.degrees, nullptr, d.csr_edges, d.csr_weights))) return rc;
+		if ((rc = gphi::write_lines(p + "-nneibor", d.degrees, d.n))) return rc;
+	}
+	if (d.tile_heads && d.tile_tails) {
+		if (!d.tile_offsets) return set_error(nullptr, GPHI_E_STATE, "tile offsets are missing");
+		const std::string p =
+		    d.tile_order == GPHI_TILES_ROW_MAJOR
+		        ? prefix + "_coo-tiled-" + std::to_string(d.tile_width)
+		        : prefix + "_col-" + std::to_string(d.stripe_length) + "-coo-tiled-" + std::to_string(d.tile_width);
+		const uint64_t side = d.n % d.tile_width ? d.n / d.tile_width + 1 : d.n / d.tile_width;
+		if ((rc = gphi::write_parts(p, d.n, d.m, d.degrees, d.tile_heads, d.tile_tails, d.tile_weights)))
+			return rc;
+		if ((rc = gphi::write_lines(p + "-offsets", d.tile_offsets, side * side))) return rc;
+		if ((rc = gphi::write_lines(p + "-nneibor", d.degrees, d.n))) return rc;
+	}
+	return GPHI_OK;
+}
+
+}  // extern "C"

@@ -132,6 +132,8 @@ void glibc_rand_seq(uint32_t seed, uint64_t n, int32_t *out);
 void random_perm_host(uint32_t n, uint64_t seed, uint32_t *perm);
 
 // host text formats (formats.cpp)
+// moves st[0..6] (degrees, csr, csr w, heads, tails, tile w, offsets) into a new host graph
+int host_adopt(const gphi_graph_desc *d, std::vector<uint32_t> *st, gphi_host_graph **out);
 int read_text_graph(gphi_ctx *ctx, const char *prefix, uint32_t W, uint32_t S, int weighted,
                     bool row_major, gphi_graph_desc *d, std::vector<uint32_t> *store);
 }  // namespace gphi

@@ -157,6 +157,24 @@ int gphi_graph_download(gphi_graph *g, uint32_t *degrees, uint32_t *csr_edges,
                         uint32_t *csr_weights, uint32_t *tile_heads, uint32_t *tile_tails,
                         uint32_t *tile_weights);
 
+/* ---- host-side graphs: the reference's text formats without a device ----
+ * A gphi_host_graph holds graph_input's arrays in host memory.  Reading
+ * validates every count and id (the reference readers never check fscanf,
+ * apps/bfs_simd/bfs.cpp:1046-1051); writing produces the exact bytes of the
+ * reference tools (tools/csr_tiling/page_rank.cpp:570-598,
+ * tools/coo_tiling/page_rank.cpp:450-499, tools/column_major_tile/kcore.cpp:419-450).
+ * Errors are reported through gphi_last_error(NULL). */
+typedef struct gphi_host_graph gphi_host_graph;
+/* row_major = 0: prefix_col-S-coo-tiled-W + prefix_untiled; 1: prefix_coo-tiled-W */
+int gphi_host_read_text(const char *prefix, uint32_t W, uint32_t S, int weighted, int row_major,
+                        gphi_host_graph **out);
+int gphi_host_from_device(gphi_graph *g, gphi_host_graph **out);
+/* Writes prefix_untiled-* (if it holds the CSR) and its tile files
+ * (prefix_col-S-coo-tiled-W-* or prefix_coo-tiled-W-*, with -offsets, -nneibor). */
+int gphi_host_write_text(const gphi_host_graph *h, const char *prefix);
+int gphi_host_desc(const gphi_host_graph *h, gphi_graph_desc *out);
+void gphi_host_free(gphi_host_graph *h);
+
 /* Exclusive tile offsets in the stored tile order (the -offsets file);
  * out has SIDE^2 entries. */
 int gphi_graph_tile_offsets(gphi_graph *g, uint32_t *out);
