@@ -48,7 +48,8 @@ int main(int argc, char **argv)
 		} else {
 			float ms = 0;
 			uint32_t sweeps = 0;
-			if ((rc = gphi_bellman_ford(g, a.source, 0xFFFFFFFFu, outp, &sweeps, &ms)))
+			// hop distances reach their fixpoint within n sweeps
+			if ((rc = gphi_bellman_ford(g, a.source, n + 1, outp, &sweeps, &ms)))
 				return cli::die(ctx, "gphi_bellman_ford", rc);
 			tm.record(ms * 1e-3);
 		}

@@ -1161,26 +1161,33 @@ int run_bellman_ford(gphi_graph *gr, uint32_t src, uint32_t max_sweeps, uint32_t
 	int rc;
 	if (!gr->dist && (rc = dmalloc(ctx, (void **)&gr->dist, (size_t)G.n_pad * 4, nullptr))) return rc;
 	if (!gr->ctrl && (rc = dmalloc(ctx, (void **)&gr->ctrl, sizeof(StepCtrl), nullptr))) return rc;
-	const uint32_t batch = 4;
+	// one "lowered something" counter per sweep of a batch; the batch size,
+	// not max_sweeps, sizes the buffer
+	constexpr uint32_t kBatch = 4;
 	uint32_t *counts = nullptr;
-	if ((rc = dmalloc(ctx, (void **)&counts, (max_sweeps + batch) * 4, nullptr))) return rc;
+	if ((rc = dmalloc(ctx, (void **)&counts, kBatch * 4, nullptr))) return rc;
 	if (ctx->ev_ok) GPHI_HIP(ctx, hipEventRecord(ctx->ev[0], s));
 	fill_u32_kernel<<<grid_for(G.n_pad, kBlock, 4096), kBlock, 0, s>>>(gr->dist, G.n_pad, INT_MAX - 2);
 	uint32_t zero = 0;
 	GPHI_HIP(ctx, hipMemcpyAsync(gr->dist + src, &zero, 4, hipMemcpyHostToDevice, s));
-	GPHI_HIP(ctx, hipMemsetAsync(counts, 0, (max_sweeps + batch) * 4, s));
 	unsigned grid = (unsigned)std::min<uint64_t>(kDenseGrid, (G.nchunks + kNumXcd - 1) / kNumXcd * kNumXcd);
 	grid = std::max<unsigned>(grid / kNumXcd * kNumXcd, kNumXcd);
 	uint32_t done = 0, run = 0;
-	std::vector<uint32_t> hcnt(batch);
+	uint32_t hcnt[kBatch];
+	int err = GPHI_OK;
 	while (run < max_sweeps && !done) {
-		uint32_t nb = std::min(batch, max_sweeps - run);
-		for (uint32_t k = 0; k < nb; ++k)
-			hop_sweep_kernel<<<grid, kBlock, 0, s>>>(G.heads, G.tails, G.nchunks, G.n, gr->dist,
-			                                        counts + run + k);
-		GPHI_HIP(ctx, hipGetLastError());
-		GPHI_HIP(ctx, hipMemcpyAsync(hcnt.data(), counts + run, nb * 4, hipMemcpyDeviceToHost, s));
-		GPHI_HIP(ctx, hipStreamSynchronize(s));
+		const uint32_t nb = std::min<uint32_t>(kBatch, max_sweeps - run);
+		hipError_t e = hipMemsetAsync(counts, 0, kBatch * 4, s);
+		for (uint32_t k = 0; k < nb && e == hipSuccess; ++k) {
+			hop_sweep_kernel<<<grid, kBlock, 0, s>>>(G.heads, G.tails, G.nchunks, G.n, gr->dist, counts + k);
+			e = hipGetLastError();
+		}
+		if (e == hipSuccess) e = hipMemcpyAsync(hcnt, counts, nb * 4, hipMemcpyDeviceToHost, s);
+		if (e == hipSuccess) e = hipStreamSynchronize(s);
+		if (e != hipSuccess) {
+			err = hip_fail(ctx, e, "hop relaxation batch", __FILE__, __LINE__);
+			break;
+		}
 		for (uint32_t k = 0; k < nb; ++k) {
 			++run;
 			if (hcnt[k] == 0) {
@@ -1189,10 +1196,11 @@ int run_bellman_ford(gphi_graph *gr, uint32_t src, uint32_t max_sweeps, uint32_t
 			}
 		}
 	}
+	(void)hipFree(counts);
+	if (err) return err;
 	if (ctx->ev_ok) GPHI_HIP(ctx, hipEventRecord(ctx->ev[1], s));
 	if (dist) GPHI_HIP(ctx, hipMemcpyAsync(dist, gr->dist, (size_t)G.n * 4, hipMemcpyDeviceToHost, s));
 	GPHI_HIP(ctx, hipStreamSynchronize(s));
-	(void)hipFree(counts);
 	if (ms) *ms = ctx->ev_ok ? ev_ms(ctx->ev[0], ctx->ev[1]) : 0.f;
 	if (sweeps_run) *sweeps_run = run;
 	return GPHI_OK;

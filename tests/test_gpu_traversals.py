@@ -137,3 +137,16 @@ def test_bellman_ford_c1(ctx, c1):
     lv = O.bfs_levels(c1.n, c1.deg, c1.cols, 0)
     assert np.array_equal(d, np.where(lv >= 0, lv, O.INT_MAX - 2).astype(np.uint32))
     assert run < 99
+
+
+def test_bellman_ford_unbounded_sweeps(ctx, c1):
+    """max_sweeps = 2^32-1 must not size any buffer (regression: a counter
+    buffer of (max_sweeps + 4) * 4 bytes wrapped in 32-bit arithmetic)."""
+    print("case: bellman-ford c1 max_sweeps=2^32-1", flush=True)
+    gg = ctx.graph_from_arrays(c1.n, c1.deg, W=c1.W, S=c1.S, tile_heads=c1.rh, tile_tails=c1.rt,
+                               tile_offsets=c1.roff, tile_order=1)
+    try:
+        d, run, _ = gg.bellman_ford(0, 0xFFFFFFFF)
+    finally:
+        gg.free()
+    assert np.array_equal(d, O.bellman_ford_hops(c1.n, c1.rh, c1.rt, 0, 99)) and run < 99
