@@ -106,6 +106,7 @@ def _declare(lib):
         "gphi_graph_free": (None, [_P]),
         "gphi_bfs": (i32, [_P, u32, _P, _P, C.POINTER(Stats)]),
         "gphi_sssp": (i32, [_P, u32, _P, C.POINTER(Stats)]),
+        "gphi_bfs_batch": (i32, [_P, _P, u32, _P, C.POINTER(Stats)]),
         "gphi_pagerank": (i32, [_P, u32, C.c_float, C.c_float, _P, _P]),
         "gphi_bellman_ford": (i32, [_P, u32, u32, _P, _P, _P]),
         "gphi_bfs_alg_bytes": (i32, [_P, _P, C.POINTER(Stats), _P]),
@@ -293,6 +294,16 @@ class Graph:
         st = Stats()
         self.ctx.check(self.ctx.lib.gphi_bfs(self.h, src, _ptr(lv), _ptr(pa), C.byref(st)))
         return lv, pa, st
+
+    def bfs_batch(self, srcs, levels: bool = True):
+        """Multi-source BFS of up to 256 sources in one traversal.  Returns
+        (levels uint8[k, n] or None, Stats); 255 = unreached."""
+        srcs = _u32(srcs)
+        k = len(srcs)
+        lv = np.empty((k, self.n), np.uint8) if levels else None
+        st = Stats()
+        self.ctx.check(self.ctx.lib.gphi_bfs_batch(self.h, _ptr(srcs), k, _ptr(lv), C.byref(st)))
+        return lv, st
 
     def sssp(self, src: int):
         d = np.empty(self.n, np.uint32)

@@ -218,6 +218,12 @@ int gphi_bfs(gphi_graph *g, uint32_t src, int32_t *levels, int32_t *parents, gph
 	return gphi::run_bfs(g, src, levels, parents, st);
 }
 
+int gphi_bfs_batch(gphi_graph *g, const uint32_t *srcs, uint32_t k, uint8_t *levels, gphi_stats *st)
+{
+	GPHI_ENTER(g);
+	return gphi::run_bfs_batch(g, srcs, k, levels, st);
+}
+
 int gphi_sssp(gphi_graph *g, uint32_t src, uint32_t *dist, gphi_stats *st)
 {
 	GPHI_ENTER(g);

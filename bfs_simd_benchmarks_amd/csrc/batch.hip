@@ -1,0 +1,373 @@
+// batch.hip -- multi-source BFS: k <= 256 sources in one traversal.
+//
+// Each vertex carries k-bit masks (NW 64-bit words): the sources that have
+// reached it (seen), that reached it in the last step (front) and that reach
+// it in the current step (next).  One edge stream serves every source:
+//   dense step : the column-stripe tile sweep of the single-source BFS
+//                (apps/bfs_simd/bfs.cpp:181-388), with next[t] |= front[h] & ~seen[t]
+//   sparse step: the CSR push (:510-711) from the vertices whose front mask is
+//                non-empty
+// and the direction rule |F| + sum(outdeg F) > M/T_RATIO (:829-834) is applied
+// to the union frontier F.  A vertex v gets source s at level L+1 exactly when
+// an in-neighbour had s in its front mask at level L and v had not seen s, so
+// the level of every (source, vertex) pair equals that source's own BFS level.
+// Levels are stored as uint8, source-major; 255 = unreached.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "internal.h"
+
+namespace gphi {
+
+namespace {
+
+constexpr uint8_t kUnreached = 255;
+typedef unsigned long long u64;
+
+__device__ __forceinline__ bool test_bit32(const uint32_t *__restrict__ bm, uint32_t v)
+{
+	return (bm[v >> 5] >> (v & 31)) & 1u;
+}
+
+template <int NW>
+struct Mask {
+	u64 w[NW];
+	__device__ __forceinline__ bool any() const
+	{
+		u64 a = 0;
+#pragma unroll
+		for (int i = 0; i < NW; ++i) a |= w[i];
+		return a != 0;
+	}
+};
+
+template <int NW>
+__device__ __forceinline__ Mask<NW> load_mask(const u64 *__restrict__ p, uint32_t v)
+{
+	Mask<NW> m;
+#pragma unroll
+	for (int i = 0; i < NW; ++i) m.w[i] = p[(uint64_t)v * NW + i];
+	return m;
+}
+
+// next[t] |= f & ~seen[t], skipping words already set in next
+template <int NW>
+__device__ __forceinline__ void push_mask(const Mask<NW> &f, uint32_t t, const u64 *__restrict__ seen,
+                                          u64 *next)
+{
+#pragma unroll
+	for (int i = 0; i < NW; ++i) {
+		if (!f.w[i]) continue;
+		const u64 m = f.w[i] & ~seen[(uint64_t)t * NW + i];
+		if (m && (m & ~next[(uint64_t)t * NW + i])) atomicOr(next + (uint64_t)t * NW + i, m);
+	}
+}
+
+template <int NW>
+__global__ __launch_bounds__(kBlock) void batch_dense_kernel(
+    const uint32_t *__restrict__ heads, const uint32_t *__restrict__ tails,
+    const uint32_t *__restrict__ lists, uint32_t list_cap, const uint32_t *__restrict__ counts,
+    const uint32_t *__restrict__ cur, const u64 *__restrict__ front, const u64 *__restrict__ seen,
+    u64 *next)
+{
+	const uint32_t xcd = blockIdx.x % kNumXcd, per = gridDim.x / kNumXcd;
+	const uint32_t cnt = counts[xcd];
+	const uint32_t *lst = lists + (uint64_t)xcd * list_cap;
+	for (uint32_t i = blockIdx.x / kNumXcd; i < cnt; i += per) {
+		const uint32_t c = lst[i];
+		const uint4 *H = reinterpret_cast<const uint4 *>(heads + (uint64_t)c * kChunk);
+		const uint4 *T = reinterpret_cast<const uint4 *>(tails + (uint64_t)c * kChunk);
+		const uint4 h0 = H[threadIdx.x], h1 = H[threadIdx.x + kBlock];
+		const uint32_t h[8] = {h0.x, h0.y, h0.z, h0.w, h1.x, h1.y, h1.z, h1.w};
+		// union-frontier bit of the head first (4 MB bitmap, cache resident)
+		uint32_t hit = 0;
+#pragma unroll
+		for (int j = 0; j < 8; ++j) hit |= (uint32_t)test_bit32(cur, h[j]) << j;
+		if (!hit) continue;
+		uint4 t0 = make_uint4(0, 0, 0, 0), t1 = t0;
+		if (hit & 0x0Fu) t0 = T[threadIdx.x];
+		if (hit & 0xF0u) t1 = T[threadIdx.x + kBlock];
+		const uint32_t t[8] = {t0.x, t0.y, t0.z, t0.w, t1.x, t1.y, t1.z, t1.w};
+#pragma unroll
+		for (int j = 0; j < 8; ++j) {
+			if (!(hit >> j & 1u)) continue;
+			const Mask<NW> f = load_mask<NW>(front, h[j]);
+			push_mask<NW>(f, t[j], seen, next);
+		}
+	}
+}
+
+// CSR push from the union-frontier queue (same edge partition as the
+// single-source push: tiles of kSparseTile edges starting at tile_lo)
+template <int NW>
+__global__ __launch_bounds__(kBlock) void batch_sparse_kernel(
+    const uint32_t *__restrict__ queue, const uint32_t *__restrict__ qoff,
+    const uint32_t *__restrict__ tile_lo, uint32_t qsize, uint32_t qdeg,
+    const uint32_t *__restrict__ row_off, const uint32_t *__restrict__ cols,
+    const u64 *__restrict__ front, const u64 *__restrict__ seen, u64 *next)
+{
+	__shared__ uint32_t s_off[kSparseTile + 2];
+	__shared__ uint32_t s_u[kSparseTile + 2];
+	__shared__ uint32_t s_base[kSparseTile + 2];
+	const uint32_t ntile = (qdeg + kSparseTile - 1) / kSparseTile;
+	for (uint32_t k = blockIdx.x; k < ntile; k += gridDim.x) {
+		const uint32_t lo = tile_lo[k];
+		uint32_t hi = (k + 1 < ntile) ? tile_lo[k + 1] : qsize - 1;
+		if (hi > qsize - 1) hi = qsize - 1;
+		const uint32_t cnt = hi - lo + 1;
+		for (uint32_t j = threadIdx.x; j < cnt; j += kBlock) {
+			const uint32_t u = queue[lo + j];
+			s_off[j] = qoff[lo + j];
+			s_u[j] = u;
+			s_base[j] = row_off[u];
+		}
+		__syncthreads();
+		const uint32_t e0 = k * kSparseTile, e1 = min(e0 + kSparseTile, qdeg);
+		for (uint32_t e = e0 + threadIdx.x; e < e1; e += kBlock) {
+			uint32_t a = 0, b = cnt;
+			while (b - a > 1) {
+				const uint32_t mid = (a + b) >> 1;
+				if (s_off[mid] <= e)
+					a = mid;
+				else
+					b = mid;
+			}
+			const uint32_t u = s_u[a];
+			const uint32_t v = cols[s_base[a] + (e - s_off[a])];
+			push_mask<NW>(load_mask<NW>(front, u), v, seen, next);
+		}
+		__syncthreads();
+	}
+}
+
+// Per vertex: new = next & ~seen; seen |= new; front = new; next = 0; the
+// level of every new source is L+1; flag marks the union frontier for the
+// shared mark/scan/emit update.
+template <int NW>
+__global__ __launch_bounds__(kBlock) void batch_update_kernel(uint32_t n, uint32_t n_pitch, uint32_t k,
+                                                              uint32_t next_level, u64 *__restrict__ front,
+                                                              u64 *__restrict__ seen, u64 *__restrict__ next,
+                                                              uint8_t *__restrict__ level,
+                                                              uint8_t *__restrict__ flag)
+{
+	const uint32_t v = blockIdx.x * kBlock + threadIdx.x;
+	if (v >= n) return;
+	bool any = false;
+#pragma unroll
+	for (int i = 0; i < NW; ++i) {
+		const uint64_t idx = (uint64_t)v * NW + i;
+		const u64 nx = next[idx];
+		const u64 sn = seen[idx];
+		const u64 nw = nx & ~sn;
+		const u64 fr = front[idx];
+		if (nw | fr) front[idx] = nw;
+		if (nx) next[idx] = 0;
+		if (nw) {
+			seen[idx] = sn | nw;
+			any = true;
+			for (u64 b = nw; b; b &= b - 1) {
+				const uint32_t s = (uint32_t)i * 64 + (uint32_t)__ffsll((long long)b) - 1;
+				level[(uint64_t)s * n_pitch + v] = (uint8_t)next_level;
+			}
+		}
+	}
+	if (any) flag[v] = 1;
+	(void)k;
+}
+
+__global__ void batch_seed_kernel(const uint32_t *__restrict__ srcs, uint32_t k, uint32_t nw,
+                                  uint32_t n_pitch, u64 *front, u64 *seen, uint8_t *level,
+                                  uint8_t *flag)
+{
+	for (uint32_t i = threadIdx.x; i < k; i += blockDim.x) {
+		const uint32_t v = srcs[i];
+		atomicOr(front + (uint64_t)v * nw + i / 64, 1ull << (i % 64));
+		atomicOr(seen + (uint64_t)v * nw + i / 64, 1ull << (i % 64));
+		level[(uint64_t)i * n_pitch + v] = 0;
+		flag[v] = 1;
+	}
+}
+
+// totals: sum over v of popcount(seen[v]) and of deg[v] * popcount(seen[v])
+template <int NW>
+__global__ __launch_bounds__(kBlock) void batch_sums_kernel(uint32_t n, const u64 *__restrict__ seen,
+                                                            const uint32_t *__restrict__ deg, u64 *sums)
+{
+	u64 r = 0, e = 0;
+	for (uint32_t v = blockIdx.x * kBlock + threadIdx.x; v < n; v += gridDim.x * kBlock) {
+		uint32_t c = 0;
+#pragma unroll
+		for (int i = 0; i < NW; ++i) c += (uint32_t)__popcll(seen[(uint64_t)v * NW + i]);
+		r += c;
+		e += (u64)c * deg[v];
+	}
+	for (int o = 32; o > 0; o >>= 1) {
+		r += __shfl_xor(r, o, 64);
+		e += __shfl_xor(e, o, 64);
+	}
+	if ((threadIdx.x & 63) == 0) {
+		atomicAdd(sums, r);
+		atomicAdd(sums + 1, e);
+	}
+}
+
+unsigned grid_n(uint64_t items) { return (unsigned)std::max<uint64_t>(1, (items + kBlock - 1) / kBlock); }
+
+float ev_ms(hipEvent_t a, hipEvent_t b)
+{
+	float ms = 0.f;
+	return hipEventElapsedTime(&ms, a, b) == hipSuccess ? ms : 0.f;
+}
+
+template <int NW>
+int batch_loop(gphi_graph *gr, const uint32_t *d_srcs, uint32_t k, gphi_stats *st, u64 *sums_host)
+{
+	gphi_ctx *ctx = gr->ctx;
+	DevGraph &G = gr->g;
+	hipStream_t s = ctx->stream;
+	const uint32_t threshold = (uint32_t)G.m / ctx->t_ratio;
+	StepCtrl *hc = ctx->h_ctrl;
+	hipEvent_t *ev = ctx->ev;
+	const bool timing = st != nullptr && ctx->ev_ok;
+	const size_t mask_bytes = (size_t)G.n_pad * NW * 8;
+	if (timing) GPHI_HIP(ctx, hipEventRecord(ev[0], s));
+	GPHI_HIP(ctx, hipMemsetAsync(gr->b_front, 0, mask_bytes, s));
+	GPHI_HIP(ctx, hipMemsetAsync(gr->b_seen, 0, mask_bytes, s));
+	GPHI_HIP(ctx, hipMemsetAsync(gr->b_next, 0, mask_bytes, s));
+	GPHI_HIP(ctx, hipMemsetAsync(gr->b_level, kUnreached, (size_t)k * G.n_pad, s));
+	GPHI_HIP(ctx, hipMemsetAsync(gr->cur, 0, (size_t)G.n_words * 4, s));
+	launch_init_blocks(gr);
+	batch_seed_kernel<<<1, kBlock, 0, s>>>(d_srcs, k, NW, G.n_pad, gr->b_front, gr->b_seen, gr->b_level,
+	                                       gr->flag);
+	// the sources form the first (union) frontier
+	launch_frontier_update_flags(gr, 0, threshold);
+	GPHI_HIP(ctx, hipGetLastError());
+	if (timing) GPHI_HIP(ctx, hipEventRecord(ev[1], s));
+	GPHI_HIP(ctx, hipMemcpyAsync(hc, gr->ctrl, sizeof(StepCtrl), hipMemcpyDeviceToHost, s));
+	GPHI_HIP(ctx, hipStreamSynchronize(s));
+	uint32_t steps = 0;
+	while (!hc->done) {
+		const uint32_t L = steps;
+		if (L + 1 >= kUnreached)
+			return set_error(ctx, GPHI_E_RANGE, "a batched BFS deeper than 254 levels does not fit uint8 levels");
+		// the rule applies to the union frontier from the first step on
+		const uint32_t dir = hc->dir;
+		if (st && steps < GPHI_MAX_STEPS) {
+			st->dir[steps] = dir;
+			st->fsize[steps] = hc->fsize;
+			st->fdeg[steps] = hc->fdeg;
+		}
+		const bool ev_step = timing && steps < GPHI_MAX_STEPS - 1;
+		if (dir == 0) {
+			if (hc->fdeg)
+				batch_sparse_kernel<NW><<<(unsigned)std::min<uint64_t>((hc->fdeg + kSparseTile - 1) / kSparseTile, 4096),
+				                          kBlock, 0, s>>>(gr->queue, gr->qoff, gr->tile_lo, hc->qsize, hc->fdeg,
+				                                          G.row_off, G.cols, gr->b_front, gr->b_seen, gr->b_next);
+		} else {
+			uint32_t list_cap = 0;
+			int rc = launch_chunk_select_rows(gr, L, &list_cap);
+			if (rc) return rc;
+			batch_dense_kernel<NW><<<dense_grid(G), kBlock, 0, s>>>(G.heads, G.tails, gr->live, list_cap,
+			                                                      gr->live_cnt, gr->cur, gr->b_front,
+			                                                      gr->b_seen, gr->b_next);
+		}
+		GPHI_HIP(ctx, hipGetLastError());
+		if (ev_step) GPHI_HIP(ctx, hipEventRecord(ev[2 + 2 * steps], s));
+		batch_update_kernel<NW><<<grid_n(G.n), kBlock, 0, s>>>(G.n, G.n_pad, k, L + 1, gr->b_front, gr->b_seen,
+		                                                     gr->b_next, gr->b_level, gr->flag);
+		launch_frontier_update_flags(gr, L + 1, threshold);
+		GPHI_HIP(ctx, hipGetLastError());
+		if (ev_step) GPHI_HIP(ctx, hipEventRecord(ev[3 + 2 * steps], s));
+		GPHI_HIP(ctx, hipMemcpyAsync(hc, gr->ctrl, sizeof(StepCtrl), hipMemcpyDeviceToHost, s));
+		GPHI_HIP(ctx, hipStreamSynchronize(s));
+		++steps;
+		if (steps > G.n + 2) return set_error(ctx, GPHI_E_STATE, "batched traversal did not terminate");
+	}
+	GPHI_HIP(ctx, hipMemsetAsync(gr->b_sums, 0, 16, s));
+	batch_sums_kernel<NW><<<std::min<unsigned>(grid_n(G.n), 4096), kBlock, 0, s>>>(G.n, gr->b_seen, G.deg,
+	                                                                               gr->b_sums);
+	if (timing) GPHI_HIP(ctx, hipEventRecord(ev[2 * GPHI_MAX_STEPS + 2], s));
+	GPHI_HIP(ctx, hipMemcpyAsync(sums_host, gr->b_sums, 16, hipMemcpyDeviceToHost, s));
+	GPHI_HIP(ctx, hipStreamSynchronize(s));
+	if (st) {
+		st->steps = std::min<uint32_t>(steps, GPHI_MAX_STEPS);
+		if (timing) {
+			st->init_ms = ev_ms(ev[0], ev[1]);
+			hipEvent_t prev = ev[1];
+			const uint32_t ns = std::min<uint32_t>(steps, GPHI_MAX_STEPS - 1);
+			for (uint32_t q = 0; q < ns; ++q) {
+				st->sweep_ms[q] = ev_ms(prev, ev[2 + 2 * q]);
+				st->update_ms[q] = ev_ms(ev[2 + 2 * q], ev[3 + 2 * q]);
+				prev = ev[3 + 2 * q];
+			}
+			st->total_ms = ev_ms(ev[0], ev[2 * GPHI_MAX_STEPS + 2]);
+		}
+	}
+	return GPHI_OK;
+}
+
+}  // namespace
+
+int run_bfs_batch(gphi_graph *gr, const uint32_t *srcs, uint32_t k, uint8_t *levels, gphi_stats *st)
+{
+	gphi_ctx *ctx = gr->ctx;
+	DevGraph &G = gr->g;
+	if (k == 0 || k > 256) return set_error(ctx, GPHI_E_ARG, "batch size %u not in [1, 256]", k);
+	if (!srcs) return set_error(ctx, GPHI_E_ARG, "sources are NULL");
+	for (uint32_t i = 0; i < k; ++i)
+		if (srcs[i] >= G.n) return set_error(ctx, GPHI_E_RANGE, "source %u >= N %u", srcs[i], G.n);
+	if (!G.has_csr || !G.has_tiles) return set_error(ctx, GPHI_E_STATE, "graph needs CSR and tile arrays");
+	int rc = ensure_traversal_state(gr);
+	if (rc) return rc;
+	const uint32_t nw = k <= 64 ? 1 : k <= 128 ? 2 : 4;
+	if (gr->batch_words < nw || !gr->b_level) {
+		void *ps[] = {gr->b_front, gr->b_seen, gr->b_next, gr->b_level, gr->b_sums};
+		for (void *p : ps)
+			if (p) (void)hipFree(p);
+		gr->b_front = gr->b_seen = gr->b_next = gr->b_sums = nullptr;
+		gr->b_level = nullptr;
+		gr->batch_words = 0;
+		const size_t mb = (size_t)G.n_pad * nw * 8;
+		hipError_t e = hipMalloc((void **)&gr->b_front, mb);
+		if (e == hipSuccess) e = hipMalloc((void **)&gr->b_seen, mb);
+		if (e == hipSuccess) e = hipMalloc((void **)&gr->b_next, mb);
+		if (e == hipSuccess) e = hipMalloc((void **)&gr->b_level, (size_t)nw * 64 * G.n_pad);
+		if (e == hipSuccess) e = hipMalloc((void **)&gr->b_sums, 16);
+		if (e != hipSuccess)
+			return set_error(ctx, GPHI_E_OOM, "batched BFS buffers (%zu MB): %s",
+			                 (3 * mb + (size_t)nw * 64 * G.n_pad) >> 20, hipGetErrorString(e));
+		gr->batch_words = nw;
+	}
+	hipStream_t s = ctx->stream;
+	uint32_t *d_srcs = nullptr;
+	GPHI_HIP(ctx, hipMalloc((void **)&d_srcs, (size_t)k * 4));
+	hipError_t e = hipMemcpy(d_srcs, srcs, (size_t)k * 4, hipMemcpyHostToDevice);
+	if (e != hipSuccess) {
+		(void)hipFree(d_srcs);
+		return hip_fail(ctx, e, "copy sources", __FILE__, __LINE__);
+	}
+	if (st) memset(st, 0, sizeof(*st));
+	u64 sums[2] = {0, 0};
+	// the masks are sized for gr->batch_words >= nw words; use that stride
+	switch (gr->batch_words) {
+	case 1: rc = batch_loop<1>(gr, d_srcs, k, st, sums); break;
+	case 2: rc = batch_loop<2>(gr, d_srcs, k, st, sums); break;
+	default: rc = batch_loop<4>(gr, d_srcs, k, st, sums); break;
+	}
+	(void)hipFree(d_srcs);
+	if (rc) return rc;
+	if (st) {
+		st->reached = sums[0];
+		st->edges_traversed = sums[1];
+	}
+	if (levels) {
+		GPHI_HIP(ctx, hipMemcpy2DAsync(levels, G.n, gr->b_level, G.n_pad, G.n, k, hipMemcpyDeviceToHost, s));
+		GPHI_HIP(ctx, hipStreamSynchronize(s));
+	}
+	return GPHI_OK;
+}
+
+}  // namespace gphi

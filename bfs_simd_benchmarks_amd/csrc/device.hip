@@ -333,7 +333,8 @@ void free_graph_device(gphi_graph *gr)
 	void *ps[] = {G.row_off, G.cols, G.csr_w, G.deg, G.heads, G.tails, G.coo_w, G.meta,
 	              gr->level, gr->parent, gr->dist, gr->flag, gr->visited, gr->cur,
 	              gr->row_stamp, gr->col_unvis, gr->queue, gr->qoff, gr->ctrl, gr->rank,
-	              gr->sum, gr->contrib, gr->tile_lo, gr->blk_cnt, gr->blk_base, gr->live, gr->live_cnt};
+	              gr->sum, gr->contrib, gr->tile_lo, gr->blk_cnt, gr->blk_base, gr->live, gr->live_cnt,
+	              gr->b_front, gr->b_seen, gr->b_next, gr->b_level, gr->b_sums};
 	for (void *p : ps)
 		if (p) (void)hipFree(p);
 }
@@ -438,15 +439,27 @@ __global__ __launch_bounds__(kBlock) void chunk_select_kernel(
 	if (c < nchunks) {
 		const ChunkMeta md = meta[c];
 		if (md.row_lo <= md.row_hi) {
-			if (md.row_hi - md.row_lo >= 64) {
+			// up to 16 rows / columns are checked with independent loads;
+			// wider ranges count as live
+			const uint32_t nr = md.row_hi - md.row_lo + 1;
+			if (nr > 16) {
 				live = true;
 			} else {
-				for (uint32_t r = md.row_lo; r <= md.row_hi && !live; ++r) live = row_stamp[r] == level;
-			}
-			if (kColSkip && live && md.col_hi - md.col_lo < 64) {
 				bool any = false;
-				for (uint32_t k = md.col_lo; k <= md.col_hi && !any; ++k) any = col_unvis[k] != 0;
+#pragma unroll
+				for (uint32_t k = 0; k < 16; ++k)
+					any |= k < nr && row_stamp[md.row_lo + k] == level;
 				live = any;
+			}
+			if (kColSkip && live) {
+				const uint32_t nc = md.col_hi - md.col_lo + 1;
+				if (nc <= 16) {
+					bool any = false;
+#pragma unroll
+					for (uint32_t k = 0; k < 16; ++k)
+						any |= k < nc && col_unvis[md.col_lo + k] != 0;
+					live = any;
+				}
 			}
 		}
 	}
@@ -483,13 +496,19 @@ __device__ __forceinline__ uint4 ld_stream(const uint4 *p)
 	return make_uint4(v.x, v.y, v.z, v.w);
 }
 
-// 8 consecutive-in-memory edges of one lane: frontier bit of the head, then
-// visited bit and flag of the tail; the winner writes parent and flag.
-__device__ __forceinline__ void bfs_visit8(const uint32_t (&h)[8], const uint32_t (&t)[8],
-                                           const uint32_t *__restrict__ cur,
-                                           const uint32_t *__restrict__ visited,
-                                           uint8_t *__restrict__ flag, int32_t *__restrict__ parent)
+// One chunk, 8 edges per lane (two uint4 of heads).  The frontier bits of the
+// heads are tested first; a lane loads the tails of a 4-edge quad only when
+// one of its heads is in the frontier, so frontier-sparse dense steps read
+// little more than the heads.  Then the visited bit and the flag of each
+// candidate tail; the winner writes parent and flag (benign races: every
+// writer stores a valid frontier head and the same flag).
+__device__ __forceinline__ void bfs_dense_chunk(uint32_t c, const uint4 h0, const uint4 h1,
+                                                const uint32_t *__restrict__ tails,
+                                                const uint32_t *__restrict__ cur,
+                                                const uint32_t *__restrict__ visited,
+                                                uint8_t *__restrict__ flag, int32_t *__restrict__ parent)
 {
+	const uint32_t h[8] = {h0.x, h0.y, h0.z, h0.w, h1.x, h1.y, h1.z, h1.w};
 	uint32_t cw[8];
 #pragma unroll
 	for (int j = 0; j < 8; ++j) cw[j] = cur[h[j] >> 5];
@@ -497,6 +516,11 @@ __device__ __forceinline__ void bfs_visit8(const uint32_t (&h)[8], const uint32_
 #pragma unroll
 	for (int j = 0; j < 8; ++j) hit |= ((cw[j] >> (h[j] & 31)) & 1u) << j;
 	if (!hit) return;
+	const uint4 *T = reinterpret_cast<const uint4 *>(tails + (uint64_t)c * kChunk);
+	uint4 t0 = make_uint4(0, 0, 0, 0), t1 = t0;
+	if (hit & 0x0Fu) t0 = ld_stream(T + threadIdx.x);
+	if (hit & 0xF0u) t1 = ld_stream(T + threadIdx.x + kBlock);
+	const uint32_t t[8] = {t0.x, t0.y, t0.z, t0.w, t1.x, t1.y, t1.z, t1.w};
 	uint32_t vw[8];
 #pragma unroll
 	for (int j = 0; j < 8; ++j) vw[j] = (hit >> j & 1u) ? visited[t[j] >> 5] : 0xFFFFFFFFu;
@@ -516,37 +540,42 @@ __device__ __forceinline__ void bfs_visit8(const uint32_t (&h)[8], const uint32_
 		}
 }
 
+// Walk of this XCD's live-chunk list; the heads of the next chunk (and the
+// list entry after it) are loaded before the current chunk is processed, so
+// HBM reads overlap the dependent bitmap lookups.
 __global__ __launch_bounds__(kBlock) void bfs_dense_kernel(
     const uint32_t *__restrict__ heads, const uint32_t *__restrict__ tails,
     const uint32_t *__restrict__ lists, uint32_t list_cap, const uint32_t *__restrict__ counts,
     const uint32_t *__restrict__ cur, const uint32_t *__restrict__ visited,
     uint8_t *__restrict__ flag, int32_t *__restrict__ parent)
 {
-	GPHI_FOR_LIVE(ca, cb, has_b, lists, list_cap, counts)
-	{
-		const uint4 *Ha = reinterpret_cast<const uint4 *>(heads + (uint64_t)ca * kChunk);
-		const uint4 *Ta = reinterpret_cast<const uint4 *>(tails + (uint64_t)ca * kChunk);
-		const uint4 *Hb = reinterpret_cast<const uint4 *>(heads + (uint64_t)cb * kChunk);
-		const uint4 *Tb = reinterpret_cast<const uint4 *>(tails + (uint64_t)cb * kChunk);
-		const uint4 a0 = ld_stream(Ha + threadIdx.x), a1 = ld_stream(Ha + threadIdx.x + kBlock);
-		const uint4 b0 = ld_stream(Ta + threadIdx.x), b1 = ld_stream(Ta + threadIdx.x + kBlock);
-		uint4 c0 = make_uint4(0, 0, 0, 0), c1 = c0, d0 = c0, d1 = c0;
-		if (has_b) {
-			c0 = ld_stream(Hb + threadIdx.x);
-			c1 = ld_stream(Hb + threadIdx.x + kBlock);
-			d0 = ld_stream(Tb + threadIdx.x);
-			d1 = ld_stream(Tb + threadIdx.x + kBlock);
+	const uint32_t xcd = blockIdx.x % kNumXcd, per = gridDim.x / kNumXcd;
+	const uint32_t cnt = counts[xcd];
+	const uint32_t *lst = lists + (uint64_t)xcd * list_cap;
+	uint32_t i = blockIdx.x / kNumXcd;
+	if (i >= cnt) return;
+	uint32_t c = lst[i];
+	uint32_t c_next = i + per < cnt ? lst[i + per] : 0;
+	const uint4 *H = reinterpret_cast<const uint4 *>(heads + (uint64_t)c * kChunk);
+	uint4 h0 = ld_stream(H + threadIdx.x), h1 = ld_stream(H + threadIdx.x + kBlock);
+	for (;;) {
+		const uint32_t in = i + per;
+		const bool more = in < cnt;
+		uint4 g0 = make_uint4(0, 0, 0, 0), g1 = g0;
+		uint32_t c_after = 0;
+		if (more) {
+			const uint4 *Hn = reinterpret_cast<const uint4 *>(heads + (uint64_t)c_next * kChunk);
+			g0 = ld_stream(Hn + threadIdx.x);
+			g1 = ld_stream(Hn + threadIdx.x + kBlock);
+			c_after = in + per < cnt ? lst[in + per] : 0;
 		}
-		{
-			const uint32_t h[8] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w};
-			const uint32_t t[8] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w};
-			bfs_visit8(h, t, cur, visited, flag, parent);
-		}
-		if (has_b) {
-			const uint32_t h[8] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
-			const uint32_t t[8] = {d0.x, d0.y, d0.z, d0.w, d1.x, d1.y, d1.z, d1.w};
-			bfs_visit8(h, t, cur, visited, flag, parent);
-		}
+		bfs_dense_chunk(c, h0, h1, tails, cur, visited, flag, parent);
+		if (!more) break;
+		i = in;
+		c = c_next;
+		c_next = c_after;
+		h0 = g0;
+		h1 = g1;
 	}
 }
 static_assert(kChunk == 2 * 4 * kBlock, "dense sweep loads two uint4 per array per thread");
@@ -814,12 +843,13 @@ __global__ __launch_bounds__(kScanThreads) void frontier_scan_kernel(uint32_t nb
 }
 
 __global__ __launch_bounds__(kBlock) void frontier_emit_kernel(
-    const uint32_t *__restrict__ deg, const uint32_t *__restrict__ cur,
+    const StepCtrl *__restrict__ ctrl, const uint32_t *__restrict__ deg, const uint32_t *__restrict__ cur,
     const uint4 *__restrict__ blk_cnt, const uint2 *__restrict__ blk_base,
     uint32_t *__restrict__ queue, uint32_t *__restrict__ qoff, uint32_t *__restrict__ tile_lo)
 {
 	__shared__ uint32_t sq[kBlock / 64], sd[kBlock / 64];
-	if (blk_cnt[blockIdx.x].y == 0) return;  // uniform: no queue entries here
+	// the queue only feeds a sparse step; empty blocks have nothing to emit
+	if (ctrl->dir != 0 || blk_cnt[blockIdx.x].y == 0) return;
 	const uint32_t w = blockIdx.x * kBlock + threadIdx.x;
 	const uint32_t v0 = w * 32;
 	const uint32_t bits = cur[w];
@@ -874,7 +904,7 @@ __global__ __launch_bounds__(kBlock) void frontier_emit_kernel(
 // ---------------------------------------------------------------------------
 // host drivers
 // ---------------------------------------------------------------------------
-static constexpr unsigned kDenseGrid = 4096;   // multiple of kNumXcd
+constexpr unsigned kDenseGrid = 4096;   // multiple of kNumXcd
 static constexpr unsigned kSparseGrid = 4096;
 
 static float ev_ms(hipEvent_t a, hipEvent_t b)
@@ -963,7 +993,7 @@ static int frontier_loop(gphi_graph *gr, uint32_t src, gphi_stats *st)
 		frontier_mark_kernel<kBfs><<<nblk, kBlock, 0, s>>>(G.W, L + 1, gr->flag, G.deg, gr->level, gr->visited,
 		                                                  gr->cur, gr->row_stamp, gr->col_unvis, gr->blk_cnt);
 		frontier_scan_kernel<<<1, kScanThreads, 0, s>>>(nblk, threshold, gr->blk_cnt, gr->blk_base, gr->ctrl);
-		frontier_emit_kernel<<<nblk, kBlock, 0, s>>>(G.deg, gr->cur, gr->blk_cnt, gr->blk_base, gr->queue,
+		frontier_emit_kernel<<<nblk, kBlock, 0, s>>>(gr->ctrl, G.deg, gr->cur, gr->blk_cnt, gr->blk_base, gr->queue,
 		                                            gr->qoff, gr->tile_lo);
 		GPHI_HIP(ctx, hipGetLastError());
 		if (ev_step) GPHI_HIP(ctx, hipEventRecord(ev[3 + 2 * steps], s));
@@ -989,6 +1019,49 @@ static int frontier_loop(gphi_graph *gr, uint32_t src, gphi_stats *st)
 		}
 	}
 	return GPHI_OK;
+}
+
+// ---- helpers shared with other translation units (batch.hip) ----
+int ensure_traversal_state(gphi_graph *gr) { return ensure_state(gr); }
+
+void launch_init_blocks(gphi_graph *gr)
+{
+	DevGraph &G = gr->g;
+	init_blocks_kernel<<<grid_for(G.side, kBlock, 65535), kBlock, 0, gr->ctx->stream>>>(
+	    G.n, G.W, G.side, gr->row_stamp, gr->col_unvis);
+}
+
+// frontier update from the byte flags (mark / scan / emit), no level writes
+void launch_frontier_update_flags(gphi_graph *gr, uint32_t next_level, uint32_t threshold)
+{
+	DevGraph &G = gr->g;
+	hipStream_t s = gr->ctx->stream;
+	const unsigned nblk = G.n_pad / kPadVerts;
+	frontier_mark_kernel<false><<<nblk, kBlock, 0, s>>>(G.W, next_level, gr->flag, G.deg, gr->level,
+	                                                   gr->visited, gr->cur, gr->row_stamp, gr->col_unvis,
+	                                                   gr->blk_cnt);
+	frontier_scan_kernel<<<1, kScanThreads, 0, s>>>(nblk, threshold, gr->blk_cnt, gr->blk_base, gr->ctrl);
+	frontier_emit_kernel<<<nblk, kBlock, 0, s>>>(gr->ctrl, G.deg, gr->cur, gr->blk_cnt, gr->blk_base,
+	                                            gr->queue, gr->qoff, gr->tile_lo);
+}
+
+// per-XCD live-chunk lists of the rows stamped `level` (no column test)
+int launch_chunk_select_rows(gphi_graph *gr, uint32_t level, uint32_t *list_cap)
+{
+	DevGraph &G = gr->g;
+	hipStream_t s = gr->ctx->stream;
+	*list_cap = (uint32_t)((G.nchunks + kNumXcd - 1) / kNumXcd);
+	GPHI_HIP(gr->ctx, hipMemsetAsync(gr->live_cnt, 0, kNumXcd * 4, s));
+	const unsigned sgrid = (unsigned)std::max<uint64_t>(1, (G.nchunks + kBlock - 1) / kBlock);
+	chunk_select_kernel<false><<<sgrid, kBlock, 0, s>>>(G.meta, G.nchunks, gr->row_stamp, gr->col_unvis,
+	                                                  level, gr->live, *list_cap, gr->live_cnt);
+	return GPHI_OK;
+}
+
+unsigned dense_grid(const DevGraph &G)
+{
+	unsigned grid = (unsigned)std::min<uint64_t>(kDenseGrid, (G.nchunks + kNumXcd - 1) / kNumXcd * kNumXcd);
+	return std::max<unsigned>(grid / kNumXcd * kNumXcd, kNumXcd);
 }
 
 int run_bfs(gphi_graph *gr, uint32_t src, int32_t *levels, int32_t *parents, gphi_stats *st)

@@ -102,6 +102,11 @@ struct gphi_graph {
 	uint32_t *live = nullptr;  // per-XCD lists of live chunks of a dense step
 	uint32_t *live_cnt = nullptr;
 	float *rank = nullptr, *sum = nullptr, *contrib = nullptr;
+	// multi-source BFS (batch.hip): per-vertex source masks and uint8 levels
+	uint32_t batch_words = 0;  // 64-bit words per vertex the buffers were sized for
+	unsigned long long *b_front = nullptr, *b_seen = nullptr, *b_next = nullptr;
+	uint8_t *b_level = nullptr;
+	unsigned long long *b_sums = nullptr;
 };
 
 namespace gphi {
@@ -124,6 +129,13 @@ int count_active_tiles(gphi_graph *gr, const int32_t *levels, const std::vector<
 int tile_offsets(gphi_graph *gr, uint32_t *out);
 int download_graph(gphi_graph *gr, uint32_t *deg, uint32_t *cols, uint32_t *cw, uint32_t *th,
                    uint32_t *tt, uint32_t *tw);
+
+int ensure_traversal_state(gphi_graph *gr);
+void launch_init_blocks(gphi_graph *gr);
+void launch_frontier_update_flags(gphi_graph *gr, uint32_t next_level, uint32_t threshold);
+int launch_chunk_select_rows(gphi_graph *gr, uint32_t level, uint32_t *list_cap);
+unsigned dense_grid(const DevGraph &G);
+int run_bfs_batch(gphi_graph *gr, const uint32_t *srcs, uint32_t k, uint8_t *levels, gphi_stats *st);
 
 // generator + tool pipeline on the device (gen.hip)
 int generate_graph(gphi_ctx *ctx, const gphi_rmat_spec *sp, uint32_t W, uint32_t S, int reorder,

@@ -122,6 +122,17 @@ int gphi_pagerank(gphi_graph *g, uint32_t iters, float damping, float init, floa
 int gphi_bellman_ford(gphi_graph *g, uint32_t src, uint32_t max_sweeps, uint32_t *dist,
                       uint32_t *sweeps_run, float *ms);
 
+/* Multi-source BFS: k (1..256) sources in one traversal, bit-packed per
+ * vertex; one edge stream serves all of them (no reference counterpart: the
+ * batched form of graph_prepare, apps/bfs_simd/bfs.cpp:753-963, with the
+ * same tile sweep, CSR push and direction rule applied to the union frontier).
+ * levels: k*n uint8, source-major (levels[i*n + v]); 255 = unreached; NULL
+ * keeps them in HBM.  Each row equals gphi_bfs's levels of that source; a
+ * traversal deeper than 254 levels returns GPHI_E_RANGE.  st->reached and
+ * st->edges_traversed are summed over the k sources. */
+int gphi_bfs_batch(gphi_graph *g, const uint32_t *srcs, uint32_t k, uint8_t *levels,
+                   gphi_stats *st);
+
 /* ---- synthetic inputs and the tool pipeline on the device ----
  * The reference's inputs came from external generators (PBBS rMat,
  * tools/adj2snap/adj2snap.cpp; Galois graph-convert).  gphi_graph_generate

@@ -1,0 +1,79 @@
+"""Multi-source BFS (gphi_bfs_batch): every row must equal that source's own
+BFS levels (bit-exact), for 1..256 sources, duplicates and sources without
+out-edges."""
+import numpy as np
+import pytest
+
+import oracle as O
+from bfs_simd_benchmarks_amd.workloads import pick_sources
+
+pytestmark = pytest.mark.gpu
+
+
+def upload(ctx, g):
+    return ctx.graph_from_arrays(g.n, g.deg, W=g.W, S=g.S, csr_edges=g.cols, tile_heads=g.sh,
+                                 tile_tails=g.st, tile_offsets=g.soff)
+
+
+def expected(g, srcs):
+    rows = []
+    for s in srcs:
+        lv = O.bfs_levels(g.n, g.deg, g.cols, int(s))
+        rows.append(np.where(lv >= 0, lv, 255).astype(np.uint8))
+    return np.stack(rows)
+
+
+@pytest.mark.parametrize("k", [1, 3, 64, 65, 100, 256])
+def test_batch_rows_equal_single_source_levels(ctx, c1, k):
+    print(f"case: batch c1 k={k}", flush=True)
+    srcs = pick_sources(c1.deg, k, 1000 + k)
+    gg = upload(ctx, c1)
+    try:
+        lv, st = gg.bfs_batch(srcs)
+    finally:
+        gg.free()
+    want = expected(c1, srcs)
+    bad = np.argwhere(lv != want)
+    assert bad.size == 0, f"{len(bad)} mismatches, first (source row, vertex) {bad[:5].tolist()}"
+    reached = (want != 255)
+    assert st.reached == int(reached.sum())
+    assert st.edges_traversed == int((reached * c1.deg[None, :].astype(np.uint64)).sum())
+
+
+def test_batch_duplicates_and_isolated_sources(ctx, c1):
+    iso = int(np.flatnonzero(c1.deg == 0)[0])
+    srcs = np.array([0, 0, iso, 5, 5, 0], np.uint32)
+    print("case: batch c1 duplicate/isolated sources", flush=True)
+    gg = upload(ctx, c1)
+    try:
+        lv, _ = gg.bfs_batch(srcs)
+        lv2, _ = gg.bfs_batch(srcs[:2])          # buffers reused with a smaller batch
+    finally:
+        gg.free()
+    assert np.array_equal(lv, expected(c1, srcs))
+    assert np.array_equal(lv2, expected(c1, srcs[:2]))
+
+
+def test_batch_other_tilings(ctx):
+    n, h, t = O.config_edges("c1")
+    g = O.build_from_edgelist(n, h, t, 1000, 3)
+    srcs = pick_sources(g.deg, 64, 7)
+    print("case: batch c1 W=1000 S=3", flush=True)
+    gg = upload(ctx, g)
+    try:
+        lv, _ = gg.bfs_batch(srcs)
+    finally:
+        gg.free()
+    assert np.array_equal(lv, expected(g, srcs))
+
+
+def test_batch_rejects_bad_arguments(ctx, c1):
+    import bfs_simd_benchmarks_amd as G
+    gg = upload(ctx, c1)
+    try:
+        for srcs, code in (([], -6), (list(range(257)), -6), ([c1.n], -3)):
+            with pytest.raises(G.GphiError) as ei:
+                gg.bfs_batch(np.array(srcs, np.uint32))
+            assert ei.value.code == code
+    finally:
+        gg.free()
