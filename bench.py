@@ -46,10 +46,19 @@ def main() -> int:
     ap.add_argument("--S", type=int, default=None, help="stripe length")
     ap.add_argument("--cpu-seconds", type=float, default=20.0,
                     help="budget of the CPU baseline (0 disables it)")
+    ap.add_argument("--batches", default="64,256",
+                    help="batched multi-source BFS sizes to measure ('' disables)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB five times (calibrates the "
                          "FETCH_SIZE / WRITE_SIZE counters against known bytes)")
+    ap.add_argument("--serving", action="store_true",
+                    help="run the serving mix instead: 8 resident RMAT graphs (scales 18-24), a "
+                         "Poisson stream of BFS/SSSP queries, batched vs one-at-a-time")
+    ap.add_argument("--qps", type=float, default=4000.0, help="offered queries/s (serving)")
+    ap.add_argument("--queries", type=int, default=100_000, help="queries in the stream (serving)")
     args = ap.parse_args()
+    if args.serving:
+        return serving(args)
     if args.copy_probe:
         import torch
         n = args.copy_probe * (1 << 20) // 4
@@ -131,6 +140,34 @@ def main() -> int:
                 "alg_bytes_per_launch": round(per_launch_bytes)}
     whole = step_bytes_total / (step_ms_total * 1e-3) / 1e9 if step_ms_total else 0.0
 
+    # batched multi-source BFS vs repeated plain runs (same GPU, same graph)
+    plain_lat = np.array([st.total_ms for st in stats])
+    batch_out = {}
+    for k in [int(x) for x in args.batches.split(",") if x.strip()]:
+        bsrcs = pick_sources(deg, k, wl.seed + 1000 + k)
+        g.bfs_batch(bsrcs, levels=False)                      # warm-up
+        reps = 3
+        t1 = time.perf_counter()
+        bst = None
+        for _ in range(reps):
+            _, bst = g.bfs_batch(bsrcs, levels=False)
+        bwall = (time.perf_counter() - t1) / reps
+        # exactness: every row equals that source's single-source levels
+        blv, _ = g.bfs_batch(bsrcs, levels=True)
+        exact = True
+        for i in range(0, k, max(1, k // 4)):
+            lv_i, _, _ = g.bfs(int(bsrcs[i]), parents=False, levels=True)
+            exact &= bool(np.array_equal(blv[i], np.where(lv_i >= 0, lv_i, 255).astype(np.uint8)))
+        del blv
+        batch_out[str(k)] = {
+            "queries_per_s": round(k / bwall, 1),
+            "vs_plain": round((k / bwall) / (1e3 / plain_lat.mean()), 2),
+            "latency_ms_p50": round(bwall * 1e3, 3), "latency_ms_p99": round(bwall * 1e3, 3),
+            "device_ms": round(bst.total_ms, 3), "steps": int(bst.steps),
+            "gteps": round(bst.edges_traversed / bwall / 1e9, 1), "levels_exact": exact}
+        print(f"[bench] batch {k}: {k / bwall:.0f} queries/s ({bwall * 1e3:.1f} ms per launch), "
+              f"exact={exact}", file=sys.stderr, flush=True)
+
     cpu = None
     levels_match = None
     if args.cpu_seconds > 0:
@@ -183,11 +220,86 @@ def main() -> int:
         "device_ms_per_bfs": round(sum(st.total_ms for st in stats) / max(len(stats), 1), 4),
         "whole_bfs_alg_GBps": round(whole, 1),
         "levels_match_cpu": levels_match,
+        "plain_queries_per_s": round(1e3 / plain_lat.mean(), 1),
+        "plain_latency_ms_p50": round(float(np.percentile(plain_lat, 50)), 3),
+        "plain_latency_ms_p99": round(float(np.percentile(plain_lat, 99)), 3),
+        "batched": batch_out,
         "graph_build_s": round(gen_s, 2),
     }
     print(json.dumps(out))
     g.free()
     ctx.close()
+    return 0
+
+
+def serving(args) -> int:
+    """Serving mix: 8 resident graphs (RMAT scales 18,19,20,21,22,22,23,24, edge
+    factor 16, seeds 10-17, every other graph with uint32 weights in [1,255]);
+    a Poisson stream of `queries` queries at `qps` offered rate (seed 5): 80% BFS
+    on a uniform random graph, 20% SSSP on a uniform random weighted graph,
+    uniform random source.  The same stream runs against a server that batches
+    the BFS queries pending on a graph (continuous batching, up to 256 per
+    launch) and against one that runs every query alone (max_batch 1)."""
+    import threading
+
+    import bfs_simd_benchmarks_amd as G
+    from bfs_simd_benchmarks_amd.workloads import SERVING_MIX
+
+    rng = np.random.default_rng(5)
+    gaps = rng.exponential(1.0 / args.qps, args.queries)
+    arrivals = np.cumsum(gaps)
+    weighted_ids = [i for i, w in enumerate(SERVING_MIX) if w.weight_seed]
+    is_sssp = rng.random(args.queries) < 0.2
+    gsel = np.where(is_sssp, rng.choice(weighted_ids, args.queries),
+                    rng.integers(0, len(SERVING_MIX), args.queries))
+    unif = rng.random(args.queries)
+    results = {}
+    for mode, max_batch in (("batched", 256), ("one_at_a_time", 1)):
+        srv = G.Server(max_batch=max_batch)
+        sizes = []
+        t0 = time.perf_counter()
+        for w in SERVING_MIX:
+            ctx = G.Context(0)
+            g = w.generate(ctx, W=1024, S=16, weighted=bool(w.weight_seed))
+            sizes.append((g.n, g.m))
+            srv.add_graph(g)
+        build_s = time.perf_counter() - t0
+        srcs = (unif * np.array([sizes[i][0] for i in gsel])).astype(np.int64)
+        tickets = [0] * args.queries
+        start = time.perf_counter()
+
+        def producer():
+            for q in range(args.queries):
+                dt = arrivals[q] - (time.perf_counter() - start)
+                if dt > 0:
+                    time.sleep(dt)
+                tickets[q] = srv.submit(int(gsel[q]), G.QUERY_SSSP if is_sssp[q] else G.QUERY_BFS,
+                                        int(srcs[q]))
+        th = threading.Thread(target=producer)
+        th.start()
+        th.join()
+        lat, batch = np.empty(args.queries), np.empty(args.queries)
+        for q in range(args.queries):
+            info = srv.wait(tickets[q])
+            lat[q], batch[q] = info.latency_us, info.batch
+        wall = time.perf_counter() - start
+        bfs_lat = lat[~is_sssp]
+        results[mode] = {
+            "queries_per_s": round(args.queries / wall, 1),
+            "latency_ms_p50": round(float(np.percentile(lat, 50)) / 1e3, 3),
+            "latency_ms_p99": round(float(np.percentile(lat, 99)) / 1e3, 3),
+            "bfs_latency_ms_p99": round(float(np.percentile(bfs_lat, 99)) / 1e3, 3),
+            "mean_bfs_batch": round(float(batch[~is_sssp].mean()), 2),
+            "launches": int(sum(srv.stats(i)["launches"] for i in range(len(SERVING_MIX)))),
+            "graph_build_s": round(build_s, 2)}
+        print(f"[serving] {mode}: {results[mode]}", file=sys.stderr, flush=True)
+        srv.close()
+    out = {"metric": "serving mix: queries/s and p50/p99 latency, batched vs one-at-a-time",
+           "offered_qps": args.qps, "queries": args.queries, "bfs_share": 0.8,
+           "graphs": [f"{w.name} n={n} m={m}" + (" weighted" if w.weight_seed else "")
+                      for w, (n, m) in zip(SERVING_MIX, sizes)],
+           **results}
+    print(json.dumps(out))
     return 0
 
 

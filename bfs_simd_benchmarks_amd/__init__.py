@@ -63,6 +63,18 @@ class WeightSpec(C.Structure):
     _fields_ = [("seed", C.c_uint64), ("lo", C.c_uint32), ("hi", C.c_uint32)]
 
 
+class ServerOpts(C.Structure):
+    _fields_ = [("max_batch", C.c_uint32), ("hbm_budget", C.c_uint64)]
+
+
+class QueryInfo(C.Structure):
+    _fields_ = [("latency_us", C.c_double), ("queue_us", C.c_double), ("run_us", C.c_double),
+                ("batch", C.c_uint32)]
+
+
+QUERY_BFS = 0
+QUERY_SSSP = 1
+
 REORDER_NONE = 0
 REORDER_DEGREE = 1
 
@@ -114,11 +126,18 @@ def _declare(lib):
                                       i32, C.POINTER(_P)]),
         "gphi_graph_download": (i32, [_P, _P, _P, _P, _P, _P, _P]),
         "gphi_graph_tile_offsets": (i32, [_P, _P]),
+        "gphi_graph_retile": (i32, [_P, u32, u32, i32]),
         "gphi_host_read_text": (i32, [C.c_char_p, u32, u32, i32, i32, C.POINTER(_P)]),
         "gphi_host_from_device": (i32, [_P, C.POINTER(_P)]),
         "gphi_host_write_text": (i32, [_P, C.c_char_p]),
         "gphi_host_desc": (i32, [_P, C.POINTER(GraphDesc)]),
         "gphi_host_free": (None, [_P]),
+        "gphi_server_create": (i32, [C.POINTER(ServerOpts), C.POINTER(_P)]),
+        "gphi_server_add_graph": (i32, [_P, _P, C.POINTER(C.c_uint32)]),
+        "gphi_server_submit": (i32, [_P, u32, u32, u32, _P, C.POINTER(C.c_uint64)]),
+        "gphi_server_wait": (i32, [_P, C.c_uint64, C.POINTER(QueryInfo)]),
+        "gphi_server_stats": (i32, [_P, u32, _P, _P, _P]),
+        "gphi_server_destroy": (None, [_P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -197,6 +216,47 @@ class HostGraph:
             self.free()
         except Exception:
             pass
+
+
+class Server:
+    """Resident graphs with continuous batching (gphi_server_*).  Each graph
+    added must live in its own Context; the server takes both over."""
+
+    def __init__(self, max_batch: int = 64, hbm_budget: int = 0):
+        self.lib = load_library()
+        h = _P()
+        _host_check(self.lib, self.lib.gphi_server_create(C.byref(ServerOpts(max_batch, hbm_budget)),
+                                                          C.byref(h)))
+        self.h = h
+        self.graphs = []
+
+    def add_graph(self, g: "Graph") -> int:
+        gid = C.c_uint32()
+        _host_check(self.lib, self.lib.gphi_server_add_graph(self.h, g.h, C.byref(gid)))
+        g.h = None                      # owned by the server now
+        g.ctx.h = None
+        self.graphs.append(g)
+        return gid.value
+
+    def submit(self, gid: int, kind: int, src: int, out: np.ndarray | None = None) -> int:
+        t = C.c_uint64()
+        _host_check(self.lib, self.lib.gphi_server_submit(self.h, gid, kind, src, _ptr(out), C.byref(t)))
+        return t.value
+
+    def wait(self, ticket: int) -> QueryInfo:
+        info = QueryInfo()
+        _host_check(self.lib, self.lib.gphi_server_wait(self.h, ticket, C.byref(info)))
+        return info
+
+    def stats(self, gid: int):
+        b, s_, m = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        _host_check(self.lib, self.lib.gphi_server_stats(self.h, gid, C.byref(b), C.byref(s_), C.byref(m)))
+        return {"launches": b.value, "served": s_.value, "hbm_bytes": m.value}
+
+    def close(self):
+        if self.h:
+            self.lib.gphi_server_destroy(self.h)
+            self.h = None
 
 
 class Context:
@@ -340,6 +400,11 @@ class Graph:
             _ptr(g("tw"))))
         return out
 
+    def retile(self, W: int, S: int, tile_order: int = TILES_COL_STRIPE):
+        """Rebuild the tile arrays from the CSR with another (W, S) or order."""
+        self.ctx.check(self.ctx.lib.gphi_graph_retile(self.h, W, S, tile_order))
+        self.W, self.S = W, S
+
     def to_host(self) -> HostGraph:
         """Download into a HostGraph (writable as the reference's text files)."""
         h = _P()
@@ -363,5 +428,5 @@ class Graph:
         return out[: st.steps]
 
 
-__all__ = ["Context", "Graph", "HostGraph", "GphiError", "Stats", "load_library", "TILES_COL_STRIPE",
+__all__ = ["Context", "Graph", "HostGraph", "Server", "QUERY_BFS", "QUERY_SSSP", "GphiError", "Stats", "load_library", "TILES_COL_STRIPE",
            "TILES_ROW_MAJOR", "LIB_PATH"]

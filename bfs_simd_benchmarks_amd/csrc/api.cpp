@@ -351,6 +351,12 @@ int gphi_host_from_device(gphi_graph *g, gphi_host_graph **out)
 	return gphi::host_adopt(&d, st, out);
 }
 
+int gphi_graph_retile(gphi_graph *g, uint32_t W, uint32_t S, int tile_order)
+{
+	GPHI_ENTER(g);
+	return gphi::retile_graph(g, W, S, tile_order);
+}
+
 int gphi_graph_tile_offsets(gphi_graph *g, uint32_t *out)
 {
 	GPHI_ENTER(g);

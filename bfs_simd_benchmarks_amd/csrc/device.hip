@@ -327,6 +327,22 @@ int upload_graph(gphi_ctx *ctx, const gphi_graph_desc *d, gphi_graph *gr)
 	return GPHI_OK;
 }
 
+// drops the lazily allocated traversal buffers (they depend on W and S)
+void free_traversal_state(gphi_graph *gr)
+{
+	void **ps[] = {(void **)&gr->level, (void **)&gr->parent, (void **)&gr->dist, (void **)&gr->flag,
+	               (void **)&gr->visited, (void **)&gr->cur, (void **)&gr->row_stamp, (void **)&gr->col_unvis,
+	               (void **)&gr->queue, (void **)&gr->qoff, (void **)&gr->tile_lo, (void **)&gr->ctrl,
+	               (void **)&gr->blk_cnt, (void **)&gr->blk_base, (void **)&gr->live, (void **)&gr->live_cnt,
+	               (void **)&gr->rank, (void **)&gr->sum, (void **)&gr->contrib, (void **)&gr->b_front,
+	               (void **)&gr->b_seen, (void **)&gr->b_next, (void **)&gr->b_level, (void **)&gr->b_sums};
+	for (void **p : ps) {
+		if (*p) (void)hipFree(*p);
+		*p = nullptr;
+	}
+	gr->batch_words = 0;
+}
+
 void free_graph_device(gphi_graph *gr)
 {
 	DevGraph &G = gr->g;

@@ -536,6 +536,83 @@ int generate_graph(gphi_ctx *ctx, const gphi_rmat_spec *sp, uint32_t W, uint32_t
 	return finish_tiles(ctx, gr);
 }
 
+namespace {
+// heads of the CSR order: vertex v at row_off[v] (if deg > 0), then a max-scan
+__global__ void row_start_kernel(uint32_t n, const uint32_t *row_off, const uint32_t *deg, uint32_t *h)
+{
+	GRID_STRIDE(v, n)
+	if (deg[v]) h[row_off[v]] = (uint32_t)v;
+}
+
+struct MaxOp {
+	__device__ __forceinline__ uint32_t operator()(uint32_t a, uint32_t b) const { return a > b ? a : b; }
+};
+}  // namespace
+
+// Re-tiles a graph from its CSR (the coo_tiling / column_major_tile steps
+// again with another W, S or order); traversal state is dropped and rebuilt
+// on the next call.
+int retile_graph(gphi_graph *gr, uint32_t W, uint32_t S, int tile_order)
+{
+	gphi_ctx *ctx = gr->ctx;
+	DevGraph &G = gr->g;
+	hipStream_t s = ctx->stream;
+	if (!G.has_csr) return set_error(ctx, GPHI_E_STATE, "re-tiling needs the CSR arrays");
+	if (W == 0 || S == 0) return set_error(ctx, GPHI_E_ARG, "tile width and stripe length must be > 0");
+	if (tile_order != GPHI_TILES_COL_STRIPE && tile_order != GPHI_TILES_ROW_MAJOR)
+		return set_error(ctx, GPHI_E_ARG, "unknown tile order %d", tile_order);
+	const uint32_t n = G.n;
+	const uint64_t m = G.m;
+	const uint32_t side = n % W ? n / W + 1 : n / W;
+	if ((uint64_t)side * side > (1ull << 32))
+		return set_error(ctx, GPHI_E_RANGE, "SIDE^2 = %llu tiles exceeds 2^32", (unsigned long long)side * side);
+	Scratch sc{ctx, {}};
+	uint32_t *h = sc.alloc<uint32_t>(m), *key = sc.alloc<uint32_t>(m), *perm = sc.alloc<uint32_t>(m);
+	OOM_IF(h && key && perm);
+	int rc;
+	if (m) {
+		GPHI_HIP(ctx, hipMemsetAsync(h, 0, m * 4, s));
+		row_start_kernel<<<grid1(n), kB, 0, s>>>(n, G.row_off, G.deg, h);
+		size_t tb = 0;
+		GPHI_HIP(ctx, hipcub::DeviceScan::InclusiveScan(nullptr, tb, h, h, MaxOp(), m, s));
+		void *tmp = sc.alloc<uint8_t>(tb);
+		OOM_IF(tmp);
+		GPHI_HIP(ctx, hipcub::DeviceScan::InclusiveScan(tmp, tb, h, h, MaxOp(), m, s));
+		tile_key_kernel<<<grid1(m), kB, 0, s>>>(m, h, G.cols, W, side, S, tile_order == GPHI_TILES_COL_STRIPE, key);
+		if ((rc = stable_order(ctx, sc, key, m, bits_for((uint64_t)side * side - 1), perm))) return rc;
+	}
+	uint32_t *nh = sc.alloc<uint32_t>(G.m_pad), *nt = sc.alloc<uint32_t>(G.m_pad);
+	uint32_t *nw = G.csr_w ? sc.alloc<uint32_t>(G.m_pad) : nullptr;
+	OOM_IF(nh && nt && (nw || !G.csr_w));
+	if (m) {
+		gather_kernel<<<grid1(m), kB, 0, s>>>(m, perm, h, nh);
+		gather_kernel<<<grid1(m), kB, 0, s>>>(m, perm, G.cols, nt);
+		if (nw) gather_kernel<<<grid1(m), kB, 0, s>>>(m, perm, G.csr_w, nw);
+	}
+	GPHI_HIP(ctx, hipStreamSynchronize(s));
+	free_traversal_state(gr);
+	const uint64_t arr = G.m_pad * 4;
+	if (G.heads) G.device_bytes -= arr, (void)hipFree(G.heads);
+	if (G.tails) G.device_bytes -= arr, (void)hipFree(G.tails);
+	if (G.coo_w) G.device_bytes -= arr, (void)hipFree(G.coo_w);
+	if (G.meta) G.device_bytes -= G.nchunks * sizeof(ChunkMeta) + 16, (void)hipFree(G.meta);
+	G.device_bytes += arr * (nw ? 3 : 2);
+	G.heads = nh;
+	G.tails = nt;
+	G.coo_w = nw;
+	sc.keep(nh);
+	sc.keep(nt);
+	if (nw) sc.keep(nw);
+	G.meta = nullptr;
+	G.W = W;
+	G.S = S;
+	G.side = side;
+	G.ntiles = (uint64_t)side * side;
+	G.tile_order = tile_order;
+	G.has_tiles = true;
+	return finish_tiles(ctx, gr);
+}
+
 // -offsets of the stored tile order: histogram of tile positions + scan
 int tile_offsets(gphi_graph *gr, uint32_t *out)
 {

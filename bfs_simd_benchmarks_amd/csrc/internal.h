@@ -116,6 +116,8 @@ int hip_fail(gphi_ctx *ctx, hipError_t e, const char *what, const char *file, in
 // device-side entry points (device.hip)
 int upload_graph(gphi_ctx *ctx, const gphi_graph_desc *d, gphi_graph *gr);
 void free_graph_device(gphi_graph *gr);
+void free_traversal_state(gphi_graph *gr);
+int retile_graph(gphi_graph *gr, uint32_t W, uint32_t S, int tile_order);
 int run_bfs(gphi_graph *gr, uint32_t src, int32_t *levels, int32_t *parents, gphi_stats *st);
 int run_sssp(gphi_graph *gr, uint32_t src, uint32_t *dist, gphi_stats *st);
 int run_pagerank(gphi_graph *gr, uint32_t iters, float damping, float init, float *rank,

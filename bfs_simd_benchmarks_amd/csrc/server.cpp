@@ -1,0 +1,275 @@
+// server.cpp -- query serving over resident graphs with continuous batching.
+//
+// No reference counterpart (the reference runs one traversal per process,
+// apps/bfs_simd/bfs.cpp:1176-1199).  Each admitted graph gets a lane: its own
+// gphi_ctx (HIP stream) and worker thread, so the lanes' kernels overlap on
+// the device.  A lane's worker takes every BFS query pending for its graph
+// when it becomes free (up to max_batch) and answers them with one
+// multi-source traversal (gphi_bfs_batch); queries that arrive meanwhile join
+// the next launch.  SSSP queries run one at a time on the lane.  Graphs are
+// admitted while their HBM bytes plus traversal state fit the budget.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <memory>
+#include <new>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "internal.h"
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+double now_us()
+{
+	return std::chrono::duration<double, std::micro>(Clock::now().time_since_epoch()).count();
+}
+
+struct Query {
+	uint64_t ticket;
+	uint32_t kind;  // GPHI_QUERY_*
+	uint32_t src;
+	void *out;      // caller buffer (int32 levels / uint32 distances), may be null
+	double t_submit = 0, t_start = 0, t_done = 0;
+	uint32_t batch = 0;
+	int status = GPHI_OK;
+	bool done = false;
+};
+
+struct Lane {
+	gphi_graph *g = nullptr;
+	gphi_ctx *ctx = nullptr;
+	uint32_t n = 0;
+	std::deque<std::shared_ptr<Query>> pending;
+	std::thread worker;
+	uint64_t device_bytes = 0;
+	uint64_t batches = 0, served = 0;
+};
+
+}  // namespace
+
+struct gphi_server {
+	gphi_server_opts opts{};
+	std::mutex mu;
+	std::condition_variable cv_work, cv_done;
+	std::vector<std::unique_ptr<Lane>> lanes;
+	std::unordered_map<uint64_t, std::shared_ptr<Query>> queries;
+	uint64_t next_ticket = 1;
+	uint64_t hbm_used = 0;
+	bool stopping = false;
+};
+
+namespace gphi {
+namespace {
+
+// state a lane needs besides the graph: single-source arrays (~42 B per
+// vertex) plus batched masks and levels for max_batch sources
+uint64_t lane_state_bytes(const DevGraph &G, uint32_t max_batch)
+{
+	const uint64_t nw = max_batch <= 64 ? 1 : max_batch <= 128 ? 2 : 4;
+	return (uint64_t)G.n_pad * (42 + 3 * 8 * nw + 64 * nw) + G.m / 256;
+}
+
+void lane_loop(gphi_server *S, Lane *L)
+{
+	std::vector<uint32_t> srcs;
+	std::vector<uint8_t> lv8;
+	std::vector<std::shared_ptr<Query>> take;
+	for (;;) {
+		take.clear();
+		{
+			std::unique_lock<std::mutex> lk(S->mu);
+			S->cv_work.wait(lk, [&] { return S->stopping || !L->pending.empty(); });
+			if (L->pending.empty()) return;  // stopping and drained
+			// one SSSP query alone, or every pending BFS query (up to max_batch)
+			if (L->pending.front()->kind == GPHI_QUERY_SSSP) {
+				take.push_back(L->pending.front());
+				L->pending.pop_front();
+			} else {
+				for (auto it = L->pending.begin(); it != L->pending.end() && take.size() < S->opts.max_batch;) {
+					if ((*it)->kind == GPHI_QUERY_BFS) {
+						take.push_back(*it);
+						it = L->pending.erase(it);
+					} else {
+						++it;
+					}
+				}
+			}
+		}
+		const double t0 = now_us();
+		int rc = GPHI_OK;
+		if (take[0]->kind == GPHI_QUERY_SSSP) {
+			rc = gphi_sssp(L->g, take[0]->src, (uint32_t *)take[0]->out, nullptr);
+		} else if (take.size() == 1 && S->opts.max_batch == 1) {
+			rc = gphi_bfs(L->g, take[0]->src, (int32_t *)take[0]->out, nullptr, nullptr);
+		} else {
+			srcs.resize(take.size());
+			bool want = false;
+			for (size_t i = 0; i < take.size(); ++i) {
+				srcs[i] = take[i]->src;
+				want |= take[i]->out != nullptr;
+			}
+			if (want) lv8.resize((size_t)take.size() * L->n);
+			rc = gphi_bfs_batch(L->g, srcs.data(), (uint32_t)take.size(), want ? lv8.data() : nullptr, nullptr);
+			if (rc == GPHI_OK && want) {
+				for (size_t i = 0; i < take.size(); ++i) {
+					if (!take[i]->out) continue;
+					int32_t *o = (int32_t *)take[i]->out;
+					const uint8_t *r = lv8.data() + i * L->n;
+					for (uint32_t v = 0; v < L->n; ++v) o[v] = r[v] == 255 ? -1 : (int32_t)r[v];
+				}
+			}
+		}
+		const double t1 = now_us();
+		{
+			std::lock_guard<std::mutex> lk(S->mu);
+			for (auto &q : take) {
+				q->t_start = t0;
+				q->t_done = t1;
+				q->batch = (uint32_t)take.size();
+				q->status = rc;
+				q->done = true;
+			}
+			L->batches += 1;
+			L->served += take.size();
+		}
+		S->cv_done.notify_all();
+	}
+}
+
+}  // namespace
+}  // namespace gphi
+
+using gphi::set_error;
+
+extern "C" {
+
+int gphi_server_create(const gphi_server_opts *opts, gphi_server **out)
+{
+	if (!out) return set_error(nullptr, GPHI_E_ARG, "out is NULL");
+	*out = nullptr;
+	gphi_server *S = new (std::nothrow) gphi_server();
+	if (!S) return set_error(nullptr, GPHI_E_OOM, "out of host memory");
+	if (opts) S->opts = *opts;
+	if (S->opts.max_batch == 0) S->opts.max_batch = 64;
+	if (S->opts.max_batch > 256) {
+		delete S;
+		return set_error(nullptr, GPHI_E_ARG, "max_batch must be <= 256");
+	}
+	if (S->opts.hbm_budget == 0) {
+		size_t fr = 0, tot = 0;
+		if (hipMemGetInfo(&fr, &tot) == hipSuccess) S->opts.hbm_budget = (uint64_t)(tot * 0.9);
+	}
+	*out = S;
+	return GPHI_OK;
+}
+
+int gphi_server_add_graph(gphi_server *S, gphi_graph *g, uint32_t *graph_id)
+{
+	if (!S || !g || !graph_id) return set_error(nullptr, GPHI_E_ARG, "NULL argument");
+	const uint64_t need = g->g.device_bytes + gphi::lane_state_bytes(g->g, S->opts.max_batch);
+	std::lock_guard<std::mutex> lk(S->mu);
+	if (S->hbm_used + need > S->opts.hbm_budget)
+		return set_error(nullptr, GPHI_E_OOM,
+		                 "graph needs %llu MB of HBM; %llu MB of the %llu MB budget are in use",
+		                 (unsigned long long)(need >> 20), (unsigned long long)(S->hbm_used >> 20),
+		                 (unsigned long long)(S->opts.hbm_budget >> 20));
+	auto L = std::make_unique<Lane>();
+	L->g = g;
+	L->ctx = g->ctx;
+	L->n = g->g.n;
+	L->device_bytes = need;
+	S->hbm_used += need;
+	Lane *lp = L.get();
+	*graph_id = (uint32_t)S->lanes.size();
+	S->lanes.push_back(std::move(L));
+	lp->worker = std::thread(gphi::lane_loop, S, lp);
+	return GPHI_OK;
+}
+
+int gphi_server_submit(gphi_server *S, uint32_t graph_id, uint32_t kind, uint32_t src, void *out,
+                       uint64_t *ticket)
+{
+	if (!S || !ticket) return set_error(nullptr, GPHI_E_ARG, "NULL argument");
+	auto q = std::make_shared<Query>();
+	{
+		std::lock_guard<std::mutex> lk(S->mu);
+		if (graph_id >= S->lanes.size()) return set_error(nullptr, GPHI_E_RANGE, "no graph %u", graph_id);
+		Lane *L = S->lanes[graph_id].get();
+		if (src >= L->n) return set_error(nullptr, GPHI_E_RANGE, "source %u >= N %u", src, L->n);
+		if (kind != GPHI_QUERY_BFS && kind != GPHI_QUERY_SSSP)
+			return set_error(nullptr, GPHI_E_ARG, "unknown query kind %u", kind);
+		if (kind == GPHI_QUERY_SSSP && !L->g->g.weighted)
+			return set_error(nullptr, GPHI_E_STATE, "graph %u carries no weights", graph_id);
+		if (S->stopping) return set_error(nullptr, GPHI_E_STATE, "server is stopping");
+		q->ticket = S->next_ticket++;
+		q->kind = kind;
+		q->src = src;
+		q->out = out;
+		q->t_submit = now_us();
+		S->queries[q->ticket] = q;
+		L->pending.push_back(q);
+		*ticket = q->ticket;
+	}
+	S->cv_work.notify_all();
+	return GPHI_OK;
+}
+
+int gphi_server_wait(gphi_server *S, uint64_t ticket, gphi_query_info *info)
+{
+	if (!S) return set_error(nullptr, GPHI_E_ARG, "server is NULL");
+	std::unique_lock<std::mutex> lk(S->mu);
+	auto it = S->queries.find(ticket);
+	if (it == S->queries.end()) return set_error(nullptr, GPHI_E_ARG, "unknown ticket %llu", (unsigned long long)ticket);
+	std::shared_ptr<Query> q = it->second;
+	S->cv_done.wait(lk, [&] { return q->done; });
+	if (info) {
+		info->latency_us = q->t_done - q->t_submit;
+		info->queue_us = q->t_start - q->t_submit;
+		info->run_us = q->t_done - q->t_start;
+		info->batch = q->batch;
+	}
+	S->queries.erase(it);
+	return q->status;
+}
+
+int gphi_server_stats(gphi_server *S, uint32_t graph_id, uint64_t *batches, uint64_t *served,
+                      uint64_t *hbm_bytes)
+{
+	if (!S) return set_error(nullptr, GPHI_E_ARG, "server is NULL");
+	std::lock_guard<std::mutex> lk(S->mu);
+	if (graph_id >= S->lanes.size()) return set_error(nullptr, GPHI_E_RANGE, "no graph %u", graph_id);
+	const Lane *L = S->lanes[graph_id].get();
+	if (batches) *batches = L->batches;
+	if (served) *served = L->served;
+	if (hbm_bytes) *hbm_bytes = L->device_bytes;
+	return GPHI_OK;
+}
+
+void gphi_server_destroy(gphi_server *S)
+{
+	if (!S) return;
+	{
+		std::lock_guard<std::mutex> lk(S->mu);
+		S->stopping = true;
+	}
+	S->cv_work.notify_all();
+	for (auto &L : S->lanes)
+		if (L->worker.joinable()) L->worker.join();
+	std::vector<gphi_ctx *> ctxs;
+	for (auto &L : S->lanes) {
+		gphi_graph_free(L->g);
+		if (std::find(ctxs.begin(), ctxs.end(), L->ctx) == ctxs.end()) ctxs.push_back(L->ctx);
+	}
+	for (gphi_ctx *c : ctxs) gphi_close(c);
+	delete S;
+}
+
+}  // extern "C"

@@ -133,6 +133,39 @@ int gphi_bellman_ford(gphi_graph *g, uint32_t src, uint32_t max_sweeps, uint32_t
 int gphi_bfs_batch(gphi_graph *g, const uint32_t *srcs, uint32_t k, uint8_t *levels,
                    gphi_stats *st);
 
+/* ---- serving: resident graphs, continuous batching (no reference counterpart) ----
+ * Each added graph becomes a lane with the graph's own context (HIP stream)
+ * and a worker thread; lanes run concurrently on the device.  A lane answers
+ * all BFS queries pending for its graph when it becomes free (up to
+ * max_batch) with one gphi_bfs_batch launch; queries arriving meanwhile join
+ * the next launch.  SSSP queries run one at a time.  A graph is admitted only
+ * while its HBM bytes plus traversal state fit hbm_budget (GPHI_E_OOM
+ * otherwise).  The server takes ownership of each added graph and of its
+ * context (create one context per graph). */
+typedef struct gphi_server gphi_server;
+typedef struct gphi_server_opts {
+	uint32_t max_batch;   /* BFS queries per launch, 1..256 (0 -> 64); 1 = no batching */
+	uint64_t hbm_budget;  /* bytes; 0 -> 90% of device memory */
+} gphi_server_opts;
+#define GPHI_QUERY_BFS 0
+#define GPHI_QUERY_SSSP 1
+typedef struct gphi_query_info {
+	double latency_us;  /* submit -> result */
+	double queue_us;    /* submit -> start of its launch */
+	double run_us;      /* its launch */
+	uint32_t batch;     /* queries answered by that launch */
+} gphi_query_info;
+int gphi_server_create(const gphi_server_opts *opts, gphi_server **out);
+int gphi_server_add_graph(gphi_server *s, gphi_graph *g, uint32_t *graph_id);
+/* out: NULL, or n int32 levels (BFS, -1 unreached) / n uint32 distances (SSSP),
+ * written before the query completes */
+int gphi_server_submit(gphi_server *s, uint32_t graph_id, uint32_t kind, uint32_t src, void *out,
+                       uint64_t *ticket);
+int gphi_server_wait(gphi_server *s, uint64_t ticket, gphi_query_info *info);
+int gphi_server_stats(gphi_server *s, uint32_t graph_id, uint64_t *batches, uint64_t *served,
+                      uint64_t *hbm_bytes);
+void gphi_server_destroy(gphi_server *s);
+
 /* ---- synthetic inputs and the tool pipeline on the device ----
  * The reference's inputs came from external generators (PBBS rMat,
  * tools/adj2snap/adj2snap.cpp; Galois graph-convert).  gphi_graph_generate
@@ -185,6 +218,11 @@ int gphi_host_from_device(gphi_graph *g, gphi_host_graph **out);
 int gphi_host_write_text(const gphi_host_graph *h, const char *prefix);
 int gphi_host_desc(const gphi_host_graph *h, gphi_graph_desc *out);
 void gphi_host_free(gphi_host_graph *h);
+
+/* Rebuild the tile arrays from the graph's CSR with another W, S or order
+ * (tools/coo_tiling + tools/column_major_tile again, on the device); used by
+ * the (W, S) autotuner.  Traversal state is dropped and rebuilt lazily. */
+int gphi_graph_retile(gphi_graph *g, uint32_t W, uint32_t S, int tile_order);
 
 /* Exclusive tile offsets in the stored tile order (the -offsets file);
  * out has SIDE^2 entries. */

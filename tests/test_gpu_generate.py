@@ -71,3 +71,26 @@ def test_generate_c2_pokec_shape(ctx):
     _cmp(d["cols"], ref.cols, "csr")
     _cmp(d["heads"], ref.sh, "tile heads")
     _cmp(d["tails"], ref.st, "tile tails")
+
+
+def test_retile_c3_matches_tool_pipeline(ctx, c3):
+    """Re-tiling a resident graph (the autotuner's step) lays out the same
+    tile arrays as running the tiling tools again with the new (W, S)."""
+    g = WORKLOADS["c3"].generate(ctx, weighted=True)
+    try:
+        for W, S, order in [(256, 4, 0), (4096, 2, 1), (1000, 3, 0), (1024, 16, 0)]:
+            print(f"case: retile c3 W={W} S={S} order={order}", flush=True)
+            ref = O.tile_edgelist(c3.n, c3.eh, c3.et, W, S, c3.ew)
+            g.retile(W, S, order)
+            assert g.side == ref.n // W + (ref.n % W > 0)
+            d = g.download()
+            _cmp(d["heads"], ref.rh if order else ref.sh, "tile heads")
+            _cmp(d["tails"], ref.rt if order else ref.st, "tile tails")
+            _cmp(d["tw"], ref.rw if order else ref.sw, "tile weights")
+            if order == 0:
+                lv, _, _ = g.bfs(0)
+                assert np.array_equal(lv, O.bfs_levels(c3.n, c3.deg, c3.cols, 0))
+                dist, _ = g.sssp(0)
+                assert np.array_equal(dist, O.sssp_dijkstra(c3.n, c3.deg, c3.cols, c3.cw, 0))
+    finally:
+        g.free()

@@ -1,0 +1,83 @@
+"""Serving loop (gphi_server_*): queries on several resident graphs submitted
+together are answered with batched launches that overlap across graphs, and
+each result equals the query's result alone (oracle / single-source run)."""
+import threading
+
+import numpy as np
+import pytest
+
+import oracle as O
+import bfs_simd_benchmarks_amd as G
+from bfs_simd_benchmarks_amd.workloads import pick_sources
+
+pytestmark = pytest.mark.gpu
+
+
+def lane(g, weighted=False):
+    ctx = G.Context(0)
+    return ctx.graph_from_arrays(g.n, g.deg, W=g.W, S=g.S, csr_edges=g.cols,
+                                 csr_weights=g.cw if weighted else None, tile_heads=g.sh,
+                                 tile_tails=g.st, tile_weights=g.sw if weighted else None,
+                                 tile_offsets=g.soff)
+
+
+@pytest.mark.parametrize("max_batch", [64, 1])
+def test_mixed_queries_equal_alone(c1, c3, max_batch):
+    print(f"case: server max_batch={max_batch}", flush=True)
+    n2, h2, t2 = O.config_edges("c1")
+    g2 = O.build_from_edgelist(n2, h2[::3], t2[::3], 512, 8)     # a third graph, other tiling
+    srv = G.Server(max_batch=max_batch)
+    try:
+        ids = [srv.add_graph(lane(c1)), srv.add_graph(lane(c3, weighted=True)), srv.add_graph(lane(g2))]
+        graphs = [c1, c3, g2]
+        rng = np.random.default_rng(5)
+        jobs = []
+        for q in range(90):
+            gi = int(rng.integers(0, 3))
+            kind = G.QUERY_SSSP if (gi == 1 and q % 3 == 0) else G.QUERY_BFS
+            src = int(pick_sources(graphs[gi].deg, 1 + q % 7, q)[-1]) if q % 5 else int(rng.integers(0, graphs[gi].n))
+            out = np.empty(graphs[gi].n, np.int32 if kind == G.QUERY_BFS else np.uint32)
+            jobs.append((gi, kind, src, out))
+        tickets = []
+        lock = threading.Lock()
+
+        def submit(part):
+            for gi, kind, src, out in part:
+                t = srv.submit(ids[gi], kind, src, out)
+                with lock:
+                    tickets.append(t)
+        ths = [threading.Thread(target=submit, args=(jobs[i::3],)) for i in range(3)]
+        for th in ths:
+            th.start()
+        for th in ths:
+            th.join()
+        infos = [srv.wait(t) for t in tickets]
+        for gi, kind, src, out in jobs:
+            g = graphs[gi]
+            if kind == G.QUERY_BFS:
+                assert np.array_equal(out, O.bfs_levels(g.n, g.deg, g.cols, src)), (gi, src)
+            else:
+                assert np.array_equal(out, O.sssp_dijkstra(g.n, g.deg, g.cols, g.cw, src)), (gi, src)
+        launches = sum(srv.stats(i)["launches"] for i in ids)
+        served = sum(srv.stats(i)["served"] for i in ids)
+        assert served == len(jobs)
+        if max_batch == 1:
+            assert launches == len(jobs)
+            assert max(i.batch for i in infos) == 1
+        else:
+            assert launches < len(jobs) and max(i.batch for i in infos) > 1
+    finally:
+        srv.close()
+
+
+def test_admission_by_hbm_budget(c1):
+    srv = G.Server(max_batch=64, hbm_budget=1 << 20)       # 1 MiB: nothing fits
+    gg = lane(c1)
+    try:
+        with pytest.raises(G.GphiError) as ei:
+            srv.add_graph(gg)
+        assert ei.value.code == -4
+    finally:
+        gg.free()
+        gg.ctx.close()
+        srv.close()
