@@ -127,6 +127,8 @@ def _declare(lib):
         "gphi_graph_download": (i32, [_P, _P, _P, _P, _P, _P, _P]),
         "gphi_graph_tile_offsets": (i32, [_P, _P]),
         "gphi_graph_retile": (i32, [_P, u32, u32, i32]),
+        "gphi_graph_save": (i32, [_P, C.c_char_p]),
+        "gphi_graph_load": (i32, [_P, C.c_char_p, C.POINTER(_P)]),
         "gphi_host_read_text": (i32, [C.c_char_p, u32, u32, i32, i32, C.POINTER(_P)]),
         "gphi_host_from_device": (i32, [_P, C.POINTER(_P)]),
         "gphi_host_write_text": (i32, [_P, C.c_char_p]),
@@ -321,6 +323,12 @@ class Context:
                                                  C.byref(h)))
         return Graph(self, h)
 
+    def load_binary(self, path) -> "Graph":
+        """A graph cache file written by Graph.save (gphi_graph_load)."""
+        h = _P()
+        self.check(self.lib.gphi_graph_load(self.h, str(path).encode(), C.byref(h)))
+        return Graph(self, h)
+
     def load_text_rowmajor(self, prefix: str, W: int, S: int) -> "Graph":
         """prefix_coo-tiled-W-* (apps/pageRank_simd/page_rank.cpp:468-568)."""
         h = _P()
@@ -404,6 +412,10 @@ class Graph:
         """Rebuild the tile arrays from the CSR with another (W, S) or order."""
         self.ctx.check(self.ctx.lib.gphi_graph_retile(self.h, W, S, tile_order))
         self.W, self.S = W, S
+
+    def save(self, path):
+        """Write the binary cache file of this graph (gphi_graph_save)."""
+        self.ctx.check(self.ctx.lib.gphi_graph_save(self.h, str(path).encode()))
 
     def to_host(self) -> HostGraph:
         """Download into a HostGraph (writable as the reference's text files)."""

@@ -224,6 +224,14 @@ void gphi_host_free(gphi_host_graph *h);
  * the (W, S) autotuner.  Traversal state is dropped and rebuilt lazily. */
 int gphi_graph_retile(gphi_graph *g, uint32_t W, uint32_t S, int tile_order);
 
+/* Binary tiled-graph cache: one file of raw uint32 arrays behind a 64-byte
+ * header with a checksum.  Replaces re-parsing the 64 text parts that
+ * graph_input reads on every run (apps/bfs_simd/bfs.cpp:968-1118) and the
+ * per-(W, S) re-reads of tools/tile_size/process.sh:19-21.  gphi_graph_load
+ * validates the header, the file size, the checksum and every id. */
+int gphi_graph_save(gphi_graph *g, const char *path);
+int gphi_graph_load(gphi_ctx *ctx, const char *path, gphi_graph **out);
+
 /* Exclusive tile offsets in the stored tile order (the -offsets file);
  * out has SIDE^2 entries. */
 int gphi_graph_tile_offsets(gphi_graph *g, uint32_t *out);
