@@ -3,10 +3,15 @@
 // Each vertex carries k-bit masks (NW 64-bit words): the sources that have
 // reached it (seen), that reached it in the last step (front) and that reach
 // it in the current step (next).  One edge stream serves every source:
-//   dense step : the column-stripe tile sweep of the single-source BFS
-//                (apps/bfs_simd/bfs.cpp:181-388), with next[t] |= front[h] & ~seen[t]
+//   dense step : a pull over the in-edges (next[t] = OR of front[h] over the
+//                in-neighbours h, restricted to the sources t has not seen and
+//                whose frontier is non-empty; a vertex stops scanning once all
+//                of those are found) -- the bottom-up counterpart of the
+//                reference's dense sweep (apps/bfs_simd/bfs.cpp:181-388), with
+//                no atomics: a lane owns a vertex, or a wavefront owns a piece
+//                of a high in-degree vertex
 //   sparse step: the CSR push (:510-711) from the vertices whose front mask is
-//                non-empty
+//                non-empty, next[t] |= front[h] & ~seen[t]
 // and the direction rule |F| + sum(outdeg F) > M/T_RATIO (:829-834) is applied
 // to the union frontier F.  A vertex v gets source s at level L+1 exactly when
 // an in-neighbour had s in its front mask at level L and v had not seen s, so
@@ -66,36 +71,118 @@ __device__ __forceinline__ void push_mask(const Mask<NW> &f, uint32_t t, const u
 	}
 }
 
-template <int NW>
-__global__ __launch_bounds__(kBlock) void batch_dense_kernel(
-    const uint32_t *__restrict__ heads, const uint32_t *__restrict__ tails,
-    const uint32_t *__restrict__ lists, uint32_t list_cap, const uint32_t *__restrict__ counts,
-    const uint32_t *__restrict__ cur, const u64 *__restrict__ front, const u64 *__restrict__ seen,
-    u64 *next)
+__device__ __forceinline__ u64 wave_or(u64 x)
 {
-	const uint32_t xcd = blockIdx.x % kNumXcd, per = gridDim.x / kNumXcd;
-	const uint32_t cnt = counts[xcd];
-	const uint32_t *lst = lists + (uint64_t)xcd * list_cap;
-	for (uint32_t i = blockIdx.x / kNumXcd; i < cnt; i += per) {
-		const uint32_t c = lst[i];
-		const uint4 *H = reinterpret_cast<const uint4 *>(heads + (uint64_t)c * kChunk);
-		const uint4 *T = reinterpret_cast<const uint4 *>(tails + (uint64_t)c * kChunk);
-		const uint4 h0 = H[threadIdx.x], h1 = H[threadIdx.x + kBlock];
-		const uint32_t h[8] = {h0.x, h0.y, h0.z, h0.w, h1.x, h1.y, h1.z, h1.w};
-		// union-frontier bit of the head first (4 MB bitmap, cache resident)
-		uint32_t hit = 0;
 #pragma unroll
-		for (int j = 0; j < 8; ++j) hit |= (uint32_t)test_bit32(cur, h[j]) << j;
-		if (!hit) continue;
-		uint4 t0 = make_uint4(0, 0, 0, 0), t1 = t0;
-		if (hit & 0x0Fu) t0 = T[threadIdx.x];
-		if (hit & 0xF0u) t1 = T[threadIdx.x + kBlock];
-		const uint32_t t[8] = {t0.x, t0.y, t0.z, t0.w, t1.x, t1.y, t1.z, t1.w};
+	for (int o = 32; o > 0; o >>= 1) x |= __shfl_xor(x, o, 64);
+	return x;
+}
+
+// a value every lane holds, as a scalar (keeps loops over it uniform)
+__device__ __forceinline__ u64 uniform64(u64 x)
+{
+	// readfirstlane returns int: go through uint32_t so the low half is not sign-extended
+	const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(x >> 32));
+	const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)x);
+	return ((u64)hi << 32) | (u64)lo;
+}
+
+// Pull for vertices with 1..kPullLight in-edges, a lane per vertex.  want =
+// sources not yet seen at v whose frontier is non-empty; the scan stops once
+// every wanted source is found.  next[v] is zero on entry and owned by this
+// lane, so a plain store suffices.
+template <int NW>
+__global__ __launch_bounds__(kBlock) void batch_pull_light_kernel(
+    uint32_t n, const uint32_t *__restrict__ in_off, const uint32_t *__restrict__ in_src,
+    const uint32_t *__restrict__ cur, const u64 *__restrict__ front, const u64 *__restrict__ seen,
+    const u64 *__restrict__ live, u64 *__restrict__ next)
+{
+	const uint32_t v = blockIdx.x * kBlock + threadIdx.x;
+	if (v >= n) return;
+	const uint32_t e0 = in_off[v], e1 = in_off[v + 1];
+	if (e1 == e0 || e1 - e0 > kPullLight) return;
+	u64 want[NW], acc[NW], any = 0;
 #pragma unroll
-		for (int j = 0; j < 8; ++j) {
-			if (!(hit >> j & 1u)) continue;
-			const Mask<NW> f = load_mask<NW>(front, h[j]);
-			push_mask<NW>(f, t[j], seen, next);
+	for (int i = 0; i < NW; ++i) {
+		want[i] = ~seen[(uint64_t)v * NW + i] & live[i];
+		any |= want[i];
+		acc[i] = 0;
+	}
+	if (!any) return;
+	for (uint32_t e = e0; e < e1; ++e) {
+		const uint32_t h = in_src[e];
+		if (!test_bit32(cur, h)) continue;
+		u64 miss = 0;
+#pragma unroll
+		for (int i = 0; i < NW; ++i) {
+			acc[i] |= front[(uint64_t)h * NW + i];
+			miss |= want[i] & ~acc[i];
+		}
+		if (!miss) break;
+	}
+#pragma unroll
+	for (int i = 0; i < NW; ++i) {
+		const u64 x = acc[i] & want[i];
+		if (x) next[(uint64_t)v * NW + i] = x;
+	}
+}
+
+// Pull for vertices with more than kPullLight in-edges: a wavefront per piece
+// of up to kPullPiece in-edges, 256 edges between checks of the stop rule.
+// Pieces of one vertex combine with one atomicOr per word.
+template <int NW>
+__global__ __launch_bounds__(kBlock) void batch_pull_heavy_kernel(
+    uint32_t npieces, const uint2 *__restrict__ pieces, const uint32_t *__restrict__ in_off,
+    const uint32_t *__restrict__ in_src, const uint32_t *__restrict__ cur, const u64 *__restrict__ front,
+    const u64 *__restrict__ seen, const u64 *__restrict__ live, u64 *next)
+{
+	constexpr uint32_t kUnroll = 4;
+	const uint32_t lane = threadIdx.x & 63;
+	const uint32_t nwaves = (gridDim.x * kBlock) >> 6;
+	for (uint32_t p = (blockIdx.x * kBlock + threadIdx.x) >> 6; p < npieces; p += nwaves) {
+		const uint2 pc = pieces[p];
+		const uint32_t v = pc.x;
+		const uint32_t e1 = min(pc.y + kPullPiece, in_off[v + 1]);
+		// want and red are the same in every lane (v is per wavefront); acc is
+		// this lane's own, red the wavefront's OR of acc at the last check
+		u64 want[NW], acc[NW], red[NW], any = 0;
+#pragma unroll
+		for (int i = 0; i < NW; ++i) {
+			want[i] = uniform64(~seen[(uint64_t)v * NW + i] & live[i]);
+			any |= want[i];
+			acc[i] = 0;
+			red[i] = 0;
+		}
+		if (!any) continue;
+		const uint32_t cnt = e1 - pc.y;  // 1..kPullPiece
+		for (uint32_t off = 0; off < cnt; off += 64 * kUnroll) {
+			uint32_t h[kUnroll];
+#pragma unroll
+			for (uint32_t u = 0; u < kUnroll; ++u) {
+				const uint32_t j = off + u * 64 + lane;
+				h[u] = j < cnt ? in_src[pc.y + j] : 0xFFFFFFFFu;
+			}
+#pragma unroll
+			for (uint32_t u = 0; u < kUnroll; ++u) {
+				if (h[u] != 0xFFFFFFFFu && test_bit32(cur, h[u])) {
+#pragma unroll
+					for (int i = 0; i < NW; ++i) acc[i] |= front[(uint64_t)h[u] * NW + i];
+				}
+			}
+			u64 miss = 0;
+#pragma unroll
+			for (int i = 0; i < NW; ++i) {
+				if (want[i] & ~red[i]) red[i] = uniform64(wave_or(acc[i]));  // a uniform branch
+				miss |= want[i] & ~red[i];
+			}
+			if (!miss) break;
+		}
+		if (lane == 0) {
+#pragma unroll
+			for (int i = 0; i < NW; ++i) {
+				const u64 x = red[i] & want[i];
+				if (x) atomicOr(next + (uint64_t)v * NW + i, x);
+			}
 		}
 	}
 }
@@ -145,47 +232,89 @@ __global__ __launch_bounds__(kBlock) void batch_sparse_kernel(
 
 // Per vertex: new = next & ~seen; seen |= new; front = new; next = 0; the
 // level of every new source is L+1; flag marks the union frontier for the
-// shared mark/scan/emit update.
+// shared mark/scan/emit update.  Level bytes are written one source at a time
+// across the wavefront (64 consecutive vertices, one coalesced store), and each
+// block publishes the sources it saw advance into one of kLiveSlots slots.
+constexpr uint32_t kLiveSlots = 64;
+constexpr uint32_t kLiveStride = 16;  // u64 words per slot (one 128-byte line)
+
 template <int NW>
-__global__ __launch_bounds__(kBlock) void batch_update_kernel(uint32_t n, uint32_t n_pitch, uint32_t k,
-                                                              uint32_t next_level, u64 *__restrict__ front,
-                                                              u64 *__restrict__ seen, u64 *__restrict__ next,
-                                                              uint8_t *__restrict__ level,
-                                                              uint8_t *__restrict__ flag)
+__global__ __launch_bounds__(kBlock) void batch_update_kernel(uint32_t n, uint32_t n_pitch, uint32_t next_level,
+                                                              u64 *__restrict__ front, u64 *__restrict__ seen,
+                                                              u64 *__restrict__ next, uint8_t *__restrict__ level,
+                                                              uint8_t *__restrict__ flag, u64 *slots)
 {
+	__shared__ u64 s_or[kBlock / 64][NW];
 	const uint32_t v = blockIdx.x * kBlock + threadIdx.x;
-	if (v >= n) return;
+	const bool in = v < n;
+	u64 nwm[NW];
 	bool any = false;
 #pragma unroll
 	for (int i = 0; i < NW; ++i) {
-		const uint64_t idx = (uint64_t)v * NW + i;
-		const u64 nx = next[idx];
-		const u64 sn = seen[idx];
-		const u64 nw = nx & ~sn;
-		const u64 fr = front[idx];
-		if (nw | fr) front[idx] = nw;
-		if (nx) next[idx] = 0;
-		if (nw) {
-			seen[idx] = sn | nw;
-			any = true;
-			for (u64 b = nw; b; b &= b - 1) {
-				const uint32_t s = (uint32_t)i * 64 + (uint32_t)__ffsll((long long)b) - 1;
-				level[(uint64_t)s * n_pitch + v] = (uint8_t)next_level;
+		nwm[i] = 0;
+		if (in) {
+			const uint64_t idx = (uint64_t)v * NW + i;
+			const u64 nx = next[idx];
+			const u64 sn = seen[idx];
+			const u64 nw = nx & ~sn;
+			const u64 fr = front[idx];
+			if (nw | fr) front[idx] = nw;
+			if (nx) next[idx] = 0;
+			if (nw) {
+				seen[idx] = sn | nw;
+				any = true;
 			}
+			nwm[i] = nw;
+		}
+	}
+	const uint32_t wid = threadIdx.x >> 6;
+#pragma unroll
+	for (int i = 0; i < NW; ++i) {
+		u64 u = uniform64(wave_or(nwm[i]));
+		if ((threadIdx.x & 63) == 0) s_or[wid][i] = u;
+		while (u) {
+			const uint32_t b = (uint32_t)__ffsll((long long)u) - 1;
+			u &= u - 1;
+			if ((nwm[i] >> b) & 1) level[(uint64_t)(i * 64 + b) * n_pitch + v] = (uint8_t)next_level;
 		}
 	}
 	if (any) flag[v] = 1;
-	(void)k;
+	__syncthreads();
+	if (threadIdx.x < NW) {
+		u64 u = 0;
+#pragma unroll
+		for (uint32_t w = 0; w < kBlock / 64; ++w) u |= s_or[w][threadIdx.x];
+		u64 *slot = slots + (uint64_t)(blockIdx.x % kLiveSlots) * kLiveStride + threadIdx.x;
+		// read the slot at L2 (not a stale L1 copy) and add only new sources
+		const u64 have = __hip_atomic_load(slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+		if (u & ~have) atomicOr(slot, u);
+	}
+}
+
+// live = OR of the slots (the sources whose new frontier is non-empty); the
+// slots are cleared for the next step.  One wavefront.
+template <int NW>
+__global__ void batch_live_kernel(u64 *live, u64 *slots)
+{
+	const uint32_t lane = threadIdx.x;
+#pragma unroll
+	for (int i = 0; i < NW; ++i) {
+		u64 *p = slots + (uint64_t)lane * kLiveStride + i;
+		const u64 x = wave_or(*p);
+		*p = 0;
+		if (lane == 0) live[i] = x;
+	}
 }
 
 __global__ void batch_seed_kernel(const uint32_t *__restrict__ srcs, uint32_t k, uint32_t nw,
                                   uint32_t n_pitch, u64 *front, u64 *seen, uint8_t *level,
-                                  uint8_t *flag)
+                                  uint8_t *flag, u64 *live)
 {
 	for (uint32_t i = threadIdx.x; i < k; i += blockDim.x) {
 		const uint32_t v = srcs[i];
 		atomicOr(front + (uint64_t)v * nw + i / 64, 1ull << (i % 64));
 		atomicOr(seen + (uint64_t)v * nw + i / 64, 1ull << (i % 64));
+		atomicOr(live + i / 64, 1ull << (i % 64));
 		level[(uint64_t)i * n_pitch + v] = 0;
 		flag[v] = 1;
 	}
@@ -239,9 +368,11 @@ int batch_loop(gphi_graph *gr, const uint32_t *d_srcs, uint32_t k, gphi_stats *s
 	GPHI_HIP(ctx, hipMemsetAsync(gr->b_next, 0, mask_bytes, s));
 	GPHI_HIP(ctx, hipMemsetAsync(gr->b_level, kUnreached, (size_t)k * G.n_pad, s));
 	GPHI_HIP(ctx, hipMemsetAsync(gr->cur, 0, (size_t)G.n_words * 4, s));
+	GPHI_HIP(ctx, hipMemsetAsync(gr->b_live, 0, (size_t)(1 + kLiveSlots) * kLiveStride * 8, s));
+	u64 *live = gr->b_live, *slots = gr->b_live + kLiveStride;
 	launch_init_blocks(gr);
 	batch_seed_kernel<<<1, kBlock, 0, s>>>(d_srcs, k, NW, G.n_pad, gr->b_front, gr->b_seen, gr->b_level,
-	                                       gr->flag);
+	                                       gr->flag, live);
 	// the sources form the first (union) frontier
 	launch_frontier_update_flags(gr, 0, threshold);
 	GPHI_HIP(ctx, hipGetLastError());
@@ -267,17 +398,18 @@ int batch_loop(gphi_graph *gr, const uint32_t *d_srcs, uint32_t k, gphi_stats *s
 				                          kBlock, 0, s>>>(gr->queue, gr->qoff, gr->tile_lo, hc->qsize, hc->fdeg,
 				                                          G.row_off, G.cols, gr->b_front, gr->b_seen, gr->b_next);
 		} else {
-			uint32_t list_cap = 0;
-			int rc = launch_chunk_select_rows(gr, L, &list_cap);
-			if (rc) return rc;
-			batch_dense_kernel<NW><<<dense_grid(G), kBlock, 0, s>>>(G.heads, G.tails, gr->live, list_cap,
-			                                                      gr->live_cnt, gr->cur, gr->b_front,
-			                                                      gr->b_seen, gr->b_next);
+			batch_pull_light_kernel<NW><<<grid_n(G.n), kBlock, 0, s>>>(G.n, gr->in_off, gr->in_src, gr->cur,
+			                                                         gr->b_front, gr->b_seen, live, gr->b_next);
+			if (gr->npieces)
+				batch_pull_heavy_kernel<NW><<<std::min<uint32_t>((gr->npieces + 3) / 4, 16384), kBlock, 0, s>>>(
+				    gr->npieces, gr->pieces, gr->in_off, gr->in_src, gr->cur, gr->b_front, gr->b_seen, live,
+				    gr->b_next);
 		}
 		GPHI_HIP(ctx, hipGetLastError());
 		if (ev_step) GPHI_HIP(ctx, hipEventRecord(ev[2 + 2 * steps], s));
-		batch_update_kernel<NW><<<grid_n(G.n), kBlock, 0, s>>>(G.n, G.n_pad, k, L + 1, gr->b_front, gr->b_seen,
-		                                                     gr->b_next, gr->b_level, gr->flag);
+		batch_update_kernel<NW><<<grid_n(G.n), kBlock, 0, s>>>(G.n, G.n_pad, L + 1, gr->b_front, gr->b_seen,
+		                                                     gr->b_next, gr->b_level, gr->flag, slots);
+		batch_live_kernel<NW><<<1, 64, 0, s>>>(live, slots);
 		launch_frontier_update_flags(gr, L + 1, threshold);
 		GPHI_HIP(ctx, hipGetLastError());
 		if (ev_step) GPHI_HIP(ctx, hipEventRecord(ev[3 + 2 * steps], s));
@@ -322,12 +454,13 @@ int run_bfs_batch(gphi_graph *gr, const uint32_t *srcs, uint32_t k, uint8_t *lev
 	if (!G.has_csr || !G.has_tiles) return set_error(ctx, GPHI_E_STATE, "graph needs CSR and tile arrays");
 	int rc = ensure_traversal_state(gr);
 	if (rc) return rc;
+	if ((rc = build_in_edges(gr))) return rc;
 	const uint32_t nw = k <= 64 ? 1 : k <= 128 ? 2 : 4;
 	if (gr->batch_words < nw || !gr->b_level) {
-		void *ps[] = {gr->b_front, gr->b_seen, gr->b_next, gr->b_level, gr->b_sums};
+		void *ps[] = {gr->b_front, gr->b_seen, gr->b_next, gr->b_level, gr->b_sums, gr->b_live};
 		for (void *p : ps)
 			if (p) (void)hipFree(p);
-		gr->b_front = gr->b_seen = gr->b_next = gr->b_sums = nullptr;
+		gr->b_front = gr->b_seen = gr->b_next = gr->b_sums = gr->b_live = nullptr;
 		gr->b_level = nullptr;
 		gr->batch_words = 0;
 		const size_t mb = (size_t)G.n_pad * nw * 8;
@@ -336,6 +469,7 @@ int run_bfs_batch(gphi_graph *gr, const uint32_t *srcs, uint32_t k, uint8_t *lev
 		if (e == hipSuccess) e = hipMalloc((void **)&gr->b_next, mb);
 		if (e == hipSuccess) e = hipMalloc((void **)&gr->b_level, (size_t)nw * 64 * G.n_pad);
 		if (e == hipSuccess) e = hipMalloc((void **)&gr->b_sums, 16);
+		if (e == hipSuccess) e = hipMalloc((void **)&gr->b_live, (size_t)(1 + kLiveSlots) * kLiveStride * 8);
 		if (e != hipSuccess)
 			return set_error(ctx, GPHI_E_OOM, "batched BFS buffers (%zu MB): %s",
 			                 (3 * mb + (size_t)nw * 64 * G.n_pad) >> 20, hipGetErrorString(e));

@@ -335,7 +335,8 @@ void free_traversal_state(gphi_graph *gr)
 	               (void **)&gr->queue, (void **)&gr->qoff, (void **)&gr->tile_lo, (void **)&gr->ctrl,
 	               (void **)&gr->blk_cnt, (void **)&gr->blk_base, (void **)&gr->live, (void **)&gr->live_cnt,
 	               (void **)&gr->rank, (void **)&gr->sum, (void **)&gr->contrib, (void **)&gr->b_front,
-	               (void **)&gr->b_seen, (void **)&gr->b_next, (void **)&gr->b_level, (void **)&gr->b_sums};
+	               (void **)&gr->b_seen, (void **)&gr->b_next, (void **)&gr->b_level, (void **)&gr->b_sums,
+	               (void **)&gr->b_live};
 	for (void **p : ps) {
 		if (*p) (void)hipFree(*p);
 		*p = nullptr;
@@ -350,7 +351,8 @@ void free_graph_device(gphi_graph *gr)
 	              gr->level, gr->parent, gr->dist, gr->flag, gr->visited, gr->cur,
 	              gr->row_stamp, gr->col_unvis, gr->queue, gr->qoff, gr->ctrl, gr->rank,
 	              gr->sum, gr->contrib, gr->tile_lo, gr->blk_cnt, gr->blk_base, gr->live, gr->live_cnt,
-	              gr->b_front, gr->b_seen, gr->b_next, gr->b_level, gr->b_sums};
+	              gr->b_front, gr->b_seen, gr->b_next, gr->b_level, gr->b_sums, gr->b_live,
+	              gr->in_off, gr->in_src, gr->pieces};
 	for (void *p : ps)
 		if (p) (void)hipFree(p);
 }

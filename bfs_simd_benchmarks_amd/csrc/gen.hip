@@ -613,6 +613,106 @@ int retile_graph(gphi_graph *gr, uint32_t W, uint32_t S, int tile_order)
 	return finish_tiles(ctx, gr);
 }
 
+namespace {
+// off[v] = first position e of the tail-sorted keys with keys[e] >= v
+__global__ void lower_bound_kernel(uint32_t n, const uint32_t *__restrict__ keys, uint64_t m,
+                                   uint32_t *__restrict__ off)
+{
+	GRID_STRIDE(v, (uint64_t)n + 1)
+	{
+		uint64_t a = 0, b = m;
+		while (a < b) {
+			const uint64_t mid = (a + b) >> 1;
+			if (keys[mid] < v)
+				a = mid + 1;
+			else
+				b = mid;
+		}
+		off[v] = (uint32_t)a;
+	}
+}
+
+__global__ void piece_count_kernel(uint32_t n, const uint32_t *__restrict__ off, uint32_t *__restrict__ pc)
+{
+	GRID_STRIDE(v, (uint64_t)n + 1)
+	{
+		const uint32_t d = v < n ? off[v + 1] - off[v] : 0;
+		pc[v] = d > kPullLight ? (d + kPullPiece - 1) / kPullPiece : 0;
+	}
+}
+
+__global__ void piece_emit_kernel(uint32_t n, const uint32_t *__restrict__ off, const uint32_t *__restrict__ pc,
+                                  const uint32_t *__restrict__ base, uint2 *__restrict__ pieces)
+{
+	GRID_STRIDE(v, n)
+	{
+		for (uint32_t j = 0; j < pc[v]; ++j) pieces[base[v] + j] = make_uint2((uint32_t)v, off[v] + j * kPullPiece);
+	}
+}
+}  // namespace
+
+// In-edge arrays for the pull steps of the multi-source BFS: in_src holds the
+// heads of every edge grouped by tail (ascending heads within a tail, as a
+// stable sort of the CSR order gives), in_off[v] the first in-edge of v, and
+// pieces the (vertex, first edge) of every kPullPiece-edge piece of a vertex
+// with more than kPullLight in-edges.
+int build_in_edges(gphi_graph *gr)
+{
+	gphi_ctx *ctx = gr->ctx;
+	DevGraph &G = gr->g;
+	hipStream_t s = ctx->stream;
+	if (gr->in_off) return GPHI_OK;
+	if (!G.has_csr) return set_error(ctx, GPHI_E_STATE, "in-edge arrays need the CSR");
+	const uint32_t n = G.n;
+	const uint64_t m = G.m;
+	Scratch sc{ctx, {}};
+	uint32_t *off = sc.alloc<uint32_t>((uint64_t)n + 1), *src = sc.alloc<uint32_t>(m);
+	uint32_t *h = sc.alloc<uint32_t>(m), *kout = sc.alloc<uint32_t>(m);
+	uint32_t *pc = sc.alloc<uint32_t>((uint64_t)n + 1), *base = sc.alloc<uint32_t>((uint64_t)n + 1);
+	OOM_IF(off && src && h && kout && pc && base);
+	if (m) {
+		GPHI_HIP(ctx, hipMemsetAsync(h, 0, m * 4, s));
+		row_start_kernel<<<grid1(n), kB, 0, s>>>(n, G.row_off, G.deg, h);
+		size_t tb = 0;
+		GPHI_HIP(ctx, hipcub::DeviceScan::InclusiveScan(nullptr, tb, h, h, MaxOp(), m, s));
+		void *tmp = sc.alloc<uint8_t>(tb);
+		OOM_IF(tmp);
+		GPHI_HIP(ctx, hipcub::DeviceScan::InclusiveScan(tmp, tb, h, h, MaxOp(), m, s));
+		sc.release(tmp);
+		tb = 0;
+		const int eb = bits_for(n ? n - 1 : 0);
+		GPHI_HIP(ctx, hipcub::DeviceRadixSort::SortPairs(nullptr, tb, G.cols, kout, h, src, m, 0, eb, s));
+		tmp = sc.alloc<uint8_t>(tb);
+		OOM_IF(tmp);
+		GPHI_HIP(ctx, hipcub::DeviceRadixSort::SortPairs(tmp, tb, G.cols, kout, h, src, m, 0, eb, s));
+		sc.release(tmp);
+	}
+	lower_bound_kernel<<<grid1((uint64_t)n + 1), kB, 0, s>>>(n, kout, m, off);
+	piece_count_kernel<<<grid1((uint64_t)n + 1), kB, 0, s>>>(n, off, pc);
+	size_t tb = 0;
+	GPHI_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(nullptr, tb, pc, base, (uint64_t)n + 1, s));
+	void *tmp = sc.alloc<uint8_t>(tb);
+	OOM_IF(tmp);
+	GPHI_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(tmp, tb, pc, base, (uint64_t)n + 1, s));
+	uint32_t npieces = 0;
+	GPHI_HIP(ctx, hipMemcpyAsync(&npieces, base + n, 4, hipMemcpyDeviceToHost, s));
+	GPHI_HIP(ctx, hipStreamSynchronize(s));
+	uint2 *pieces = sc.alloc<uint2>((uint64_t)npieces + 1);
+	OOM_IF(pieces);
+	if (npieces) piece_emit_kernel<<<grid1(n), kB, 0, s>>>(n, off, pc, base, pieces);
+	GPHI_HIP(ctx, hipGetLastError());
+	GPHI_HIP(ctx, hipStreamSynchronize(s));
+	sc.keep(off);
+	sc.keep(src);
+	sc.keep(pieces);
+	gr->in_off = off;
+	gr->in_src = src;
+	gr->pieces = pieces;
+	gr->npieces = npieces;
+	gr->in_bytes = ((uint64_t)n + 1) * 4 + m * 4 + ((uint64_t)npieces + 1) * 8;
+	return GPHI_OK;
+}
+
 // -offsets of the stored tile order: histogram of tile positions + scan
 int tile_offsets(gphi_graph *gr, uint32_t *out)
 {

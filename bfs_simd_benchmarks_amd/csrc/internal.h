@@ -19,6 +19,8 @@ constexpr uint32_t kChunk = 2048;
 constexpr uint32_t kBlock = 256;
 constexpr uint32_t kSparseTile = 1024;  // edges per block iteration of the CSR push
 constexpr uint32_t kPadVerts = kBlock * 32;  // vertices per frontier-update block
+constexpr uint32_t kPullLight = 64;     // pull steps: a lane per vertex up to this in-degree,
+constexpr uint32_t kPullPiece = 4096;    // else a wavefront per piece of this many in-edges
 constexpr int kNumXcd = 8;              // MI355X: 8 XCDs, blocks dispatched round-robin
 
 struct Error {
@@ -107,6 +109,12 @@ struct gphi_graph {
 	unsigned long long *b_front = nullptr, *b_seen = nullptr, *b_next = nullptr;
 	uint8_t *b_level = nullptr;
 	unsigned long long *b_sums = nullptr;
+	unsigned long long *b_live = nullptr;  // 2 x words: sources with a non-empty frontier
+	// in-edge arrays of the pull steps (built once, independent of W and S)
+	uint32_t *in_off = nullptr, *in_src = nullptr;
+	uint2 *pieces = nullptr;
+	uint32_t npieces = 0;
+	uint64_t in_bytes = 0;
 };
 
 namespace gphi {
@@ -118,6 +126,7 @@ int upload_graph(gphi_ctx *ctx, const gphi_graph_desc *d, gphi_graph *gr);
 void free_graph_device(gphi_graph *gr);
 void free_traversal_state(gphi_graph *gr);
 int retile_graph(gphi_graph *gr, uint32_t W, uint32_t S, int tile_order);
+int build_in_edges(gphi_graph *gr);
 int run_bfs(gphi_graph *gr, uint32_t src, int32_t *levels, int32_t *parents, gphi_stats *st);
 int run_sssp(gphi_graph *gr, uint32_t src, uint32_t *dist, gphi_stats *st);
 int run_pagerank(gphi_graph *gr, uint32_t iters, float damping, float init, float *rank,

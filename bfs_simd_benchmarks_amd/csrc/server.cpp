@@ -70,11 +70,13 @@ namespace gphi {
 namespace {
 
 // state a lane needs besides the graph: single-source arrays (~42 B per
-// vertex) plus batched masks and levels for max_batch sources
+// vertex), batched masks and levels for max_batch sources, and the in-edge
+// arrays of the batched pull steps (4 B per edge and per vertex, pieces)
 uint64_t lane_state_bytes(const DevGraph &G, uint32_t max_batch)
 {
 	const uint64_t nw = max_batch <= 64 ? 1 : max_batch <= 128 ? 2 : 4;
-	return (uint64_t)G.n_pad * (42 + 3 * 8 * nw + 64 * nw) + G.m / 256;
+	const uint64_t in_edges = max_batch > 1 ? 4 * G.m + 12 * (uint64_t)G.n_pad + G.m / 256 : 0;
+	return (uint64_t)G.n_pad * (42 + 3 * 8 * nw + 64 * nw) + G.m / 256 + in_edges;
 }
 
 void lane_loop(gphi_server *S, Lane *L)
