@@ -87,56 +87,128 @@ __device__ __forceinline__ u64 uniform64(u64 x)
 	return ((u64)hi << 32) | (u64)lo;
 }
 
-// Pull for vertices with 1..kPullLight in-edges, a lane per vertex.  want =
-// sources not yet seen at v whose frontier is non-empty; the scan stops once
-// every wanted source is found.  next[v] is zero on entry and owned by this
-// lane, so a plain store suffices.
+// acc |= front[h] (NW words) with 8- or 16-byte loads
 template <int NW>
+__device__ __forceinline__ void or_front(const u64 *__restrict__ front, uint32_t h, u64 (&acc)[NW])
+{
+	if constexpr (NW == 1) {
+		acc[0] |= front[h];
+	} else {
+		const ulonglong2 *p = reinterpret_cast<const ulonglong2 *>(front) + (uint64_t)h * (NW / 2);
+#pragma unroll
+		for (int i = 0; i < NW / 2; ++i) {
+			const ulonglong2 f = p[i];
+			acc[2 * i] |= f.x;
+			acc[2 * i + 1] |= f.y;
+		}
+	}
+}
+
+// acc |= front[h[j]] for the valid heads h[j] (0xFFFFFFFF = none).  A head
+// outside the frontier has front = 0, so the frontier bitmap is only a
+// filter: worth a 4-byte probe of a cache-resident word per edge when the
+// frontier is small (most heads then skip the gather), not when it is large.
+template <int NW, bool kFilter, uint32_t K>
+__device__ __forceinline__ void gather_fronts(const uint32_t (&h)[K], const uint32_t *__restrict__ cur,
+                                              const u64 *__restrict__ front, u64 (&acc)[NW])
+{
+	uint32_t use = 0;
+#pragma unroll
+	for (uint32_t j = 0; j < K; ++j)
+		if (h[j] != 0xFFFFFFFFu && (!kFilter || test_bit32(cur, h[j]))) use |= 1u << j;
+#pragma unroll
+	for (uint32_t j = 0; j < K; ++j)
+		if ((use >> j) & 1u) or_front<NW>(front, h[j], acc);
+}
+
+// Pull for vertices with 1..kPullLight in-edges.  Each lane first reads its
+// own vertex (in-edge range, want = sources not yet seen there whose frontier
+// is non-empty); the wavefront then serves its vertices that want something,
+// kLightGroup at a time, with kLightGroup lanes per vertex taking every
+// kLightGroup-th in-edge (at most kLightSlots each) and an OR across the
+// group.  next[v] is zero on entry and written only by v's group.
+constexpr uint32_t kLightGroup = 8;
+constexpr uint32_t kLightSlots = kPullLight / kLightGroup;
+
+__device__ __forceinline__ uint32_t shfl_u32(uint32_t x, uint32_t src) { return (uint32_t)__shfl((int)x, (int)src, 64); }
+
+template <int NW, bool kFilter>
 __global__ __launch_bounds__(kBlock) void batch_pull_light_kernel(
     uint32_t n, const uint32_t *__restrict__ in_off, const uint32_t *__restrict__ in_src,
     const uint32_t *__restrict__ cur, const u64 *__restrict__ front, const u64 *__restrict__ seen,
     const u64 *__restrict__ live, u64 *__restrict__ next)
 {
-	const uint32_t v = blockIdx.x * kBlock + threadIdx.x;
-	if (v >= n) return;
-	const uint32_t e0 = in_off[v], e1 = in_off[v + 1];
-	if (e1 == e0 || e1 - e0 > kPullLight) return;
-	u64 want[NW], acc[NW], any = 0;
+	const uint32_t lane = threadIdx.x & 63, g = lane % kLightGroup, grp = lane / kLightGroup;
+	const uint32_t v0 = blockIdx.x * kBlock + (threadIdx.x & ~63u);  // first vertex of the wavefront
+	const uint32_t v = v0 + lane;
+	uint32_t e0 = 0, cnt = 0;
+	u64 want[NW], any = 0;
+	if (v < n) {
+		e0 = in_off[v];
+		cnt = in_off[v + 1] - e0;
+		if (cnt > kPullLight) cnt = 0;
+	}
 #pragma unroll
 	for (int i = 0; i < NW; ++i) {
-		want[i] = ~seen[(uint64_t)v * NW + i] & live[i];
+		want[i] = cnt ? ~seen[(uint64_t)v * NW + i] & live[i] : 0;
 		any |= want[i];
-		acc[i] = 0;
 	}
-	if (!any) return;
-	for (uint32_t e = e0; e < e1; ++e) {
-		const uint32_t h = in_src[e];
-		if (!test_bit32(cur, h)) continue;
-		u64 miss = 0;
+	u64 act = uniform64(__ballot(cnt != 0 && any != 0));
+	while (act) {
+		// the next kLightGroup wanting lanes of the wavefront, one per group
+		uint32_t pick[kLightGroup];
+#pragma unroll
+		for (uint32_t q = 0; q < kLightGroup; ++q) {
+			pick[q] = act ? (uint32_t)__ffsll((long long)act) - 1 : 64u;
+			act &= act - 1;
+		}
+		uint32_t src = pick[0];
+#pragma unroll
+		for (uint32_t q = 1; q < kLightGroup; ++q) src = grp == q ? pick[q] : src;
+		// every lane takes part in every shuffle (a lane left out of one would
+		// hand 0 to the lanes reading it); groups without a vertex drop the result
+		const uint32_t from = src & 63u;
+		const uint32_t ge0 = shfl_u32(e0, from);
+		const uint32_t ocnt = shfl_u32(cnt, from);
+		const uint32_t gcnt = src < 64 ? ocnt : 0u;
+		u64 gwant[NW], acc[NW];
 #pragma unroll
 		for (int i = 0; i < NW; ++i) {
-			acc[i] |= front[(uint64_t)h * NW + i];
-			miss |= want[i] & ~acc[i];
+			gwant[i] = __shfl(want[i], (int)from, 64);
+			acc[i] = 0;
 		}
-		if (!miss) break;
-	}
+		uint32_t h[kLightSlots];
 #pragma unroll
-	for (int i = 0; i < NW; ++i) {
-		const u64 x = acc[i] & want[i];
-		if (x) next[(uint64_t)v * NW + i] = x;
+		for (uint32_t j = 0; j < kLightSlots; ++j) {
+			const uint32_t k = j * kLightGroup + g;
+			h[j] = k < gcnt ? in_src[ge0 + k] : 0xFFFFFFFFu;
+		}
+		gather_fronts<NW, kFilter, kLightSlots>(h, cur, front, acc);
+#pragma unroll
+		for (int i = 0; i < NW; ++i)
+#pragma unroll
+			for (int o = kLightGroup / 2; o > 0; o >>= 1) acc[i] |= __shfl_xor(acc[i], o, kLightGroup);
+		if (g == 0 && gcnt) {
+			const uint64_t gv = v0 + src;
+#pragma unroll
+			for (int i = 0; i < NW; ++i) {
+				const u64 x = acc[i] & gwant[i];
+				if (x) next[gv * NW + i] = x;
+			}
+		}
 	}
 }
 
 // Pull for vertices with more than kPullLight in-edges: a wavefront per piece
-// of up to kPullPiece in-edges, 256 edges between checks of the stop rule.
+// of up to kPullPiece in-edges, 512 edges between checks of the stop rule.
 // Pieces of one vertex combine with one atomicOr per word.
-template <int NW>
+template <int NW, bool kFilter>
 __global__ __launch_bounds__(kBlock) void batch_pull_heavy_kernel(
     uint32_t npieces, const uint2 *__restrict__ pieces, const uint32_t *__restrict__ in_off,
     const uint32_t *__restrict__ in_src, const uint32_t *__restrict__ cur, const u64 *__restrict__ front,
     const u64 *__restrict__ seen, const u64 *__restrict__ live, u64 *next)
 {
-	constexpr uint32_t kUnroll = 4;
+	constexpr uint32_t kUnroll = 8;
 	const uint32_t lane = threadIdx.x & 63;
 	const uint32_t nwaves = (gridDim.x * kBlock) >> 6;
 	for (uint32_t p = (blockIdx.x * kBlock + threadIdx.x) >> 6; p < npieces; p += nwaves) {
@@ -162,13 +234,7 @@ __global__ __launch_bounds__(kBlock) void batch_pull_heavy_kernel(
 				const uint32_t j = off + u * 64 + lane;
 				h[u] = j < cnt ? in_src[pc.y + j] : 0xFFFFFFFFu;
 			}
-#pragma unroll
-			for (uint32_t u = 0; u < kUnroll; ++u) {
-				if (h[u] != 0xFFFFFFFFu && test_bit32(cur, h[u])) {
-#pragma unroll
-					for (int i = 0; i < NW; ++i) acc[i] |= front[(uint64_t)h[u] * NW + i];
-				}
-			}
+			gather_fronts<NW, kFilter, kUnroll>(h, cur, front, acc);
 			u64 miss = 0;
 #pragma unroll
 			for (int i = 0; i < NW; ++i) {
@@ -351,6 +417,18 @@ float ev_ms(hipEvent_t a, hipEvent_t b)
 	return hipEventElapsedTime(&ms, a, b) == hipSuccess ? ms : 0.f;
 }
 
+template <int NW, bool kFilter>
+void launch_pull(gphi_graph *gr, const u64 *live)
+{
+	const DevGraph &G = gr->g;
+	hipStream_t s = gr->ctx->stream;
+	batch_pull_light_kernel<NW, kFilter><<<grid_n(G.n), kBlock, 0, s>>>(
+	    G.n, gr->in_off, gr->in_src, gr->cur, gr->b_front, gr->b_seen, live, gr->b_next);
+	if (gr->npieces)
+		batch_pull_heavy_kernel<NW, kFilter><<<std::min<uint32_t>((gr->npieces + 3) / 4, 16384), kBlock, 0, s>>>(
+		    gr->npieces, gr->pieces, gr->in_off, gr->in_src, gr->cur, gr->b_front, gr->b_seen, live, gr->b_next);
+}
+
 template <int NW>
 int batch_loop(gphi_graph *gr, const uint32_t *d_srcs, uint32_t k, gphi_stats *st, u64 *sums_host)
 {
@@ -398,12 +476,11 @@ int batch_loop(gphi_graph *gr, const uint32_t *d_srcs, uint32_t k, gphi_stats *s
 				                          kBlock, 0, s>>>(gr->queue, gr->qoff, gr->tile_lo, hc->qsize, hc->fdeg,
 				                                          G.row_off, G.cols, gr->b_front, gr->b_seen, gr->b_next);
 		} else {
-			batch_pull_light_kernel<NW><<<grid_n(G.n), kBlock, 0, s>>>(G.n, gr->in_off, gr->in_src, gr->cur,
-			                                                         gr->b_front, gr->b_seen, live, gr->b_next);
-			if (gr->npieces)
-				batch_pull_heavy_kernel<NW><<<std::min<uint32_t>((gr->npieces + 3) / 4, 16384), kBlock, 0, s>>>(
-				    gr->npieces, gr->pieces, gr->in_off, gr->in_src, gr->cur, gr->b_front, gr->b_seen, live,
-				    gr->b_next);
+			// the frontier filter pays while the union frontier is under 1/16 of the vertices
+			if (hc->fsize < G.n / 16)
+				launch_pull<NW, true>(gr, live);
+			else
+				launch_pull<NW, false>(gr, live);
 		}
 		GPHI_HIP(ctx, hipGetLastError());
 		if (ev_step) GPHI_HIP(ctx, hipEventRecord(ev[2 + 2 * steps], s));
