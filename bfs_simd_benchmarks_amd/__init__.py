@@ -83,6 +83,7 @@ class Stats(C.Structure):
     _fields_ = [("steps", C.c_uint32), ("dir", C.c_uint32 * MAX_STEPS),
                 ("fsize", C.c_uint32 * MAX_STEPS), ("fdeg", C.c_uint32 * MAX_STEPS),
                 ("sweep_ms", C.c_float * MAX_STEPS), ("update_ms", C.c_float * MAX_STEPS),
+                ("chunks", C.c_uint32 * MAX_STEPS),
                 ("init_ms", C.c_float), ("total_ms", C.c_float), ("reached", C.c_uint64),
                 ("edges_traversed", C.c_uint64)]
 
@@ -97,6 +98,10 @@ class Stats(C.Structure):
 
     def update_times(self) -> np.ndarray:
         return np.ctypeslib.as_array(self.update_ms)[: self.steps].copy()
+
+    def swept_chunks(self) -> np.ndarray:
+        """2048-edge chunks each dense step swept (0 for sparse steps)."""
+        return np.ctypeslib.as_array(self.chunks)[: self.steps].copy()
 
 
 _lib = None

@@ -481,8 +481,10 @@ __global__ __launch_bounds__(kBlock) void chunk_select_kernel(
 			}
 		}
 	}
-	const uint64_t cpx = (nchunks + kNumXcd - 1) / kNumXcd;
-	const uint32_t x = c < nchunks ? (uint32_t)(c / cpx) : 0;
+	// chunk c goes to XCD c % 8: every region of the edge order is shared by
+	// all XCDs, so the lists stay balanced however the live chunks cluster
+	// (each XCD still walks its chunks in edge order)
+	const uint32_t x = (uint32_t)(c % kNumXcd);
 	for (uint32_t xx = 0; xx < (uint32_t)kNumXcd; ++xx) {
 		const bool mine = live && x == xx;
 		const unsigned long long m = __ballot(mine);
@@ -811,6 +813,7 @@ constexpr uint32_t kScanThreads = 1024;
 __global__ __launch_bounds__(kScanThreads) void frontier_scan_kernel(uint32_t nblk, uint32_t threshold,
                                                                      const uint4 *__restrict__ blk_cnt,
                                                                      uint2 *__restrict__ blk_base,
+                                                                     const uint32_t *__restrict__ live_cnt,
                                                                      StepCtrl *ctrl)
 {
 	__shared__ uint32_t sq[kScanThreads / 64], sd[kScanThreads / 64], sf[kScanThreads / 64];
@@ -857,6 +860,9 @@ __global__ __launch_bounds__(kScanThreads) void frontier_scan_kernel(uint32_t nb
 		ctrl->qsize = qall;
 		ctrl->dir = (uint32_t)(fall + dall) > threshold ? 1u : 0u;
 		ctrl->done = fall == 0;
+		uint32_t chunks = 0;  // chunks the step's dense sweep took (its live lists)
+		for (int x = 0; x < kNumXcd; ++x) chunks += live_cnt[x];
+		ctrl->chunks = chunks;
 	}
 }
 
@@ -1010,13 +1016,15 @@ static int frontier_loop(gphi_graph *gr, uint32_t src, gphi_stats *st)
 		if (ev_step) GPHI_HIP(ctx, hipEventRecord(ev[2 + 2 * steps], s));
 		frontier_mark_kernel<kBfs><<<nblk, kBlock, 0, s>>>(G.W, L + 1, gr->flag, G.deg, gr->level, gr->visited,
 		                                                  gr->cur, gr->row_stamp, gr->col_unvis, gr->blk_cnt);
-		frontier_scan_kernel<<<1, kScanThreads, 0, s>>>(nblk, threshold, gr->blk_cnt, gr->blk_base, gr->ctrl);
+		frontier_scan_kernel<<<1, kScanThreads, 0, s>>>(nblk, threshold, gr->blk_cnt, gr->blk_base, gr->live_cnt,
+	                                              gr->ctrl);
 		frontier_emit_kernel<<<nblk, kBlock, 0, s>>>(gr->ctrl, G.deg, gr->cur, gr->blk_cnt, gr->blk_base, gr->queue,
 		                                            gr->qoff, gr->tile_lo);
 		GPHI_HIP(ctx, hipGetLastError());
 		if (ev_step) GPHI_HIP(ctx, hipEventRecord(ev[3 + 2 * steps], s));
 		GPHI_HIP(ctx, hipMemcpyAsync(hc, gr->ctrl, sizeof(StepCtrl), hipMemcpyDeviceToHost, s));
 		GPHI_HIP(ctx, hipStreamSynchronize(s));
+		if (st && dir && steps < GPHI_MAX_STEPS) st->chunks[steps] = hc->chunks;
 		++steps;
 		if (steps > G.n + 2) return set_error(ctx, GPHI_E_STATE, "frontier loop did not terminate");
 	}
@@ -1058,7 +1066,8 @@ void launch_frontier_update_flags(gphi_graph *gr, uint32_t next_level, uint32_t 
 	frontier_mark_kernel<false><<<nblk, kBlock, 0, s>>>(G.W, next_level, gr->flag, G.deg, gr->level,
 	                                                   gr->visited, gr->cur, gr->row_stamp, gr->col_unvis,
 	                                                   gr->blk_cnt);
-	frontier_scan_kernel<<<1, kScanThreads, 0, s>>>(nblk, threshold, gr->blk_cnt, gr->blk_base, gr->ctrl);
+	frontier_scan_kernel<<<1, kScanThreads, 0, s>>>(nblk, threshold, gr->blk_cnt, gr->blk_base, gr->live_cnt,
+	                                              gr->ctrl);
 	frontier_emit_kernel<<<nblk, kBlock, 0, s>>>(gr->ctrl, G.deg, gr->cur, gr->blk_cnt, gr->blk_base,
 	                                            gr->queue, gr->qoff, gr->tile_lo);
 }

@@ -47,7 +47,8 @@ struct StepCtrl {
 	uint32_t pad0;
 	unsigned long long qpack;  // accumulator: (queue count << 32) | degree sum
 	uint32_t changed;       // hop relaxation: edges that lowered a distance
-	uint32_t pad[5];
+	uint32_t chunks;        // chunks on the live lists of the last dense sweep
+	uint32_t pad[4];
 };
 
 struct DevGraph {

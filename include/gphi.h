@@ -74,6 +74,7 @@ typedef struct gphi_stats {
 	uint32_t fdeg[GPHI_MAX_STEPS];       /* sum of out-degrees of F */
 	float sweep_ms[GPHI_MAX_STEPS];      /* sweep kernel time (HIP events) */
 	float update_ms[GPHI_MAX_STEPS];     /* frontier update kernel time */
+	uint32_t chunks[GPHI_MAX_STEPS];     /* 2048-edge chunks a dense step swept (0 if sparse) */
 	float init_ms;                       /* state initialisation (device) */
 	float total_ms;                      /* init + all steps (device events) */
 	uint64_t reached;                    /* vertices with a finite result */
