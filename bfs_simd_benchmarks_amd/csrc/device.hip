@@ -516,53 +516,22 @@ __device__ __forceinline__ uint4 ld_stream(const uint4 *p)
 	return make_uint4(v.x, v.y, v.z, v.w);
 }
 
-// One chunk, 8 edges per lane (two uint4 of heads).  The frontier bits of the
-// heads are tested first; a lane loads the tails of a 4-edge quad only when
-// one of its heads is in the frontier, so frontier-sparse dense steps read
-// little more than the heads.  Then the visited bit and the flag of each
-// candidate tail; the winner writes parent and flag (benign races: every
-// writer stores a valid frontier head and the same flag).
-__device__ __forceinline__ void bfs_dense_chunk(uint32_t c, const uint4 h0, const uint4 h1,
-                                                const uint32_t *__restrict__ tails,
-                                                const uint32_t *__restrict__ cur,
-                                                const uint32_t *__restrict__ visited,
-                                                uint8_t *__restrict__ flag, int32_t *__restrict__ parent)
+// Software-pipelined walk of this XCD's live-chunk list, 8 edges per lane per
+// chunk.  While chunk k finishes (visited bits, flags, writes), the frontier
+// bits of chunk k+1 are looked up, its tails are requested, and the heads of
+// chunk k+2 are requested, so each iteration waits on about one HBM latency
+// instead of a chain of them.  A wavefront's loads complete in issue order,
+// so within an iteration the loads needed first are issued first, and every
+// load is unconditional (a lane with nothing to fetch reads a cached dummy
+// word) so the compiler can count them.  Tails of a 4-edge quad are fetched
+// only when one of its heads is in the frontier.  The winner of a tail writes
+// parent and flag (benign races: every writer stores a valid frontier head and
+// the same flag).
+__device__ __forceinline__ void unpack8(const uint4 a, const uint4 b, uint32_t (&x)[8])
 {
-	const uint32_t h[8] = {h0.x, h0.y, h0.z, h0.w, h1.x, h1.y, h1.z, h1.w};
-	uint32_t cw[8];
-#pragma unroll
-	for (int j = 0; j < 8; ++j) cw[j] = cur[h[j] >> 5];
-	uint32_t hit = 0;
-#pragma unroll
-	for (int j = 0; j < 8; ++j) hit |= ((cw[j] >> (h[j] & 31)) & 1u) << j;
-	if (!hit) return;
-	const uint4 *T = reinterpret_cast<const uint4 *>(tails + (uint64_t)c * kChunk);
-	uint4 t0 = make_uint4(0, 0, 0, 0), t1 = t0;
-	if (hit & 0x0Fu) t0 = ld_stream(T + threadIdx.x);
-	if (hit & 0xF0u) t1 = ld_stream(T + threadIdx.x + kBlock);
-	const uint32_t t[8] = {t0.x, t0.y, t0.z, t0.w, t1.x, t1.y, t1.z, t1.w};
-	uint32_t vw[8];
-#pragma unroll
-	for (int j = 0; j < 8; ++j) vw[j] = (hit >> j & 1u) ? visited[t[j] >> 5] : 0xFFFFFFFFu;
-	uint32_t cand = 0;
-#pragma unroll
-	for (int j = 0; j < 8; ++j) cand |= (((vw[j] >> (t[j] & 31)) & 1u) ^ 1u) << j;
-	cand &= hit;
-	if (!cand) return;
-	uint8_t fl[8];
-#pragma unroll
-	for (int j = 0; j < 8; ++j) fl[j] = (cand >> j & 1u) ? flag[t[j]] : 1;
-#pragma unroll
-	for (int j = 0; j < 8; ++j)
-		if (!fl[j]) {
-			parent[t[j]] = (int32_t)h[j];
-			flag[t[j]] = 1;
-		}
+	x[0] = a.x, x[1] = a.y, x[2] = a.z, x[3] = a.w, x[4] = b.x, x[5] = b.y, x[6] = b.z, x[7] = b.w;
 }
 
-// Walk of this XCD's live-chunk list; the heads of the next chunk (and the
-// list entry after it) are loaded before the current chunk is processed, so
-// HBM reads overlap the dependent bitmap lookups.
 __global__ __launch_bounds__(kBlock) void bfs_dense_kernel(
     const uint32_t *__restrict__ heads, const uint32_t *__restrict__ tails,
     const uint32_t *__restrict__ lists, uint32_t list_cap, const uint32_t *__restrict__ counts,
@@ -572,30 +541,86 @@ __global__ __launch_bounds__(kBlock) void bfs_dense_kernel(
 	const uint32_t xcd = blockIdx.x % kNumXcd, per = gridDim.x / kNumXcd;
 	const uint32_t cnt = counts[xcd];
 	const uint32_t *lst = lists + (uint64_t)xcd * list_cap;
-	uint32_t i = blockIdx.x / kNumXcd;
-	if (i >= cnt) return;
-	uint32_t c = lst[i];
-	uint32_t c_next = i + per < cnt ? lst[i + per] : 0;
-	const uint4 *H = reinterpret_cast<const uint4 *>(heads + (uint64_t)c * kChunk);
-	uint4 h0 = ld_stream(H + threadIdx.x), h1 = ld_stream(H + threadIdx.x + kBlock);
+	uint32_t pos = blockIdx.x / kNumXcd;
+	if (pos >= cnt) return;
+	const uint32_t tid = threadIdx.x;
+	// list entries: chunks k, k+1, k+2 (an absent one repeats chunk k's id, so
+	// every address stays valid; its results are masked off)
+	const uint32_t c0 = lst[pos];
+	bool v1 = pos + per < cnt, v2 = pos + 2 * per < cnt;
+	uint32_t c1 = v1 ? lst[pos + per] : c0;
+	uint32_t c2 = v2 ? lst[pos + 2 * per] : c0;
+	// prologue: heads and frontier bits of chunk k, then heads of k+1 and tails of k
+	uint32_t h[8], hn[8], t[8];
+	{
+		const uint4 *H = reinterpret_cast<const uint4 *>(heads + (uint64_t)c0 * kChunk);
+		unpack8(ld_stream(H + tid), ld_stream(H + tid + kBlock), h);
+	}
+	uint32_t hit = 0;
+	{
+		uint32_t cw[8];
+#pragma unroll
+		for (int j = 0; j < 8; ++j) cw[j] = cur[h[j] >> 5];
+#pragma unroll
+		for (int j = 0; j < 8; ++j) hit |= ((cw[j] >> (h[j] & 31)) & 1u) << j;
+	}
+	{
+		const uint4 *H = reinterpret_cast<const uint4 *>(heads + (uint64_t)c1 * kChunk);
+		unpack8(ld_stream(H + tid), ld_stream(H + tid + kBlock), hn);
+		const uint4 *T = reinterpret_cast<const uint4 *>(tails + (uint64_t)c0 * kChunk);
+		const uint4 *D = reinterpret_cast<const uint4 *>(tails);  // dummy: first 16 bytes
+		unpack8(ld_stream((hit & 0x0Fu) ? T + tid : D), ld_stream((hit & 0xF0u) ? T + tid + kBlock : D), t);
+	}
 	for (;;) {
-		const uint32_t in = i + per;
-		const bool more = in < cnt;
-		uint4 g0 = make_uint4(0, 0, 0, 0), g1 = g0;
-		uint32_t c_after = 0;
-		if (more) {
-			const uint4 *Hn = reinterpret_cast<const uint4 *>(heads + (uint64_t)c_next * kChunk);
-			g0 = ld_stream(Hn + threadIdx.x);
-			g1 = ld_stream(Hn + threadIdx.x + kBlock);
-			c_after = in + per < cnt ? lst[in + per] : 0;
+		// stage A: visited bits of chunk k's candidate tails, frontier bits of chunk k+1
+		uint32_t vw[8], cw[8];
+#pragma unroll
+		for (int j = 0; j < 8; ++j) vw[j] = visited[((hit >> j) & 1u) ? t[j] >> 5 : 0u];
+#pragma unroll
+		for (int j = 0; j < 8; ++j) cw[j] = cur[hn[j] >> 5];
+		uint32_t cand = 0, hit_n = 0;
+#pragma unroll
+		for (int j = 0; j < 8; ++j) cand |= (((vw[j] >> (t[j] & 31)) & 1u) ^ 1u) << j;
+		cand &= hit;
+#pragma unroll
+		for (int j = 0; j < 8; ++j) hit_n |= ((cw[j] >> (hn[j] & 31)) & 1u) << j;
+		if (!v1) hit_n = 0;
+		// stage B: flags of chunk k's candidates (needed first), then the
+		// HBM requests of the next iteration: tails of k+1, heads of k+2
+		uint8_t fl[8];
+#pragma unroll
+		for (int j = 0; j < 8; ++j) fl[j] = flag[((cand >> j) & 1u) ? t[j] : 0u];
+		uint32_t tn[8], h2[8];
+		{
+			const uint4 *T = reinterpret_cast<const uint4 *>(tails + (uint64_t)c1 * kChunk);
+			const uint4 *D = reinterpret_cast<const uint4 *>(tails);
+			unpack8(ld_stream((hit_n & 0x0Fu) ? T + tid : D), ld_stream((hit_n & 0xF0u) ? T + tid + kBlock : D),
+			        tn);
+			const uint4 *H = reinterpret_cast<const uint4 *>(heads + (uint64_t)c2 * kChunk);
+			unpack8(ld_stream(H + tid), ld_stream(H + tid + kBlock), h2);
 		}
-		bfs_dense_chunk(c, h0, h1, tails, cur, visited, flag, parent);
-		if (!more) break;
-		i = in;
-		c = c_next;
-		c_next = c_after;
-		h0 = g0;
-		h1 = g1;
+		const bool v3 = pos + 3 * per < cnt;
+		const uint32_t c3 = v3 ? lst[pos + 3 * per] : c0;
+		// stage C: chunk k's writes
+#pragma unroll
+		for (int j = 0; j < 8; ++j)
+			if (((cand >> j) & 1u) && !fl[j]) {
+				parent[t[j]] = (int32_t)h[j];
+				flag[t[j]] = 1;
+			}
+		if (!v1) break;
+		pos += per;
+#pragma unroll
+		for (int j = 0; j < 8; ++j) {
+			h[j] = hn[j];
+			t[j] = tn[j];
+			hn[j] = h2[j];
+		}
+		hit = hit_n;
+		c1 = c2;
+		c2 = c3;
+		v1 = v2;
+		v2 = v3;
 	}
 }
 static_assert(kChunk == 2 * 4 * kBlock, "dense sweep loads two uint4 per array per thread");
@@ -929,6 +954,7 @@ __global__ __launch_bounds__(kBlock) void frontier_emit_kernel(
 // host drivers
 // ---------------------------------------------------------------------------
 constexpr unsigned kDenseGrid = 4096;   // multiple of kNumXcd
+constexpr unsigned kBfsDenseGrid = 2048;  // 8 blocks per CU: every block resident at once
 static constexpr unsigned kSparseGrid = 4096;
 
 static float ev_ms(hipEvent_t a, hipEvent_t b)
@@ -1003,7 +1029,8 @@ static int frontier_loop(gphi_graph *gr, uint32_t src, gphi_stats *st)
 				chunk_select_kernel<false><<<sgrid, kBlock, 0, s>>>(G.meta, G.nchunks, gr->row_stamp,
 				                                                  gr->col_unvis, L, gr->live, list_cap,
 				                                                  gr->live_cnt);
-			unsigned grid = (unsigned)std::min<uint64_t>(kDenseGrid, (G.nchunks + kNumXcd - 1) / kNumXcd * kNumXcd);
+			unsigned grid = (unsigned)std::min<uint64_t>(kBfs ? kBfsDenseGrid : kDenseGrid,
+			                                             (G.nchunks + kNumXcd - 1) / kNumXcd * kNumXcd);
 			grid = std::max<unsigned>(grid / kNumXcd * kNumXcd, kNumXcd);
 			if (kBfs)
 				bfs_dense_kernel<<<grid, kBlock, 0, s>>>(G.heads, G.tails, gr->live, list_cap, gr->live_cnt,
