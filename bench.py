@@ -277,11 +277,16 @@ def serving(args) -> int:
                                         int(srcs[q]))
         th = threading.Thread(target=producer)
         th.start()
-        th.join()
         lat, batch = np.empty(args.queries), np.empty(args.queries)
         for q in range(args.queries):
+            while tickets[q] == 0:          # not submitted yet
+                time.sleep(0.0005)
             info = srv.wait(tickets[q])
             lat[q], batch[q] = info.latency_us, info.batch
+            if (q + 1) % 10000 == 0:
+                print(f"[serving] {mode}: {q + 1} answered after {time.perf_counter() - start:.1f} s",
+                      file=sys.stderr, flush=True)
+        th.join()
         wall = time.perf_counter() - start
         bfs_lat = lat[~is_sssp]
         results[mode] = {

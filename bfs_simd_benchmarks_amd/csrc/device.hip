@@ -357,6 +357,14 @@ void free_graph_device(gphi_graph *gr)
 		if (p) (void)hipFree(p);
 }
 
+// Capacity of one XCD's live-chunk list: chunk selection hands out chunks in
+// groups of 64 (a wavefront's), group g to XCD g % 8.
+static uint32_t live_list_cap(uint64_t nchunks)
+{
+	const uint64_t groups = (nchunks + 63) / 64;
+	return (uint32_t)((groups + kNumXcd - 1) / kNumXcd * 64);
+}
+
 // ---------------------------------------------------------------------------
 // traversal state
 // ---------------------------------------------------------------------------
@@ -378,7 +386,7 @@ static int ensure_state(gphi_graph *gr)
 	if ((rc = dmalloc(ctx, (void **)&gr->qoff, nb, nullptr))) return rc;
 	if ((rc = dmalloc(ctx, (void **)&gr->tile_lo, (G.m / kSparseTile + 2) * 4, nullptr))) return rc;
 	if ((rc = dmalloc(ctx, (void **)&gr->ctrl, sizeof(StepCtrl), nullptr))) return rc;
-	const size_t list_cap = (G.nchunks + kNumXcd - 1) / kNumXcd;
+	const size_t list_cap = live_list_cap(G.nchunks);
 	if ((rc = dmalloc(ctx, (void **)&gr->live, list_cap * kNumXcd * 4 + 4, nullptr))) return rc;
 	if ((rc = dmalloc(ctx, (void **)&gr->live_cnt, kNumXcd * 4, nullptr))) return rc;
 	const size_t nblk = G.n_pad / kPadVerts;
@@ -443,9 +451,8 @@ __global__ void init_source_kernel(uint32_t src, uint32_t W, const uint32_t *__r
 // Chunk selection (one lane per chunk): a chunk is live when a head row of
 // it holds a frontier vertex (the reference's is_active_side test,
 // bfs.cpp:275) and, for BFS, a tail column of it still has an unvisited
-// vertex.  Live chunks are appended to one list per XCD range of chunks, so
-// the sweep's blocks on one XCD walk one contiguous region of the tile arrays
-// and that XCD's L2 keeps the bitmap windows of its stripes.
+// vertex.  Live chunks are appended to one list per XCD (the XCD that runs
+// blocks b with b % 8 == x), 64 consecutive chunks at a time.
 template <bool kColSkip>
 __global__ __launch_bounds__(kBlock) void chunk_select_kernel(
     const ChunkMeta *__restrict__ meta, uint64_t nchunks, const uint32_t *__restrict__ row_stamp,
@@ -481,10 +488,11 @@ __global__ __launch_bounds__(kBlock) void chunk_select_kernel(
 			}
 		}
 	}
-	// chunk c goes to XCD c % 8: every region of the edge order is shared by
-	// all XCDs, so the lists stay balanced however the live chunks cluster
-	// (each XCD still walks its chunks in edge order)
-	const uint32_t x = (uint32_t)(c % kNumXcd);
+	// the 64 chunks of wavefront w go to XCD w % 8: every region of the edge
+	// order is shared by all XCDs, so the lists stay balanced however the live
+	// chunks cluster (each XCD still walks its chunks in edge order), and a
+	// wavefront appends with one atomic
+	const uint32_t x = (uint32_t)((c / 64) % kNumXcd);
 	for (uint32_t xx = 0; xx < (uint32_t)kNumXcd; ++xx) {
 		const bool mine = live && x == xx;
 		const unsigned long long m = __ballot(mine);
@@ -1018,7 +1026,7 @@ static int frontier_loop(gphi_graph *gr, uint32_t src, gphi_stats *st)
 				    gr->visited, gr->flag, gr->parent, gr->dist);
 		} else {
 			if (!G.has_tiles) return set_error(ctx, GPHI_E_STATE, "graph has no tile arrays");
-			const uint32_t list_cap = (uint32_t)((G.nchunks + kNumXcd - 1) / kNumXcd);
+			const uint32_t list_cap = live_list_cap(G.nchunks);
 			GPHI_HIP(ctx, hipMemsetAsync(gr->live_cnt, 0, kNumXcd * 4, s));
 			const unsigned sgrid = (unsigned)std::max<uint64_t>(1, (G.nchunks + kBlock - 1) / kBlock);
 			if (kBfs)
@@ -1104,7 +1112,7 @@ int launch_chunk_select_rows(gphi_graph *gr, uint32_t level, uint32_t *list_cap)
 {
 	DevGraph &G = gr->g;
 	hipStream_t s = gr->ctx->stream;
-	*list_cap = (uint32_t)((G.nchunks + kNumXcd - 1) / kNumXcd);
+	*list_cap = live_list_cap(G.nchunks);
 	GPHI_HIP(gr->ctx, hipMemsetAsync(gr->live_cnt, 0, kNumXcd * 4, s));
 	const unsigned sgrid = (unsigned)std::max<uint64_t>(1, (G.nchunks + kBlock - 1) / kBlock);
 	chunk_select_kernel<false><<<sgrid, kBlock, 0, s>>>(G.meta, G.nchunks, gr->row_stamp, gr->col_unvis,
