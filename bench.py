@@ -22,6 +22,7 @@ import json
 import os
 import sys
 import time
+from pathlib import Path
 
 import numpy as np
 
@@ -34,6 +35,33 @@ def sweep_bytes(n: int, e_act: float, t_act: float, f_next: float) -> float:
     4 B per active tile, frontier and visited bitmaps read, parent write per
     discovered vertex."""
     return 8 * e_act + 4 * t_act + 2 * ((n + 7) // 8) + 4 * f_next
+
+
+PROFILES = Path(__file__).resolve().parent / "profiles"
+
+
+def measured_traffic(config: str, W: int, S: int):
+    """HBM bytes per bfs_dense_kernel launch from the committed rocprofv3
+    counter summary of this workload (FETCH_SIZE and WRITE_SIZE in KiB per
+    dispatch, each scaled by what the counter reads for a device copy of known
+    size), or (None, None) when no summary covers it."""
+    from bfs_simd_benchmarks_amd.workloads import WORKLOADS
+    f = PROFILES / f"bfs_dense_kernel_{WORKLOADS[config].name}.md"
+    if not f.exists():
+        return None, None
+    vals = {}
+    for line in f.read_text().splitlines():
+        if "=" in line and line.strip().startswith("traffic."):
+            k, v = line.strip().split("=", 1)
+            vals[k.strip()[len("traffic."):]] = v.strip()
+    try:
+        if (int(vals["W"]), int(vals["S"])) != (W, S):
+            return None, None
+        b = (float(vals["fetch_kib"]) * float(vals["fetch_factor"]) +
+             float(vals["write_kib"]) * float(vals["write_factor"])) * 1024
+    except (KeyError, ValueError):
+        return None, None
+    return round(b), str(f.relative_to(f.parents[1]))
 
 
 def main() -> int:
@@ -107,7 +135,7 @@ def main() -> int:
 
     # roofline of the dense sweep: bytes from the (deterministic) levels of
     # each timed source, durations from the timed runs' HIP events
-    dense_ms, dense_bytes, dense_launches = 0.0, 0.0, 0
+    dense_ms, dense_bytes, dense_launches, select_ms = 0.0, 0.0, 0, 0.0
     step_bytes_total, step_ms_total = 0.0, 0.0
     levels0 = None
     for s, st in zip(timed, stats):
@@ -116,11 +144,12 @@ def main() -> int:
             levels0 = lv
         tr = st2.trace()
         ab = g.alg_bytes(lv, st2)
-        sw = st.sweep_times()
+        sw, sel = st.sweep_times(), st.select_times()
         for k in range(st.steps):
             f_next = float(tr[k + 1][1]) if k + 1 < st.steps else 0.0
             step_bytes_total += ab[k]
-            step_ms_total += float(sw[k]) + float(st.update_times()[k])
+            step_ms_total += float(sw[k]) + float(sel[k]) + float(st.update_times()[k])
+            select_ms += float(sel[k])
             if tr[k][0] == 1:
                 # e_act/t_act are folded into ab; recover the sweep share
                 other = 3 * ((g.n + 7) // 8) + 8 * f_next
@@ -133,11 +162,15 @@ def main() -> int:
     per_launch_bytes = dense_bytes / max(dense_launches, 1)
     per_launch_ms = dense_ms / max(dense_launches, 1)
     achieved = per_launch_bytes / (per_launch_ms * 1e-3) / 1e9 if dense_launches else 0.0
+    traffic, traffic_src = measured_traffic(args.config, g.W, g.S)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "kernel": "bfs_dense_kernel", "launches": dense_launches,
                 "mean_launch_ms": round(per_launch_ms, 4),
-                "alg_bytes_per_launch": round(per_launch_bytes)}
+                "alg_bytes_per_launch": round(per_launch_bytes),
+                "chunk_select_ms_per_launch": round(select_ms / max(dense_launches, 1), 4)}
+    if traffic_src:
+        roofline["traffic_source"] = traffic_src
     whole = step_bytes_total / (step_ms_total * 1e-3) / 1e9 if step_ms_total else 0.0
 
     # batched multi-source BFS vs repeated plain runs (same GPU, same graph)

@@ -82,7 +82,8 @@ REORDER_DEGREE = 1
 class Stats(C.Structure):
     _fields_ = [("steps", C.c_uint32), ("dir", C.c_uint32 * MAX_STEPS),
                 ("fsize", C.c_uint32 * MAX_STEPS), ("fdeg", C.c_uint32 * MAX_STEPS),
-                ("sweep_ms", C.c_float * MAX_STEPS), ("update_ms", C.c_float * MAX_STEPS),
+                ("sweep_ms", C.c_float * MAX_STEPS), ("select_ms", C.c_float * MAX_STEPS),
+                ("update_ms", C.c_float * MAX_STEPS),
                 ("chunks", C.c_uint32 * MAX_STEPS),
                 ("init_ms", C.c_float), ("total_ms", C.c_float), ("reached", C.c_uint64),
                 ("edges_traversed", C.c_uint64)]
@@ -95,6 +96,10 @@ class Stats(C.Structure):
 
     def sweep_times(self) -> np.ndarray:
         return np.ctypeslib.as_array(self.sweep_ms)[: self.steps].copy()
+
+    def select_times(self) -> np.ndarray:
+        """Chunk selection time of each dense step (0 for sparse steps)."""
+        return np.ctypeslib.as_array(self.select_ms)[: self.steps].copy()
 
     def update_times(self) -> np.ndarray:
         return np.ctypeslib.as_array(self.update_ms)[: self.steps].copy()

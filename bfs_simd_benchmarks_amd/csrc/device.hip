@@ -505,16 +505,6 @@ __global__ __launch_bounds__(kBlock) void chunk_select_kernel(
 	}
 }
 
-// Walk of this XCD's live-chunk list, two chunks per iteration so each lane
-// has 16 edges of loads in flight.
-#define GPHI_FOR_LIVE(ca, cb, has_b, lists, list_cap, counts)                      \
-	const uint32_t xcd_ = blockIdx.x % kNumXcd, slot_ = blockIdx.x / kNumXcd;        \
-	const uint32_t per_ = gridDim.x / kNumXcd;                                      \
-	const uint32_t cnt_ = counts[xcd_];                                             \
-	const uint32_t *lst_ = lists + (uint64_t)xcd_ * (list_cap);                     \
-	for (uint32_t i_ = slot_; i_ < cnt_; i_ += 2 * per_)                            \
-		if (const uint32_t ca = lst_[i_], has_b = i_ + per_ < cnt_, cb = has_b ? lst_[i_ + per_] : ca; true)
-
 typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
 // streaming (non-temporal) 16-byte load: the tile arrays are read once per
 // sweep and should not evict the bitmaps and state arrays from L2 / MALL
@@ -643,17 +633,17 @@ static_assert(kChunk == 2 * 4 * kBlock, "dense sweep loads two uint4 per array p
 	for (uint64_t c = xcd_ * cpx_ + slot_; c < cend_; c += per_)
 
 // SSSP relax (apps/sssp_simd/sssp.cpp:344-445): if mask[h] and dist[h]+w <
-// dist[t] then lower dist[t] and mark t for the next frontier.
-__device__ __forceinline__ void sssp_relax(uint32_t h, uint32_t t, uint32_t w,
-                                           const uint32_t *__restrict__ cur, uint32_t *dist,
-                                           uint8_t *__restrict__ flag)
+// dist[t] then lower dist[t] and mark t for the next frontier.  The same
+// software pipeline as bfs_dense_kernel: while chunk k is relaxed, the
+// frontier bits of chunk k+1 are looked up and its tails and weights (only
+// for 4-edge quads with a frontier head) and the heads of chunk k+2 are
+// requested.  A lane that sees dist[h]+w below the dist[t] it read issues an
+// atomicMin (no return) and marks t: dist[t] then ends the step below the
+// value read during it, i.e. t improved this step, which is exactly the
+// condition for t to be in the next frontier.
+__device__ __forceinline__ void unpack8u(const uint4 a, const uint4 b, uint32_t (&x)[8])
 {
-	if (!test_bit(cur, h)) return;
-	uint32_t nd = dist[h] + w;
-	if (nd < dist[t]) {
-		uint32_t old = atomicMin(dist + t, nd);
-		if (nd < old && !flag[t]) flag[t] = 1;
-	}
+	x[0] = a.x, x[1] = a.y, x[2] = a.z, x[3] = a.w, x[4] = b.x, x[5] = b.y, x[6] = b.z, x[7] = b.w;
 }
 
 __global__ __launch_bounds__(kBlock) void sssp_dense_kernel(
@@ -662,26 +652,90 @@ __global__ __launch_bounds__(kBlock) void sssp_dense_kernel(
     const uint32_t *__restrict__ counts, const uint32_t *__restrict__ cur, uint32_t *dist,
     uint8_t *__restrict__ flag)
 {
-	GPHI_FOR_LIVE(ca, cb, has_b, lists, list_cap, counts)
+	const uint32_t xcd = blockIdx.x % kNumXcd, per = gridDim.x / kNumXcd;
+	const uint32_t cnt = counts[xcd];
+	const uint32_t *lst = lists + (uint64_t)xcd * list_cap;
+	uint32_t pos = blockIdx.x / kNumXcd;
+	if (pos >= cnt) return;
+	const uint32_t tid = threadIdx.x;
+	const uint4 *D = reinterpret_cast<const uint4 *>(tails);  // dummy: first 16 bytes
+	const uint32_t c0 = lst[pos];
+	bool v1 = pos + per < cnt, v2 = pos + 2 * per < cnt;
+	uint32_t c1 = v1 ? lst[pos + per] : c0;
+	uint32_t c2 = v2 ? lst[pos + 2 * per] : c0;
+	uint32_t h[8], hn[8], t[8], w[8];
 	{
-		for (int q = 0; q < 2; ++q) {
-			if (q == 1 && !has_b) break;
-			const uint64_t c = q ? cb : ca;
-			const uint4 *H = reinterpret_cast<const uint4 *>(heads + c * kChunk);
-			const uint4 *T = reinterpret_cast<const uint4 *>(tails + c * kChunk);
-			const uint4 *Wt = reinterpret_cast<const uint4 *>(wts + c * kChunk);
-			const uint4 h0 = ld_stream(H + threadIdx.x), h1 = ld_stream(H + threadIdx.x + kBlock);
-			const uint4 t0 = ld_stream(T + threadIdx.x), t1 = ld_stream(T + threadIdx.x + kBlock);
-			const uint4 w0 = ld_stream(Wt + threadIdx.x), w1 = ld_stream(Wt + threadIdx.x + kBlock);
-			sssp_relax(h0.x, t0.x, w0.x, cur, dist, flag);
-			sssp_relax(h0.y, t0.y, w0.y, cur, dist, flag);
-			sssp_relax(h0.z, t0.z, w0.z, cur, dist, flag);
-			sssp_relax(h0.w, t0.w, w0.w, cur, dist, flag);
-			sssp_relax(h1.x, t1.x, w1.x, cur, dist, flag);
-			sssp_relax(h1.y, t1.y, w1.y, cur, dist, flag);
-			sssp_relax(h1.z, t1.z, w1.z, cur, dist, flag);
-			sssp_relax(h1.w, t1.w, w1.w, cur, dist, flag);
+		const uint4 *H = reinterpret_cast<const uint4 *>(heads + (uint64_t)c0 * kChunk);
+		unpack8u(ld_stream(H + tid), ld_stream(H + tid + kBlock), h);
+	}
+	uint32_t hit = 0;
+	{
+		uint32_t cw[8];
+#pragma unroll
+		for (int j = 0; j < 8; ++j) cw[j] = cur[h[j] >> 5];
+#pragma unroll
+		for (int j = 0; j < 8; ++j) hit |= ((cw[j] >> (h[j] & 31)) & 1u) << j;
+	}
+	{
+		const uint4 *H = reinterpret_cast<const uint4 *>(heads + (uint64_t)c1 * kChunk);
+		unpack8u(ld_stream(H + tid), ld_stream(H + tid + kBlock), hn);
+		const uint4 *T = reinterpret_cast<const uint4 *>(tails + (uint64_t)c0 * kChunk);
+		const uint4 *Wt = reinterpret_cast<const uint4 *>(wts + (uint64_t)c0 * kChunk);
+		const bool q0 = hit & 0x0Fu, q1 = hit & 0xF0u;
+		unpack8u(ld_stream(q0 ? T + tid : D), ld_stream(q1 ? T + tid + kBlock : D), t);
+		unpack8u(ld_stream(q0 ? Wt + tid : D), ld_stream(q1 ? Wt + tid + kBlock : D), w);
+	}
+	for (;;) {
+		// stage A: distances at both ends of chunk k's frontier edges, frontier bits of chunk k+1
+		uint32_t dh[8], dt[8], cw[8];
+#pragma unroll
+		for (int j = 0; j < 8; ++j) {
+			const bool on = (hit >> j) & 1u;
+			dh[j] = dist[on ? h[j] : 0u];
+			dt[j] = dist[on ? t[j] : 0u];
 		}
+#pragma unroll
+		for (int j = 0; j < 8; ++j) cw[j] = cur[hn[j] >> 5];
+		uint32_t better = 0, hit_n = 0;
+#pragma unroll
+		for (int j = 0; j < 8; ++j) better |= (uint32_t)(dh[j] + w[j] < dt[j]) << j;
+		better &= hit;
+#pragma unroll
+		for (int j = 0; j < 8; ++j) hit_n |= ((cw[j] >> (hn[j] & 31)) & 1u) << j;
+		if (!v1) hit_n = 0;
+		// stage B: chunk k's relaxations, then the requests of the next iteration
+#pragma unroll
+		for (int j = 0; j < 8; ++j)
+			if ((better >> j) & 1u) {
+				atomicMin(dist + t[j], dh[j] + w[j]);
+				flag[t[j]] = 1;
+			}
+		uint32_t tn[8], wn[8], h2[8];
+		{
+			const uint4 *T = reinterpret_cast<const uint4 *>(tails + (uint64_t)c1 * kChunk);
+			const uint4 *Wt = reinterpret_cast<const uint4 *>(wts + (uint64_t)c1 * kChunk);
+			const bool q0 = hit_n & 0x0Fu, q1 = hit_n & 0xF0u;
+			unpack8u(ld_stream(q0 ? T + tid : D), ld_stream(q1 ? T + tid + kBlock : D), tn);
+			unpack8u(ld_stream(q0 ? Wt + tid : D), ld_stream(q1 ? Wt + tid + kBlock : D), wn);
+			const uint4 *H = reinterpret_cast<const uint4 *>(heads + (uint64_t)c2 * kChunk);
+			unpack8u(ld_stream(H + tid), ld_stream(H + tid + kBlock), h2);
+		}
+		const bool v3 = pos + 3 * per < cnt;
+		const uint32_t c3 = v3 ? lst[pos + 3 * per] : c0;
+		if (!v1) break;
+		pos += per;
+#pragma unroll
+		for (int j = 0; j < 8; ++j) {
+			h[j] = hn[j];
+			t[j] = tn[j];
+			w[j] = wn[j];
+			hn[j] = h2[j];
+		}
+		hit = hit_n;
+		c1 = c2;
+		c2 = c3;
+		v1 = v2;
+		v2 = v3;
 	}
 }
 
@@ -962,7 +1016,7 @@ __global__ __launch_bounds__(kBlock) void frontier_emit_kernel(
 // host drivers
 // ---------------------------------------------------------------------------
 constexpr unsigned kDenseGrid = 4096;   // multiple of kNumXcd
-constexpr unsigned kBfsDenseGrid = 2048;  // 8 blocks per CU: every block resident at once
+constexpr unsigned kBfsDenseGrid = 2048;  // BFS/SSSP sweeps: 8 blocks per CU, every block resident at once
 static constexpr unsigned kSparseGrid = 4096;
 
 static float ev_ms(hipEvent_t a, hipEvent_t b)
@@ -1037,7 +1091,8 @@ static int frontier_loop(gphi_graph *gr, uint32_t src, gphi_stats *st)
 				chunk_select_kernel<false><<<sgrid, kBlock, 0, s>>>(G.meta, G.nchunks, gr->row_stamp,
 				                                                  gr->col_unvis, L, gr->live, list_cap,
 				                                                  gr->live_cnt);
-			unsigned grid = (unsigned)std::min<uint64_t>(kBfs ? kBfsDenseGrid : kDenseGrid,
+			if (ev_step) GPHI_HIP(ctx, hipEventRecord(ev[2 * GPHI_MAX_STEPS + 4 + steps], s));
+			unsigned grid = (unsigned)std::min<uint64_t>(kBfsDenseGrid,
 			                                             (G.nchunks + kNumXcd - 1) / kNumXcd * kNumXcd);
 			grid = std::max<unsigned>(grid / kNumXcd * kNumXcd, kNumXcd);
 			if (kBfs)
@@ -1072,7 +1127,13 @@ static int frontier_loop(gphi_graph *gr, uint32_t src, gphi_stats *st)
 			hipEvent_t prev = ev[1];
 			uint32_t ns = std::min<uint32_t>(steps, GPHI_MAX_STEPS - 1);
 			for (uint32_t k = 0; k < ns; ++k) {
-				st->sweep_ms[k] = ev_ms(prev, ev[2 + 2 * k]);
+				if (st->dir[k]) {
+					hipEvent_t sel = ev[2 * GPHI_MAX_STEPS + 4 + k];
+					st->select_ms[k] = ev_ms(prev, sel);
+					st->sweep_ms[k] = ev_ms(sel, ev[2 + 2 * k]);
+				} else {
+					st->sweep_ms[k] = ev_ms(prev, ev[2 + 2 * k]);
+				}
 				st->update_ms[k] = ev_ms(ev[2 + 2 * k], ev[3 + 2 * k]);
 				prev = ev[3 + 2 * k];
 			}

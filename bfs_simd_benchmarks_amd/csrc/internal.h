@@ -84,7 +84,9 @@ struct gphi_ctx {
 	std::string last;
 	// pinned host control block for per-step readback
 	gphi::StepCtrl *h_ctrl = nullptr;
-	hipEvent_t ev[2 * GPHI_MAX_STEPS + 4];
+	// per-step timing: [0] start, [1] init done, [2+2k] sweep k done, [3+2k] update k
+	// done, [2*MAX+4+k] chunk selection of dense step k done
+	hipEvent_t ev[3 * GPHI_MAX_STEPS + 4];
 	bool ev_ok = false;
 };
 
