@@ -82,7 +82,7 @@ def main() -> int:
     ap.add_argument("--serving", action="store_true",
                     help="run the serving mix instead: 8 resident RMAT graphs (scales 18-24), a "
                          "Poisson stream of BFS/SSSP queries, batched vs one-at-a-time")
-    ap.add_argument("--qps", type=float, default=4000.0, help="offered queries/s (serving)")
+    ap.add_argument("--qps", type=float, default=3000.0, help="offered queries/s (serving)")
     ap.add_argument("--queries", type=int, default=100_000, help="queries in the stream (serving)")
     args = ap.parse_args()
     if args.serving:

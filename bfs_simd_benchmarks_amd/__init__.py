@@ -64,7 +64,7 @@ class WeightSpec(C.Structure):
 
 
 class ServerOpts(C.Structure):
-    _fields_ = [("max_batch", C.c_uint32), ("hbm_budget", C.c_uint64)]
+    _fields_ = [("max_batch", C.c_uint32), ("hbm_budget", C.c_uint64), ("sssp_lanes", C.c_uint32)]
 
 
 class QueryInfo(C.Structure):
@@ -137,6 +137,7 @@ def _declare(lib):
         "gphi_graph_download": (i32, [_P, _P, _P, _P, _P, _P, _P]),
         "gphi_graph_tile_offsets": (i32, [_P, _P]),
         "gphi_graph_retile": (i32, [_P, u32, u32, i32]),
+        "gphi_graph_view": (i32, [_P, _P, C.POINTER(_P)]),
         "gphi_graph_save": (i32, [_P, C.c_char_p]),
         "gphi_graph_load": (i32, [_P, C.c_char_p, C.POINTER(_P)]),
         "gphi_host_read_text": (i32, [C.c_char_p, u32, u32, i32, i32, C.POINTER(_P)]),
@@ -234,11 +235,11 @@ class Server:
     """Resident graphs with continuous batching (gphi_server_*).  Each graph
     added must live in its own Context; the server takes both over."""
 
-    def __init__(self, max_batch: int = 64, hbm_budget: int = 0):
+    def __init__(self, max_batch: int = 64, hbm_budget: int = 0, sssp_lanes: int = 0):
         self.lib = load_library()
         h = _P()
-        _host_check(self.lib, self.lib.gphi_server_create(C.byref(ServerOpts(max_batch, hbm_budget)),
-                                                          C.byref(h)))
+        _host_check(self.lib, self.lib.gphi_server_create(
+            C.byref(ServerOpts(max_batch, hbm_budget, sssp_lanes)), C.byref(h)))
         self.h = h
         self.graphs = []
 
@@ -331,6 +332,14 @@ class Context:
         h = _P()
         self.check(self.lib.gphi_graph_load_text(self.h, str(prefix).encode(), W, S, int(weighted),
                                                  C.byref(h)))
+        return Graph(self, h)
+
+    def view(self, graph: "Graph") -> "Graph":
+        """A handle on `graph` (resident on this context's device) with its own
+        traversal state, running on this context's stream (gphi_graph_view).
+        Free it before the graph."""
+        h = _P()
+        self.check(self.lib.gphi_graph_view(self.h, graph.h, C.byref(h)))
         return Graph(self, h)
 
     def load_binary(self, path) -> "Graph":

@@ -197,6 +197,27 @@ int gphi_graph_info(const gphi_graph *g, uint32_t *n, uint64_t *m, uint32_t *W, 
 
 uint64_t gphi_graph_device_bytes(const gphi_graph *g) { return g ? g->g.device_bytes : 0; }
 
+int gphi_graph_view(gphi_ctx *ctx, gphi_graph *base, gphi_graph **out)
+{
+	if (!ctx || !base || !out) return set_error(ctx, GPHI_E_ARG, "gphi_graph_view: NULL argument");
+	*out = nullptr;
+	if (base->base) base = base->base;
+	if (ctx->device != base->ctx->device)
+		return set_error(ctx, GPHI_E_ARG, "a view must live on its graph's device (%d, not %d)",
+		                 base->ctx->device, ctx->device);
+	gphi_graph *v = new (std::nothrow) gphi_graph();
+	if (!v) return set_error(ctx, GPHI_E_OOM, "out of host memory");
+	{
+		std::lock_guard<std::mutex> lk(base->ctx->mu);
+		v->g = base->g;
+	}
+	v->g.device_bytes = 0;
+	v->ctx = ctx;
+	v->base = base;
+	*out = v;
+	return GPHI_OK;
+}
+
 void gphi_graph_free(gphi_graph *g)
 {
 	if (!g) return;
@@ -354,6 +375,7 @@ int gphi_host_from_device(gphi_graph *g, gphi_host_graph **out)
 int gphi_graph_retile(gphi_graph *g, uint32_t W, uint32_t S, int tile_order)
 {
 	GPHI_ENTER(g);
+	if (g->base) return set_error(g->ctx, GPHI_E_STATE, "a view cannot re-tile the graph it shares");
 	return gphi::retile_graph(g, W, S, tile_order);
 }
 

@@ -347,6 +347,13 @@ void free_traversal_state(gphi_graph *gr)
 void free_graph_device(gphi_graph *gr)
 {
 	DevGraph &G = gr->g;
+	if (gr->base) {  // a view: its own state only
+		free_traversal_state(gr);
+		void *in[] = {gr->in_off, gr->in_src, gr->pieces};
+		for (void *p : in)
+			if (p) (void)hipFree(p);
+		return;
+	}
 	void *ps[] = {G.row_off, G.cols, G.csr_w, G.deg, G.heads, G.tails, G.coo_w, G.meta,
 	              gr->level, gr->parent, gr->dist, gr->flag, gr->visited, gr->cur,
 	              gr->row_stamp, gr->col_unvis, gr->queue, gr->qoff, gr->ctrl, gr->rank,

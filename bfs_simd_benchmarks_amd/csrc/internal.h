@@ -93,6 +93,7 @@ struct gphi_ctx {
 struct gphi_graph {
 	gphi_ctx *ctx = nullptr;
 	gphi::DevGraph g;
+	gphi_graph *base = nullptr;  // a view (gphi_graph_view) shares base's graph arrays
 	// per-graph traversal state (allocated lazily, reused across calls)
 	int32_t *level = nullptr;  // levels (BFS) ; dist (SSSP/BF) aliases via uint32
 	int32_t *parent = nullptr;

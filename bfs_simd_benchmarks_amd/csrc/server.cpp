@@ -1,13 +1,16 @@
 // server.cpp -- query serving over resident graphs with continuous batching.
 //
 // No reference counterpart (the reference runs one traversal per process,
-// apps/bfs_simd/bfs.cpp:1176-1199).  Each admitted graph gets a lane: its own
-// gphi_ctx (HIP stream) and worker thread, so the lanes' kernels overlap on
-// the device.  A lane's worker takes every BFS query pending for its graph
+// apps/bfs_simd/bfs.cpp:1176-1199).  Each admitted graph gets a BFS lane: its
+// own gphi_ctx (HIP stream) and worker thread, so the lanes' kernels overlap
+// on the device.  A lane's worker takes every BFS query pending for its graph
 // when it becomes free (up to max_batch) and answers them with one
-// multi-source traversal (gphi_bfs_batch); queries that arrive meanwhile join
-// the next launch.  SSSP queries run one at a time on the lane.  Graphs are
-// admitted while their HBM bytes plus traversal state fit the budget.
+// multi-source traversal (gphi_bfs_batch) when that is expected to beat
+// running them one by one; queries that arrive meanwhile join the next
+// launch.  A weighted graph also gets an SSSP lane -- a view of the graph
+// (gphi_graph_view) on a stream of its own -- that runs SSSP queries one at a
+// time beside the BFS lane.  Graphs are admitted while their HBM bytes plus
+// traversal state fit the budget.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -51,7 +54,24 @@ struct Lane {
 	std::thread worker;
 	uint64_t device_bytes = 0;
 	uint64_t batches = 0, served = 0;
+	// running estimates (microseconds) of one plain BFS and of one batched
+	// launch per mask width (1, 2, 4 words); 0 = not measured yet
+	double t_plain = 0, t_batch[3] = {0, 0, 0};
+	bool busy = false;  // a launch is running (guarded by the server mutex)
 };
+
+// the multi-source traversal has a fixed cost (mask and level set-up, a
+// vertex pass per level) that small batches do not repay: run k pending BFS
+// queries as one launch only when that is expected to beat k plain runs
+bool use_batch(const Lane &L, size_t k, uint32_t max_batch)
+{
+	if (max_batch <= 1 || k <= 1) return false;
+	const int w = k <= 64 ? 0 : k <= 128 ? 1 : 2;
+	if (L.t_plain > 0 && L.t_batch[w] > 0) return L.t_batch[w] < (double)k * L.t_plain;
+	return k >= 8;  // no estimate yet: batch only clearly large groups
+}
+
+void ema(double &x, double v) { x = x > 0 ? 0.8 * x + 0.2 * v : v; }
 
 }  // namespace
 
@@ -60,6 +80,7 @@ struct gphi_server {
 	std::mutex mu;
 	std::condition_variable cv_work, cv_done;
 	std::vector<std::unique_ptr<Lane>> lanes;
+	std::vector<std::pair<Lane *, std::vector<Lane *>>> graphs;  // per graph id: BFS lane, SSSP lanes
 	std::unordered_map<uint64_t, std::shared_ptr<Query>> queries;
 	uint64_t next_ticket = 1;
 	uint64_t hbm_used = 0;
@@ -104,13 +125,22 @@ void lane_loop(gphi_server *S, Lane *L)
 					}
 				}
 			}
+			L->busy = true;
 		}
 		const double t0 = now_us();
 		int rc = GPHI_OK;
+		std::vector<double> done_at;  // per-query completion times of a run of plain BFS
 		if (take[0]->kind == GPHI_QUERY_SSSP) {
 			rc = gphi_sssp(L->g, take[0]->src, (uint32_t *)take[0]->out, nullptr);
-		} else if (take.size() == 1 && S->opts.max_batch == 1) {
-			rc = gphi_bfs(L->g, take[0]->src, (int32_t *)take[0]->out, nullptr, nullptr);
+		} else if (!use_batch(*L, take.size(), S->opts.max_batch)) {
+			for (auto &q : take) {
+				const double a = now_us();
+				const int r = gphi_bfs(L->g, q->src, (int32_t *)q->out, nullptr, nullptr);
+				const double b = now_us();
+				if (!q->out) ema(L->t_plain, b - a);
+				done_at.push_back(b);
+				if (r != GPHI_OK && rc == GPHI_OK) rc = r;
+			}
 		} else {
 			srcs.resize(take.size());
 			bool want = false;
@@ -119,7 +149,9 @@ void lane_loop(gphi_server *S, Lane *L)
 				want |= take[i]->out != nullptr;
 			}
 			if (want) lv8.resize((size_t)take.size() * L->n);
+			const double a = now_us();
 			rc = gphi_bfs_batch(L->g, srcs.data(), (uint32_t)take.size(), want ? lv8.data() : nullptr, nullptr);
+			if (!want) ema(L->t_batch[take.size() <= 64 ? 0 : take.size() <= 128 ? 1 : 2], now_us() - a);
 			if (rc == GPHI_OK && want) {
 				for (size_t i = 0; i < take.size(); ++i) {
 					if (!take[i]->out) continue;
@@ -132,15 +164,18 @@ void lane_loop(gphi_server *S, Lane *L)
 		const double t1 = now_us();
 		{
 			std::lock_guard<std::mutex> lk(S->mu);
-			for (auto &q : take) {
-				q->t_start = t0;
-				q->t_done = t1;
-				q->batch = (uint32_t)take.size();
-				q->status = rc;
-				q->done = true;
+			const bool batched = done_at.empty();
+			for (size_t i = 0; i < take.size(); ++i) {
+				Query &q = *take[i];
+				q.t_start = batched || i == 0 ? t0 : done_at[i - 1];
+				q.t_done = batched ? t1 : done_at[i];
+				q.batch = batched ? (uint32_t)take.size() : 1;
+				q.status = rc;
+				q.done = true;
 			}
-			L->batches += 1;
+			L->batches += done_at.empty() ? 1 : done_at.size();
 			L->served += take.size();
+			L->busy = false;
 		}
 		S->cv_done.notify_all();
 	}
@@ -165,6 +200,11 @@ int gphi_server_create(const gphi_server_opts *opts, gphi_server **out)
 		delete S;
 		return set_error(nullptr, GPHI_E_ARG, "max_batch must be <= 256");
 	}
+	if (S->opts.sssp_lanes == 0) S->opts.sssp_lanes = 2;
+	if (S->opts.sssp_lanes > 8) {
+		delete S;
+		return set_error(nullptr, GPHI_E_ARG, "sssp_lanes must be <= 8");
+	}
 	if (S->opts.hbm_budget == 0) {
 		size_t fr = 0, tot = 0;
 		if (hipMemGetInfo(&fr, &tot) == hipSuccess) S->opts.hbm_budget = (uint64_t)(tot * 0.9);
@@ -176,23 +216,58 @@ int gphi_server_create(const gphi_server_opts *opts, gphi_server **out)
 int gphi_server_add_graph(gphi_server *S, gphi_graph *g, uint32_t *graph_id)
 {
 	if (!S || !g || !graph_id) return set_error(nullptr, GPHI_E_ARG, "NULL argument");
-	const uint64_t need = g->g.device_bytes + gphi::lane_state_bytes(g->g, S->opts.max_batch);
+	if (g->base) return set_error(nullptr, GPHI_E_ARG, "add the graph itself, not a view of it");
+	const bool weighted = g->g.weighted;
+	const uint32_t nv = weighted ? S->opts.sssp_lanes : 0;
+	const uint64_t need = g->g.device_bytes + gphi::lane_state_bytes(g->g, S->opts.max_batch) +
+	                      nv * gphi::lane_state_bytes(g->g, 1);
+	{
+		std::lock_guard<std::mutex> lk(S->mu);
+		if (S->hbm_used + need > S->opts.hbm_budget)
+			return set_error(nullptr, GPHI_E_OOM,
+			                 "graph needs %llu MB of HBM; %llu MB of the %llu MB budget are in use",
+			                 (unsigned long long)(need >> 20), (unsigned long long)(S->hbm_used >> 20),
+			                 (unsigned long long)(S->opts.hbm_budget >> 20));
+	}
+	std::vector<gphi_graph *> views;
+	std::vector<gphi_ctx *> vctxs;
+	for (uint32_t i = 0; i < nv; ++i) {
+		gphi_opts o{g->ctx->device, g->ctx->t_ratio};
+		gphi_ctx *vc = nullptr;
+		gphi_graph *v = nullptr;
+		int rc = gphi_open(&o, &vc);
+		if (rc == GPHI_OK) rc = gphi_graph_view(vc, g, &v);
+		if (rc != GPHI_OK) {
+			if (vc) gphi_close(vc);
+			for (gphi_graph *x : views) gphi_graph_free(x);
+			for (gphi_ctx *x : vctxs) gphi_close(x);
+			return rc;
+		}
+		views.push_back(v);
+		vctxs.push_back(vc);
+	}
 	std::lock_guard<std::mutex> lk(S->mu);
-	if (S->hbm_used + need > S->opts.hbm_budget)
-		return set_error(nullptr, GPHI_E_OOM,
-		                 "graph needs %llu MB of HBM; %llu MB of the %llu MB budget are in use",
-		                 (unsigned long long)(need >> 20), (unsigned long long)(S->hbm_used >> 20),
-		                 (unsigned long long)(S->opts.hbm_budget >> 20));
 	auto L = std::make_unique<Lane>();
 	L->g = g;
 	L->ctx = g->ctx;
 	L->n = g->g.n;
 	L->device_bytes = need;
 	S->hbm_used += need;
-	Lane *lp = L.get();
-	*graph_id = (uint32_t)S->lanes.size();
+	Lane *bl = L.get();
+	std::vector<Lane *> sl;
 	S->lanes.push_back(std::move(L));
-	lp->worker = std::thread(gphi::lane_loop, S, lp);
+	for (uint32_t i = 0; i < nv; ++i) {
+		auto V = std::make_unique<Lane>();
+		V->g = views[i];
+		V->ctx = vctxs[i];
+		V->n = g->g.n;
+		sl.push_back(V.get());
+		S->lanes.push_back(std::move(V));
+	}
+	*graph_id = (uint32_t)S->graphs.size();
+	S->graphs.emplace_back(bl, sl);
+	bl->worker = std::thread(gphi::lane_loop, S, bl);
+	for (Lane *v : sl) v->worker = std::thread(gphi::lane_loop, S, v);
 	return GPHI_OK;
 }
 
@@ -203,13 +278,19 @@ int gphi_server_submit(gphi_server *S, uint32_t graph_id, uint32_t kind, uint32_
 	auto q = std::make_shared<Query>();
 	{
 		std::lock_guard<std::mutex> lk(S->mu);
-		if (graph_id >= S->lanes.size()) return set_error(nullptr, GPHI_E_RANGE, "no graph %u", graph_id);
-		Lane *L = S->lanes[graph_id].get();
-		if (src >= L->n) return set_error(nullptr, GPHI_E_RANGE, "source %u >= N %u", src, L->n);
+		if (graph_id >= S->graphs.size()) return set_error(nullptr, GPHI_E_RANGE, "no graph %u", graph_id);
 		if (kind != GPHI_QUERY_BFS && kind != GPHI_QUERY_SSSP)
 			return set_error(nullptr, GPHI_E_ARG, "unknown query kind %u", kind);
-		if (kind == GPHI_QUERY_SSSP && !L->g->g.weighted)
-			return set_error(nullptr, GPHI_E_STATE, "graph %u carries no weights", graph_id);
+		Lane *L = S->graphs[graph_id].first;
+		if (kind == GPHI_QUERY_SSSP) {
+			const auto &sl = S->graphs[graph_id].second;
+			if (sl.empty()) return set_error(nullptr, GPHI_E_STATE, "graph %u carries no weights", graph_id);
+			L = sl[0];  // the SSSP lane with the shortest queue
+			auto load = [](const Lane *x) { return x->pending.size() + (x->busy ? 1 : 0); };
+			for (Lane *v : sl)
+				if (load(v) < load(L)) L = v;
+		}
+		if (src >= L->n) return set_error(nullptr, GPHI_E_RANGE, "source %u >= N %u", src, L->n);
 		if (S->stopping) return set_error(nullptr, GPHI_E_STATE, "server is stopping");
 		q->ticket = S->next_ticket++;
 		q->kind = kind;
@@ -247,10 +328,15 @@ int gphi_server_stats(gphi_server *S, uint32_t graph_id, uint64_t *batches, uint
 {
 	if (!S) return set_error(nullptr, GPHI_E_ARG, "server is NULL");
 	std::lock_guard<std::mutex> lk(S->mu);
-	if (graph_id >= S->lanes.size()) return set_error(nullptr, GPHI_E_RANGE, "no graph %u", graph_id);
-	const Lane *L = S->lanes[graph_id].get();
-	if (batches) *batches = L->batches;
-	if (served) *served = L->served;
+	if (graph_id >= S->graphs.size()) return set_error(nullptr, GPHI_E_RANGE, "no graph %u", graph_id);
+	const Lane *L = S->graphs[graph_id].first;
+	uint64_t b = L->batches, sv = L->served;
+	for (const Lane *v : S->graphs[graph_id].second) {
+		b += v->batches;
+		sv += v->served;
+	}
+	if (batches) *batches = b;
+	if (served) *served = sv;
 	if (hbm_bytes) *hbm_bytes = L->device_bytes;
 	return GPHI_OK;
 }
@@ -265,11 +351,16 @@ void gphi_server_destroy(gphi_server *S)
 	S->cv_work.notify_all();
 	for (auto &L : S->lanes)
 		if (L->worker.joinable()) L->worker.join();
+	// views first (they share their graph's arrays), then the graphs; every
+	// context once
+	std::vector<gphi_graph *> views, bases;
 	std::vector<gphi_ctx *> ctxs;
 	for (auto &L : S->lanes) {
-		gphi_graph_free(L->g);
+		(L->g->base ? views : bases).push_back(L->g);
 		if (std::find(ctxs.begin(), ctxs.end(), L->ctx) == ctxs.end()) ctxs.push_back(L->ctx);
 	}
+	for (gphi_graph *v : views) gphi_graph_free(v);
+	for (gphi_graph *b : bases) gphi_graph_free(b);
 	for (gphi_ctx *c : ctxs) gphi_close(c);
 	delete S;
 }

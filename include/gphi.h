@@ -73,6 +73,7 @@ typedef struct gphi_stats {
 	uint32_t fsize[GPHI_MAX_STEPS];      /* |F| expanded by the step */
 	uint32_t fdeg[GPHI_MAX_STEPS];       /* sum of out-degrees of F */
 	float sweep_ms[GPHI_MAX_STEPS];      /* sweep kernel time (HIP events) */
+	float select_ms[GPHI_MAX_STEPS];     /* chunk selection before a dense sweep (0 if sparse) */
 	float update_ms[GPHI_MAX_STEPS];     /* frontier update kernel time */
 	uint32_t chunks[GPHI_MAX_STEPS];     /* 2048-edge chunks a dense step swept (0 if sparse) */
 	float init_ms;                       /* state initialisation (device) */
@@ -135,18 +136,22 @@ int gphi_bfs_batch(gphi_graph *g, const uint32_t *srcs, uint32_t k, uint8_t *lev
                    gphi_stats *st);
 
 /* ---- serving: resident graphs, continuous batching (no reference counterpart) ----
- * Each added graph becomes a lane with the graph's own context (HIP stream)
- * and a worker thread; lanes run concurrently on the device.  A lane answers
- * all BFS queries pending for its graph when it becomes free (up to
- * max_batch) with one gphi_bfs_batch launch; queries arriving meanwhile join
- * the next launch.  SSSP queries run one at a time.  A graph is admitted only
- * while its HBM bytes plus traversal state fit hbm_budget (GPHI_E_OOM
- * otherwise).  The server takes ownership of each added graph and of its
- * context (create one context per graph). */
+ * Each added graph gets a BFS lane with the graph's own context (HIP stream)
+ * and a worker thread; lanes run concurrently on the device.  A BFS lane
+ * takes all BFS queries pending for its graph when it becomes free (up to
+ * max_batch) and answers them with one gphi_bfs_batch launch when that is
+ * expected to beat running them one by one (running estimates of both);
+ * queries arriving meanwhile join the next launch.  A weighted graph also
+ * gets sssp_lanes SSSP lanes, each a gphi_graph_view on a context of its own,
+ * running SSSP queries one at a time beside the BFS lane.  A graph is
+ * admitted only while its HBM bytes plus every lane's traversal state fit
+ * hbm_budget (GPHI_E_OOM otherwise).  The server takes ownership of each
+ * added graph and of its context (create one context per graph). */
 typedef struct gphi_server gphi_server;
 typedef struct gphi_server_opts {
 	uint32_t max_batch;   /* BFS queries per launch, 1..256 (0 -> 64); 1 = no batching */
 	uint64_t hbm_budget;  /* bytes; 0 -> 90% of device memory */
+	uint32_t sssp_lanes;  /* SSSP lanes per weighted graph, 1..8 (0 -> 2) */
 } gphi_server_opts;
 #define GPHI_QUERY_BFS 0
 #define GPHI_QUERY_SSSP 1
@@ -219,6 +224,14 @@ int gphi_host_from_device(gphi_graph *g, gphi_host_graph **out);
 int gphi_host_write_text(const gphi_host_graph *h, const char *prefix);
 int gphi_host_desc(const gphi_host_graph *h, gphi_graph_desc *out);
 void gphi_host_free(gphi_host_graph *h);
+
+/* A second handle on a resident graph: it shares the graph's device arrays
+ * but has its own traversal state and runs on ctx's stream, so traversals of
+ * the two handles can run at the same time (no reference counterpart; the
+ * serving loop uses it to run SSSP beside batched BFS).  ctx must be on the
+ * graph's device.  Free every view before the graph; re-tiling the graph
+ * invalidates its views. */
+int gphi_graph_view(gphi_ctx *ctx, gphi_graph *base, gphi_graph **out);
 
 /* Rebuild the tile arrays from the graph's CSR with another W, S or order
  * (tools/coo_tiling + tools/column_major_tile again, on the device); used by

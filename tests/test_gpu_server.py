@@ -81,3 +81,40 @@ def test_admission_by_hbm_budget(c1):
         gg.free()
         gg.ctx.close()
         srv.close()
+
+
+def test_view_runs_beside_its_graph(c3):
+    """Two handles on one resident graph (gphi_graph_view), each on its own
+    stream, traverse at the same time and both match the oracle."""
+    import threading
+    base_ctx, other = G.Context(0), G.Context(0)
+    g = base_ctx.graph_from_arrays(c3.n, c3.deg, W=c3.W, S=c3.S, csr_edges=c3.cols, csr_weights=c3.cw,
+                                   tile_heads=c3.sh, tile_tails=c3.st, tile_weights=c3.sw,
+                                   tile_offsets=c3.soff)
+    v = other.view(g)
+    try:
+        srcs = [int(s) for s in pick_sources(c3.deg, 6, 21)]
+        got = {}
+
+        def run_bfs():
+            got["bfs"] = [g.bfs(s)[0] for s in srcs]
+
+        def run_sssp():
+            got["sssp"] = [v.sssp(s)[0] for s in srcs]
+        ths = [threading.Thread(target=run_bfs), threading.Thread(target=run_sssp)]
+        print("case: view sssp beside base bfs", flush=True)
+        for th in ths:
+            th.start()
+        for th in ths:
+            th.join()
+        for s, lv, d in zip(srcs, got["bfs"], got["sssp"]):
+            assert np.array_equal(lv, O.bfs_levels(c3.n, c3.deg, c3.cols, s))
+            assert np.array_equal(d, O.sssp_dijkstra(c3.n, c3.deg, c3.cols, c3.cw, s))
+        with pytest.raises(G.GphiError) as ei:
+            v.retile(256, 4)
+        assert ei.value.code == -7
+    finally:
+        v.free()
+        g.free()
+        other.close()
+        base_ctx.close()
