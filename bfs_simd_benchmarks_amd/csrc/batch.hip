@@ -306,6 +306,7 @@ constexpr uint32_t kLiveStride = 16;  // u64 words per slot (one 128-byte line)
 
 template <int NW>
 __global__ __launch_bounds__(kBlock) void batch_update_kernel(uint32_t n, uint32_t n_pitch, uint32_t next_level,
+                                                              const uint32_t *__restrict__ cur,
                                                               u64 *__restrict__ front, u64 *__restrict__ seen,
                                                               u64 *__restrict__ next, uint8_t *__restrict__ level,
                                                               uint8_t *__restrict__ flag, u64 *slots)
@@ -313,24 +314,34 @@ __global__ __launch_bounds__(kBlock) void batch_update_kernel(uint32_t n, uint32
 	__shared__ u64 s_or[kBlock / 64][NW];
 	const uint32_t v = blockIdx.x * kBlock + threadIdx.x;
 	const bool in = v < n;
+	// next first: a vertex nothing reached this step only has to drop its
+	// front mask, and only if it was in the (union) frontier, whose bitmap is
+	// cur -- so most vertices cost one read of next
+	u64 nx[NW], any_nx = 0;
+#pragma unroll
+	for (int i = 0; i < NW; ++i) {
+		nx[i] = in ? next[(uint64_t)v * NW + i] : 0ull;
+		any_nx |= nx[i];
+	}
+	const bool was_front = in && test_bit32(cur, v);
 	u64 nwm[NW];
 	bool any = false;
 #pragma unroll
 	for (int i = 0; i < NW; ++i) {
 		nwm[i] = 0;
-		if (in) {
-			const uint64_t idx = (uint64_t)v * NW + i;
-			const u64 nx = next[idx];
+		const uint64_t idx = (uint64_t)v * NW + i;
+		if (any_nx) {
 			const u64 sn = seen[idx];
-			const u64 nw = nx & ~sn;
-			const u64 fr = front[idx];
-			if (nw | fr) front[idx] = nw;
-			if (nx) next[idx] = 0;
+			const u64 nw = nx[i] & ~sn;
+			if (nw || was_front) front[idx] = nw;
+			if (nx[i]) next[idx] = 0;
 			if (nw) {
 				seen[idx] = sn | nw;
 				any = true;
 			}
 			nwm[i] = nw;
+		} else if (was_front) {
+			front[idx] = 0;
 		}
 	}
 	const uint32_t wid = threadIdx.x >> 6;
@@ -484,8 +495,8 @@ int batch_loop(gphi_graph *gr, const uint32_t *d_srcs, uint32_t k, gphi_stats *s
 		}
 		GPHI_HIP(ctx, hipGetLastError());
 		if (ev_step) GPHI_HIP(ctx, hipEventRecord(ev[2 + 2 * steps], s));
-		batch_update_kernel<NW><<<grid_n(G.n), kBlock, 0, s>>>(G.n, G.n_pad, L + 1, gr->b_front, gr->b_seen,
-		                                                     gr->b_next, gr->b_level, gr->flag, slots);
+		batch_update_kernel<NW><<<grid_n(G.n), kBlock, 0, s>>>(G.n, G.n_pad, L + 1, gr->cur, gr->b_front,
+		                                                     gr->b_seen, gr->b_next, gr->b_level, gr->flag, slots);
 		batch_live_kernel<NW><<<1, 64, 0, s>>>(live, slots);
 		launch_frontier_update_flags(gr, L + 1, threshold);
 		GPHI_HIP(ctx, hipGetLastError());
