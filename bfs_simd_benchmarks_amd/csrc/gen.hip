@@ -130,9 +130,30 @@ __global__ void iota_kernel(uint64_t m, uint32_t *p)
 	GRID_STRIDE(i, m) p[i] = (uint32_t)i;
 }
 
-__global__ void degree_kernel(uint64_t m, const uint32_t *h, uint32_t *deg)
+// out-degrees of an edge list sorted by head: deg[v] = (first position of
+// head v+1) - (first position of head v), two binary searches per vertex (an
+// atomic count per edge serializes on the hubs: 0.3 s at scale 25)
+__device__ __forceinline__ uint64_t first_at_least(const uint32_t *__restrict__ h, uint64_t m, uint64_t v)
 {
-	GRID_STRIDE(i, m) atomicAdd(deg + h[i], 1u);
+	uint64_t a = 0, b = m;
+	while (a < b) {
+		const uint64_t mid = (a + b) >> 1;
+		if (h[mid] < v)
+			a = mid + 1;
+		else
+			b = mid;
+	}
+	return a;
+}
+
+__global__ void degree_kernel(uint32_t n, uint64_t m, const uint32_t *__restrict__ h, uint32_t *__restrict__ deg)
+{
+	GRID_STRIDE(v, n) deg[v] = (uint32_t)(first_at_least(h, m, v + 1) - first_at_least(h, m, v));
+}
+
+__global__ void tile_first_kernel(uint64_t nt, uint64_t m, const uint32_t *__restrict__ key, uint32_t *__restrict__ off)
+{
+	GRID_STRIDE(p, nt) off[p] = (uint32_t)first_at_least(key, m, p);
 }
 
 // key(v) = (unsigned)((float)deg + float(rand())/RAND_MAX); RAND_MAX converts to
@@ -428,8 +449,7 @@ int generate_graph(gphi_ctx *ctx, const gphi_rmat_spec *sp, uint32_t W, uint32_t
 	uint32_t *deg = sc.alloc<uint32_t>(n);
 	OOM_IF(deg);
 	if (reorder == GPHI_REORDER_DEGREE) {
-		GPHI_HIP(ctx, hipMemsetAsync(deg, 0, (size_t)n * 4, s));
-		degree_kernel<<<grid1(m), kB, 0, s>>>(m, h0, deg);
+		degree_kernel<<<grid1(n), kB, 0, s>>>(n, m, h0, deg);  // (h0, t0) is sorted by head here
 		std::vector<int32_t> rnd(n);
 		glibc_rand_seq(1, n, rnd.data());
 		int32_t *drnd = sc.alloc<int32_t>(n);
@@ -471,8 +491,7 @@ int generate_graph(gphi_ctx *ctx, const gphi_rmat_spec *sp, uint32_t W, uint32_t
 	gather_kernel<<<grid1(m), kB, 0, s>>>(m, perm, t0, t1);
 	if (w0) gather_kernel<<<grid1(m), kB, 0, s>>>(m, perm, w0, w1);
 	// (h1, t1, w1) in CSR order
-	GPHI_HIP(ctx, hipMemsetAsync(deg, 0, (size_t)n * 4, s));
-	degree_kernel<<<grid1(m), kB, 0, s>>>(m, h1, deg);
+	degree_kernel<<<grid1(n), kB, 0, s>>>(n, m, h1, deg);
 
 	// ---- 5. tile order (stable by tile position, from CSR order) --------------
 	const uint32_t side = n % W ? n / W + 1 : n / W;
@@ -723,17 +742,13 @@ int tile_offsets(gphi_graph *gr, uint32_t *out)
 	if (G.ntiles > (1ull << 32)) return set_error(ctx, GPHI_E_RANGE, "too many tiles");
 	Scratch sc{ctx, {}};
 	const uint64_t nt = G.ntiles, m = G.m;
-	uint32_t *key = sc.alloc<uint32_t>(m), *cnt = sc.alloc<uint32_t>(nt + 1), *off = sc.alloc<uint32_t>(nt + 1);
-	OOM_IF(key && cnt && off);
+	uint32_t *key = sc.alloc<uint32_t>(m), *off = sc.alloc<uint32_t>(nt);
+	OOM_IF(key && off);
 	tile_key_kernel<<<grid1(m), kB, 0, s>>>(m, G.heads, G.tails, G.W, G.side, G.S,
 	                                       G.tile_order == GPHI_TILES_COL_STRIPE, key);
-	GPHI_HIP(ctx, hipMemsetAsync(cnt, 0, (nt + 1) * 4, s));
-	degree_kernel<<<grid1(m), kB, 0, s>>>(m, key, cnt);
-	size_t tb = 0;
-	GPHI_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(nullptr, tb, cnt, off, nt, s));
-	void *tmp = sc.alloc<uint8_t>(tb);
-	OOM_IF(tmp);
-	GPHI_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(tmp, tb, cnt, off, nt, s));
+	// the keys are the tiles' positions in the stored order, so they do not
+	// decrease along the edges: a tile's offset is the first edge at or past it
+	tile_first_kernel<<<grid1(nt), kB, 0, s>>>(nt, m, key, off);
 	GPHI_HIP(ctx, hipMemcpyAsync(out, off, nt * 4, hipMemcpyDeviceToHost, s));
 	GPHI_HIP(ctx, hipStreamSynchronize(s));
 	return GPHI_OK;
