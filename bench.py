@@ -76,6 +76,8 @@ def main() -> int:
                     help="budget of the CPU baseline (0 disables it)")
     ap.add_argument("--batches", default="64,256",
                     help="batched multi-source BFS sizes to measure ('' disables)")
+    ap.add_argument("--other-paths", type=int, default=1,
+                    help="also time PageRank, hop relaxation and weighted SSSP on the same graph (0 skips)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB five times (calibrates the "
                          "FETCH_SIZE / WRITE_SIZE counters against known bytes)")
@@ -259,10 +261,61 @@ def main() -> int:
         "batched": batch_out,
         "graph_build_s": round(gen_s, 2),
     }
+    if args.other_paths:
+        out["other_paths"] = other_paths(G, ctx, g, wl, deg)
     print(json.dumps(out))
     g.free()
     ctx.close()
     return 0
+
+
+def other_paths(G, ctx, g, wl, deg) -> dict:
+    """The same tiled sweep under the reference's other apps, on the same
+    graph: PageRank (20 FP32 iterations, apps/pageRank_simd) and hop
+    relaxation (apps/bellman-ford_simd) on the row-major tiles those apps
+    read, and weighted hybrid SSSP (apps/sssp_simd) on the graph with uint32
+    weights in [1,255].  Bytes per unit follow the byte model: PageRank
+    8M + 4T + 16N per iteration (T = non-empty tiles), a hop sweep 8M + 8N."""
+    from bfs_simd_benchmarks_amd.workloads import pick_sources
+    res = {}
+    W, S = g.W, g.S
+    g.retile(W, S, G.TILES_ROW_MAJOR)
+    offs = g.tile_offsets().astype(np.int64)
+    sizes = np.diff(np.append(offs, g.m))
+    t_nonempty = int(np.count_nonzero(sizes))
+    del offs, sizes
+    g.pagerank(1, 0.85, 1.0 / g.n)                       # warm-up
+    _, pr_ms = g.pagerank(20, 0.85, 1.0 / g.n)
+    it_ms = pr_ms / 20
+    pr_bytes = 8 * g.m + 4 * t_nonempty + 16 * g.n
+    res["pagerank"] = {"ms_per_iteration": round(it_ms, 4), "iterations": 20,
+                       "alg_GBps": round(pr_bytes / (it_ms * 1e-3) / 1e9, 1),
+                       "frac_of_hbm_peak": round(pr_bytes / (it_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                       "nonempty_tiles": t_nonempty}
+    _, sweeps, bf_ms = g.bellman_ford(0, 99)
+    bf_bytes = 8 * g.m + 8 * g.n
+    res["bellman_ford"] = {"sweeps": int(sweeps), "ms": round(bf_ms, 3),
+                           "alg_GBps": round(bf_bytes * sweeps / (bf_ms * 1e-3) / 1e9, 1)}
+    g.retile(W, S, G.TILES_COL_STRIPE)
+    print(f"[bench] pagerank {it_ms:.3f} ms/iteration, bellman-ford {sweeps} sweeps in {bf_ms:.2f} ms",
+          file=sys.stderr, flush=True)
+    import dataclasses
+    gw = dataclasses.replace(wl, weight_seed=wl.seed + 100).generate(ctx, weighted=True)
+    try:
+        srcs = pick_sources(deg, 5, wl.seed + 77)
+        gw.sssp(int(srcs[0]))
+        ms, edges = 0.0, 0
+        for s_ in srcs[1:]:
+            _, st = gw.sssp(int(s_))
+            ms += st.total_ms
+            # the vertices SSSP reaches are the ones BFS reaches from the same source
+            edges += g.bfs(int(s_), parents=False, levels=False)[2].edges_traversed
+        res["sssp"] = {"ms_per_query": round(ms / (len(srcs) - 1), 3),
+                       "gteps": round(edges / (ms * 1e-3) / 1e9, 1), "sources": len(srcs) - 1}
+        print(f"[bench] sssp {res['sssp']['ms_per_query']} ms per query", file=sys.stderr, flush=True)
+    finally:
+        gw.free()
+    return res
 
 
 def serving(args) -> int:

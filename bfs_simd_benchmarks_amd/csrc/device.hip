@@ -630,15 +630,6 @@ __global__ __launch_bounds__(kBlock) void bfs_dense_kernel(
 }
 static_assert(kChunk == 2 * 4 * kBlock, "dense sweep loads two uint4 per array per thread");
 
-// XCD-aware walk over all chunks (PageRank and hop relaxation sweep every edge):
-// blocks of one XCD (blockIdx % 8) sweep one contiguous eighth of the chunks.
-#define GPHI_FOR_CHUNKS(c, nchunks)                                              \
-	const uint32_t xcd_ = blockIdx.x % kNumXcd, slot_ = blockIdx.x / kNumXcd;   \
-	const uint32_t per_ = gridDim.x / kNumXcd;                                 \
-	const uint64_t cpx_ = ((nchunks) + kNumXcd - 1) / kNumXcd;                 \
-	const uint64_t cend_ = min((uint64_t)(nchunks), (xcd_ + 1) * cpx_);        \
-	for (uint64_t c = xcd_ * cpx_ + slot_; c < cend_; c += per_)
-
 // SSSP relax (apps/sssp_simd/sssp.cpp:344-445): if mask[h] and dist[h]+w <
 // dist[t] then lower dist[t] and mark t for the next frontier.  The same
 // software pipeline as bfs_dense_kernel: while chunk k is relaxed, the
@@ -1022,7 +1013,6 @@ __global__ __launch_bounds__(kBlock) void frontier_emit_kernel(
 // ---------------------------------------------------------------------------
 // host drivers
 // ---------------------------------------------------------------------------
-constexpr unsigned kDenseGrid = 4096;   // multiple of kNumXcd
 constexpr unsigned kBfsDenseGrid = 2048;  // BFS/SSSP sweeps: 8 blocks per CU, every block resident at once
 static constexpr unsigned kSparseGrid = 4096;
 
@@ -1176,24 +1166,6 @@ void launch_frontier_update_flags(gphi_graph *gr, uint32_t next_level, uint32_t 
 }
 
 // per-XCD live-chunk lists of the rows stamped `level` (no column test)
-int launch_chunk_select_rows(gphi_graph *gr, uint32_t level, uint32_t *list_cap)
-{
-	DevGraph &G = gr->g;
-	hipStream_t s = gr->ctx->stream;
-	*list_cap = live_list_cap(G.nchunks);
-	GPHI_HIP(gr->ctx, hipMemsetAsync(gr->live_cnt, 0, kNumXcd * 4, s));
-	const unsigned sgrid = (unsigned)std::max<uint64_t>(1, (G.nchunks + kBlock - 1) / kBlock);
-	chunk_select_kernel<false><<<sgrid, kBlock, 0, s>>>(G.meta, G.nchunks, gr->row_stamp, gr->col_unvis,
-	                                                  level, gr->live, *list_cap, gr->live_cnt);
-	return GPHI_OK;
-}
-
-unsigned dense_grid(const DevGraph &G)
-{
-	unsigned grid = (unsigned)std::min<uint64_t>(kDenseGrid, (G.nchunks + kNumXcd - 1) / kNumXcd * kNumXcd);
-	return std::max<unsigned>(grid / kNumXcd * kNumXcd, kNumXcd);
-}
-
 int run_bfs(gphi_graph *gr, uint32_t src, int32_t *levels, int32_t *parents, gphi_stats *st)
 {
 	gphi_ctx *ctx = gr->ctx;
@@ -1249,32 +1221,51 @@ __global__ void pr_prepare_kernel(uint32_t n, const float *__restrict__ rank,
 	}
 }
 
-__device__ __forceinline__ void pr_edge(uint32_t h, uint32_t t, uint32_t n,
-                                        const float *__restrict__ contrib, float *sum)
+// PageRank sum (apps/pageRank_simd/page_rank.cpp:128-199 gathers
+// rank[h]/deg[h] into sum[t] over the tiles) as a pull over the in-edges:
+// sum[t] = sum of contrib[h] over t's in-neighbours, with no atomics except
+// one per piece of a high in-degree vertex.  Vertices with up to kPullLight
+// in-edges get kPullGroup lanes each (every kPullGroup-th in-edge, then a
+// sum across the group); the others a wavefront per piece of up to
+// kPullPiece in-edges, whose partial sums are added with atomicAdd after the
+// light kernel has zeroed them.
+constexpr uint32_t kPullGroup = 8;
+
+__global__ __launch_bounds__(kBlock) void pr_pull_light_kernel(uint32_t n, const uint32_t *__restrict__ in_off,
+                                                               const uint32_t *__restrict__ in_src,
+                                                               const float *__restrict__ contrib,
+                                                               float *__restrict__ sum)
 {
-	if (h < n) atomicAdd(sum + t, contrib[h]);
+	const uint64_t t = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+	const uint32_t v = (uint32_t)(t / kPullGroup), g = (uint32_t)(t % kPullGroup);
+	uint32_t e0 = 0, cnt = 0;
+	if (v < n) {
+		e0 = in_off[v];
+		cnt = in_off[v + 1] - e0;
+	}
+	const bool heavy = cnt > kPullLight;
+	float acc = 0.f;
+	if (!heavy)
+		for (uint32_t j = g; j < cnt; j += kPullGroup) acc += contrib[in_src[e0 + j]];
+#pragma unroll
+	for (int o = kPullGroup / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, kPullGroup);
+	if (g == 0 && v < n) sum[v] = heavy ? 0.f : acc;
 }
 
-__global__ __launch_bounds__(kBlock) void pr_sweep_kernel(const uint32_t *__restrict__ heads,
-                                                          const uint32_t *__restrict__ tails,
-                                                          uint64_t nchunks, uint32_t n,
-                                                          const float *__restrict__ contrib,
-                                                          float *sum)
+__global__ __launch_bounds__(kBlock) void pr_pull_heavy_kernel(uint32_t npieces, const uint2 *__restrict__ pieces,
+                                                               const uint32_t *__restrict__ in_off,
+                                                               const uint32_t *__restrict__ in_src,
+                                                               const float *__restrict__ contrib, float *sum)
 {
-	GPHI_FOR_CHUNKS(c, nchunks)
-	{
-		const uint4 *H = reinterpret_cast<const uint4 *>(heads + c * kChunk);
-		const uint4 *T = reinterpret_cast<const uint4 *>(tails + c * kChunk);
-		uint4 h0 = H[threadIdx.x], h1 = H[threadIdx.x + kBlock];
-		uint4 t0 = T[threadIdx.x], t1 = T[threadIdx.x + kBlock];
-		pr_edge(h0.x, t0.x, n, contrib, sum);
-		pr_edge(h0.y, t0.y, n, contrib, sum);
-		pr_edge(h0.z, t0.z, n, contrib, sum);
-		pr_edge(h0.w, t0.w, n, contrib, sum);
-		pr_edge(h1.x, t1.x, n, contrib, sum);
-		pr_edge(h1.y, t1.y, n, contrib, sum);
-		pr_edge(h1.z, t1.z, n, contrib, sum);
-		pr_edge(h1.w, t1.w, n, contrib, sum);
+	const uint32_t lane = threadIdx.x & 63, nwaves = (gridDim.x * kBlock) >> 6;
+	for (uint32_t p = (blockIdx.x * kBlock + threadIdx.x) >> 6; p < npieces; p += nwaves) {
+		const uint2 pc = pieces[p];
+		const uint32_t cnt = min(pc.y + kPullPiece, in_off[pc.x + 1]) - pc.y;
+		float acc = 0.f;
+		for (uint32_t j = lane; j < cnt; j += 64) acc += contrib[in_src[pc.y + j]];
+#pragma unroll
+		for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+		if (lane == 0) atomicAdd(sum + pc.x, acc);
 	}
 }
 
@@ -1289,9 +1280,9 @@ int run_pagerank(gphi_graph *gr, uint32_t iters, float damping, float init, floa
 {
 	gphi_ctx *ctx = gr->ctx;
 	DevGraph &G = gr->g;
-	if (!G.has_tiles) return set_error(ctx, GPHI_E_STATE, "graph has no tile arrays");
 	hipStream_t s = ctx->stream;
 	int rc;
+	if ((rc = build_in_edges(gr))) return rc;
 	if (!gr->rank) {
 		if ((rc = dmalloc(ctx, (void **)&gr->rank, (size_t)G.n_pad * 4, nullptr))) return rc;
 		if ((rc = dmalloc(ctx, (void **)&gr->sum, (size_t)G.n_pad * 4, nullptr))) return rc;
@@ -1302,12 +1293,15 @@ int run_pagerank(gphi_graph *gr, uint32_t iters, float damping, float init, floa
 	fill_u32_kernel<<<grid_for(G.n, kBlock, 4096), kBlock, 0, s>>>((uint32_t *)gr->rank, G.n, bits);
 	const float first = (1.0f - damping) / (float)G.n;
 	unsigned vgrid = grid_for(G.n, kBlock, 1u << 30);
-	unsigned grid = (unsigned)std::min<uint64_t>(kDenseGrid, (G.nchunks + kNumXcd - 1) / kNumXcd * kNumXcd);
-	grid = std::max<unsigned>(grid / kNumXcd * kNumXcd, kNumXcd);
+	const unsigned lgrid = (unsigned)(((uint64_t)G.n * kPullGroup + kBlock - 1) / kBlock);
+	const unsigned hgrid = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>((gr->npieces + 3) / 4, 16384));
 	if (ctx->ev_ok) GPHI_HIP(ctx, hipEventRecord(ctx->ev[0], s));
 	for (uint32_t it = 0; it < iters; ++it) {
 		pr_prepare_kernel<<<vgrid, kBlock, 0, s>>>(G.n, gr->rank, G.deg, gr->contrib, gr->sum);
-		pr_sweep_kernel<<<grid, kBlock, 0, s>>>(G.heads, G.tails, G.nchunks, G.n, gr->contrib, gr->sum);
+		pr_pull_light_kernel<<<lgrid, kBlock, 0, s>>>(G.n, gr->in_off, gr->in_src, gr->contrib, gr->sum);
+		if (gr->npieces)
+			pr_pull_heavy_kernel<<<hgrid, kBlock, 0, s>>>(gr->npieces, gr->pieces, gr->in_off, gr->in_src,
+			                                             gr->contrib, gr->sum);
 		pr_apply_kernel<<<vgrid, kBlock, 0, s>>>(G.n, first, damping, gr->sum, gr->rank);
 	}
 	GPHI_HIP(ctx, hipGetLastError());
@@ -1323,34 +1317,64 @@ int run_pagerank(gphi_graph *gr, uint32_t iters, float damping, float init, floa
 // sweeps of dist[t] = min(dist[t], dist[h]+1); a sweep that lowers nothing is
 // the fixpoint, where further sweeps would change nothing.
 // ---------------------------------------------------------------------------
+// A sweep over the tile arrays in their stored (row-major) order, the
+// reference's edge order: per edge, dist[t] = min(dist[t], dist[h]+1).  The
+// tile order keeps both ends of a chunk's edges inside one row and one column
+// block, so the distance reads hit cache; only an edge that lowers dist[t]
+// issues an atomicMin.  Two chunks per block iteration with every load issued
+// before any is used; blocks of XCD x (blockIdx % 8) walk the chunks c with
+// c / 64 % 8 == x.  changed counts wavefronts that lowered a distance.
 __global__ __launch_bounds__(kBlock) void hop_sweep_kernel(const uint32_t *__restrict__ heads,
                                                            const uint32_t *__restrict__ tails,
                                                            uint64_t nchunks, uint32_t n, uint32_t *dist,
                                                            uint32_t *changed)
 {
-	uint32_t any = 0;
-	GPHI_FOR_CHUNKS(c, nchunks)
-	{
-		const uint4 *H = reinterpret_cast<const uint4 *>(heads + c * kChunk);
-		const uint4 *T = reinterpret_cast<const uint4 *>(tails + c * kChunk);
-		uint4 hv[2] = {H[threadIdx.x], H[threadIdx.x + kBlock]};
-		uint4 tv[2] = {T[threadIdx.x], T[threadIdx.x + kBlock]};
+	const uint32_t xcd = blockIdx.x % kNumXcd, slot = blockIdx.x / kNumXcd, per = gridDim.x / kNumXcd;
+	const uint64_t groups = (nchunks + 63) / 64;
+	const uint64_t mine = groups / kNumXcd + (xcd < groups % kNumXcd ? 1 : 0);  // groups g = xcd + 8i
+	const uint64_t my_chunks = mine * 64;
+	bool lowered = false;
+	for (uint64_t i = slot; i < my_chunks; i += 2 * per) {
+		uint64_t cc[2];
+		bool ok[2];
 #pragma unroll
-		for (int j = 0; j < 2; ++j) {
-			uint32_t hh[4] = {hv[j].x, hv[j].y, hv[j].z, hv[j].w};
-			uint32_t tt[4] = {tv[j].x, tv[j].y, tv[j].z, tv[j].w};
+		for (int q = 0; q < 2; ++q) {
+			const uint64_t k = i + (uint64_t)q * per;  // k-th chunk of this XCD
+			cc[q] = ((k / 64) * kNumXcd + xcd) * 64 + k % 64;
+			ok[q] = k < my_chunks && cc[q] < nchunks;
+		}
+		uint32_t h[16], t[16];
 #pragma unroll
-			for (int k = 0; k < 4; ++k) {
-				if (hh[k] >= n) continue;
-				uint32_t nd = dist[hh[k]] + 1;
-				if ((int32_t)nd < (int32_t)dist[tt[k]]) {
-					uint32_t old = atomicMin(dist + tt[k], nd);
-					if (nd < old) any = 1;
-				}
+		for (int q = 0; q < 2; ++q) {
+			const uint64_t c = ok[q] ? cc[q] : 0;
+			const uint4 *H = reinterpret_cast<const uint4 *>(heads + c * kChunk);
+			const uint4 *T = reinterpret_cast<const uint4 *>(tails + c * kChunk);
+			const uint4 a = ld_stream(H + threadIdx.x), b = ld_stream(H + threadIdx.x + kBlock);
+			const uint4 x = ld_stream(T + threadIdx.x), y = ld_stream(T + threadIdx.x + kBlock);
+			const uint32_t hv[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+			const uint32_t tv[8] = {x.x, x.y, x.z, x.w, y.x, y.y, y.z, y.w};
+#pragma unroll
+			for (int j = 0; j < 8; ++j) {
+				// sentinel edges (head n) and absent chunks read vertex 0 and are masked
+				const bool e = ok[q] && hv[j] < n;
+				h[8 * q + j] = e ? hv[j] : 0u;
+				t[8 * q + j] = e ? tv[j] : 0u;
 			}
 		}
+		uint32_t dh[16], dt[16];
+#pragma unroll
+		for (int j = 0; j < 16; ++j) {
+			dh[j] = dist[h[j]];
+			dt[j] = dist[t[j]];
+		}
+#pragma unroll
+		for (int j = 0; j < 16; ++j) {
+			const bool e = ok[j / 8] && h[j] != t[j];
+			const uint32_t nd = dh[j] + 1;
+			if (e && (int32_t)nd < (int32_t)dt[j] && nd < atomicMin(dist + t[j], nd)) lowered = true;
+		}
 	}
-	if (__ballot(any) && lane_id() == 0) atomicAdd(changed, 1u);
+	if (__ballot(lowered) && lane_id() == 0) atomicAdd(changed, 1u);
 }
 
 int run_bellman_ford(gphi_graph *gr, uint32_t src, uint32_t max_sweeps, uint32_t *dist,
@@ -1373,8 +1397,7 @@ int run_bellman_ford(gphi_graph *gr, uint32_t src, uint32_t max_sweeps, uint32_t
 	fill_u32_kernel<<<grid_for(G.n_pad, kBlock, 4096), kBlock, 0, s>>>(gr->dist, G.n_pad, INT_MAX - 2);
 	uint32_t zero = 0;
 	GPHI_HIP(ctx, hipMemcpyAsync(gr->dist + src, &zero, 4, hipMemcpyHostToDevice, s));
-	unsigned grid = (unsigned)std::min<uint64_t>(kDenseGrid, (G.nchunks + kNumXcd - 1) / kNumXcd * kNumXcd);
-	grid = std::max<unsigned>(grid / kNumXcd * kNumXcd, kNumXcd);
+	const unsigned grid = kBfsDenseGrid;
 	uint32_t done = 0, run = 0;
 	uint32_t hcnt[kBatch];
 	int err = GPHI_OK;

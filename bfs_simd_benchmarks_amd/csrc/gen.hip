@@ -670,18 +670,19 @@ __global__ void piece_emit_kernel(uint32_t n, const uint32_t *__restrict__ off, 
 }
 }  // namespace
 
-// In-edge arrays for the pull steps of the multi-source BFS: in_src holds the
-// heads of every edge grouped by tail (ascending heads within a tail, as a
-// stable sort of the CSR order gives), in_off[v] the first in-edge of v, and
-// pieces the (vertex, first edge) of every kPullPiece-edge piece of a vertex
-// with more than kPullLight in-edges.
+// In-edge arrays for the pull sweeps (multi-source BFS, PageRank, hop
+// relaxation): in_src holds the heads of every edge grouped by tail
+// (ascending heads within a tail, as a stable sort of the CSR order gives;
+// tile order within a tail for a graph without CSR), in_off[v] the first
+// in-edge of v, and pieces the (vertex, first edge) of every kPullPiece-edge
+// piece of a vertex with more than kPullLight in-edges.
 int build_in_edges(gphi_graph *gr)
 {
 	gphi_ctx *ctx = gr->ctx;
 	DevGraph &G = gr->g;
 	hipStream_t s = ctx->stream;
 	if (gr->in_off) return GPHI_OK;
-	if (!G.has_csr) return set_error(ctx, GPHI_E_STATE, "in-edge arrays need the CSR");
+	if (!G.has_csr && !G.has_tiles) return set_error(ctx, GPHI_E_STATE, "graph has no edge arrays");
 	const uint32_t n = G.n;
 	const uint64_t m = G.m;
 	Scratch sc{ctx, {}};
@@ -689,7 +690,17 @@ int build_in_edges(gphi_graph *gr)
 	uint32_t *h = sc.alloc<uint32_t>(m), *kout = sc.alloc<uint32_t>(m);
 	uint32_t *pc = sc.alloc<uint32_t>((uint64_t)n + 1), *base = sc.alloc<uint32_t>((uint64_t)n + 1);
 	OOM_IF(off && src && h && kout && pc && base);
-	if (m) {
+	if (m && !G.has_csr) {
+		// only the tile arrays (the row-major files PageRank and hop relaxation
+		// read): heads grouped by tail, in tile order within a tail
+		const int eb = bits_for(n ? n - 1 : 0);
+		size_t tb = 0;
+		GPHI_HIP(ctx, hipcub::DeviceRadixSort::SortPairs(nullptr, tb, G.tails, kout, G.heads, src, m, 0, eb, s));
+		void *tmp = sc.alloc<uint8_t>(tb);
+		OOM_IF(tmp);
+		GPHI_HIP(ctx, hipcub::DeviceRadixSort::SortPairs(tmp, tb, G.tails, kout, G.heads, src, m, 0, eb, s));
+		sc.release(tmp);
+	} else if (m) {
 		GPHI_HIP(ctx, hipMemsetAsync(h, 0, m * 4, s));
 		row_start_kernel<<<grid1(n), kB, 0, s>>>(n, G.row_off, G.deg, h);
 		size_t tb = 0;

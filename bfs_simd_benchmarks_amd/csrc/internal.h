@@ -148,8 +148,6 @@ int download_graph(gphi_graph *gr, uint32_t *deg, uint32_t *cols, uint32_t *cw, 
 int ensure_traversal_state(gphi_graph *gr);
 void launch_init_blocks(gphi_graph *gr);
 void launch_frontier_update_flags(gphi_graph *gr, uint32_t next_level, uint32_t threshold);
-int launch_chunk_select_rows(gphi_graph *gr, uint32_t level, uint32_t *list_cap);
-unsigned dense_grid(const DevGraph &G);
 int run_bfs_batch(gphi_graph *gr, const uint32_t *srcs, uint32_t k, uint8_t *levels, gphi_stats *st);
 
 // generator + tool pipeline on the device (gen.hip)
