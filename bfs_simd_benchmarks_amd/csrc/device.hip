@@ -334,6 +334,7 @@ void free_traversal_state(gphi_graph *gr)
 	               (void **)&gr->visited, (void **)&gr->cur, (void **)&gr->row_stamp, (void **)&gr->col_unvis,
 	               (void **)&gr->queue, (void **)&gr->qoff, (void **)&gr->tile_lo, (void **)&gr->ctrl,
 	               (void **)&gr->blk_cnt, (void **)&gr->blk_base, (void **)&gr->live, (void **)&gr->live_cnt,
+	               (void **)&gr->trace,
 	               (void **)&gr->rank, (void **)&gr->sum, (void **)&gr->contrib, (void **)&gr->b_front,
 	               (void **)&gr->b_seen, (void **)&gr->b_next, (void **)&gr->b_level, (void **)&gr->b_sums,
 	               (void **)&gr->b_live};
@@ -357,7 +358,7 @@ void free_graph_device(gphi_graph *gr)
 	void *ps[] = {G.row_off, G.cols, G.csr_w, G.deg, G.heads, G.tails, G.coo_w, G.meta,
 	              gr->level, gr->parent, gr->dist, gr->flag, gr->visited, gr->cur,
 	              gr->row_stamp, gr->col_unvis, gr->queue, gr->qoff, gr->ctrl, gr->rank,
-	              gr->sum, gr->contrib, gr->tile_lo, gr->blk_cnt, gr->blk_base, gr->live, gr->live_cnt,
+	              gr->sum, gr->contrib, gr->tile_lo, gr->blk_cnt, gr->blk_base, gr->live, gr->live_cnt, gr->trace,
 	              gr->b_front, gr->b_seen, gr->b_next, gr->b_level, gr->b_sums, gr->b_live,
 	              gr->in_off, gr->in_src, gr->pieces};
 	for (void *p : ps)
@@ -396,11 +397,13 @@ static int ensure_state(gphi_graph *gr)
 	const size_t list_cap = live_list_cap(G.nchunks);
 	if ((rc = dmalloc(ctx, (void **)&gr->live, list_cap * kNumXcd * 4 + 4, nullptr))) return rc;
 	if ((rc = dmalloc(ctx, (void **)&gr->live_cnt, kNumXcd * 4, nullptr))) return rc;
+	if ((rc = dmalloc(ctx, (void **)&gr->trace, (GPHI_MAX_STEPS + 1) * sizeof(uint4), nullptr))) return rc;
 	const size_t nblk = G.n_pad / kPadVerts;
 	if ((rc = dmalloc(ctx, (void **)&gr->blk_cnt, nblk * sizeof(uint4), nullptr))) return rc;
 	if ((rc = dmalloc(ctx, (void **)&gr->blk_base, nblk * sizeof(uint2), nullptr))) return rc;
 	GPHI_HIP(ctx, hipMemsetAsync(gr->flag, 0, G.n_pad, ctx->stream));
 	GPHI_HIP(ctx, hipMemsetAsync(gr->ctrl, 0, sizeof(StepCtrl), ctx->stream));
+	GPHI_HIP(ctx, hipMemsetAsync(gr->live_cnt, 0, kNumXcd * 4, ctx->stream));
 	return GPHI_OK;
 }
 
@@ -422,7 +425,7 @@ __global__ void init_source_kernel(uint32_t src, uint32_t W, const uint32_t *__r
                                    int32_t *level, int32_t *parent, uint32_t *dist,
                                    uint32_t *visited, uint32_t *cur, uint32_t *row_stamp,
                                    uint32_t *col_unvis, uint32_t *queue, uint32_t *qoff,
-                                   uint32_t *tile_lo, StepCtrl *ctrl)
+                                   uint32_t *tile_lo, StepCtrl *ctrl, uint4 *trace)
 {
 	uint32_t d = deg[src];
 	uint32_t ntile = (d + kSparseTile - 1) / kSparseTile;
@@ -449,6 +452,11 @@ __global__ void init_source_kernel(uint32_t src, uint32_t W, const uint32_t *__r
 		ctrl->facc = 0;
 		ctrl->qpack = 0;
 		ctrl->changed = 0;
+		ctrl->chunks = 0;
+		ctrl->level = 0;
+		ctrl->reached = 1;
+		ctrl->traversed = d;
+		trace[0] = make_uint4(0u, 1u, d, 0u);
 	}
 }
 
@@ -463,9 +471,13 @@ __global__ void init_source_kernel(uint32_t src, uint32_t W, const uint32_t *__r
 template <bool kColSkip>
 __global__ __launch_bounds__(kBlock) void chunk_select_kernel(
     const ChunkMeta *__restrict__ meta, uint64_t nchunks, const uint32_t *__restrict__ row_stamp,
-    const uint32_t *__restrict__ col_unvis, uint32_t level, uint32_t *__restrict__ lists,
+    const uint32_t *__restrict__ col_unvis, const StepCtrl *__restrict__ ctrl, uint32_t *__restrict__ lists,
     uint32_t list_cap, uint32_t *__restrict__ counts)
 {
+	// a sparse step (or a finished traversal) selects nothing; the counts
+	// stay 0, so the dense sweep launched after it returns at once
+	if (ctrl->done || ctrl->dir != 1) return;
+	const uint32_t level = ctrl->level;
 	const uint64_t c = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
 	bool live = false;
 	if (c < nchunks) {
@@ -744,7 +756,7 @@ __global__ __launch_bounds__(kBlock) void sssp_dense_kernel(
 template <bool kBfs>
 __global__ __launch_bounds__(kBlock) void sparse_push_kernel(
     const uint32_t *__restrict__ queue, const uint32_t *__restrict__ qoff,
-    const uint32_t *__restrict__ tile_lo, uint32_t qsize, uint32_t qdeg,
+    const uint32_t *__restrict__ tile_lo, const StepCtrl *__restrict__ ctrl,
     const uint32_t *__restrict__ row_off, const uint32_t *__restrict__ cols,
     const uint32_t *__restrict__ csr_w, const uint32_t *__restrict__ visited,
     uint8_t *__restrict__ flag, int32_t *__restrict__ parent, uint32_t *dist)
@@ -752,6 +764,8 @@ __global__ __launch_bounds__(kBlock) void sparse_push_kernel(
 	__shared__ uint32_t s_off[kSparseTile + 2];
 	__shared__ uint32_t s_u[kSparseTile + 2];
 	__shared__ uint32_t s_base[kSparseTile + 2];
+	if (ctrl->done || ctrl->dir != 0) return;  // a dense step, or a finished traversal
+	const uint32_t qsize = ctrl->qsize, qdeg = ctrl->fdeg;
 	uint32_t ntile = (qdeg + kSparseTile - 1) / kSparseTile;
 	for (uint32_t k = blockIdx.x; k < ntile; k += gridDim.x) {
 		uint32_t lo = tile_lo[k];
@@ -808,13 +822,20 @@ __global__ __launch_bounds__(kBlock) void sparse_push_kernel(
 //          its degree prefix and the tile partition of the next CSR push
 // ---------------------------------------------------------------------------
 
+constexpr uint32_t kLevelFromCtrl = UINT32_MAX;  // mark kernel: take the level from the control block
+
 template <bool kBfs>
 __global__ __launch_bounds__(kBlock) void frontier_mark_kernel(
     uint32_t W, uint32_t next_level, uint8_t *__restrict__ flag, const uint32_t *__restrict__ deg,
     int32_t *__restrict__ level, uint32_t *__restrict__ visited, uint32_t *__restrict__ cur,
-    uint32_t *__restrict__ row_stamp, uint32_t *__restrict__ col_unvis, uint4 *__restrict__ blk_cnt)
+    uint32_t *__restrict__ row_stamp, uint32_t *__restrict__ col_unvis, uint4 *__restrict__ blk_cnt,
+    const StepCtrl *__restrict__ ctrl)
 {
 	__shared__ uint32_t s_red[3][kBlock / 64];
+	if (next_level == kLevelFromCtrl) {  // device-driven loop
+		if (ctrl->done) return;
+		next_level = ctrl->level + 1;
+	}
 	const uint32_t w = blockIdx.x * kBlock + threadIdx.x;
 	const uint32_t v0 = w * 32;
 	uint4 *fp = reinterpret_cast<uint4 *>(flag + v0);
@@ -898,9 +919,10 @@ constexpr uint32_t kScanThreads = 1024;
 __global__ __launch_bounds__(kScanThreads) void frontier_scan_kernel(uint32_t nblk, uint32_t threshold,
                                                                      const uint4 *__restrict__ blk_cnt,
                                                                      uint2 *__restrict__ blk_base,
-                                                                     const uint32_t *__restrict__ live_cnt,
-                                                                     StepCtrl *ctrl)
+                                                                     uint32_t *__restrict__ live_cnt,
+                                                                     StepCtrl *ctrl, uint4 *__restrict__ trace)
 {
+	if (trace && ctrl->done) return;  // device-driven loop: nothing left to do
 	__shared__ uint32_t sq[kScanThreads / 64], sd[kScanThreads / 64], sf[kScanThreads / 64];
 	const uint32_t per = (nblk + kScanThreads - 1) / kScanThreads;
 	const uint32_t b0 = min(nblk, threadIdx.x * per), b1 = min(nblk, b0 + per);
@@ -948,6 +970,17 @@ __global__ __launch_bounds__(kScanThreads) void frontier_scan_kernel(uint32_t nb
 		uint32_t chunks = 0;  // chunks the step's dense sweep took (its live lists)
 		for (int x = 0; x < kNumXcd; ++x) chunks += live_cnt[x];
 		ctrl->chunks = chunks;
+		if (trace) {
+			// device-driven loop: record the step, open the next one, and clear
+			// the live-list counts for its chunk selection
+			const uint32_t L = ctrl->level;
+			if (L <= GPHI_MAX_STEPS) trace[L].w = chunks;
+			if (fall && L + 1 <= GPHI_MAX_STEPS) trace[L + 1] = make_uint4(ctrl->dir, fall, dall, 0u);
+			ctrl->level = L + 1;
+			ctrl->reached += fall;
+			ctrl->traversed += dall;
+			for (int x = 0; x < kNumXcd; ++x) live_cnt[x] = 0;
+		}
 	}
 }
 
@@ -958,7 +991,7 @@ __global__ __launch_bounds__(kBlock) void frontier_emit_kernel(
 {
 	__shared__ uint32_t sq[kBlock / 64], sd[kBlock / 64];
 	// the queue only feeds a sparse step; empty blocks have nothing to emit
-	if (ctrl->dir != 0 || blk_cnt[blockIdx.x].y == 0) return;
+	if (ctrl->done || ctrl->dir != 0 || blk_cnt[blockIdx.x].y == 0) return;
 	const uint32_t w = blockIdx.x * kBlock + threadIdx.x;
 	const uint32_t v0 = w * 32;
 	const uint32_t bits = cur[w];
@@ -1014,7 +1047,8 @@ __global__ __launch_bounds__(kBlock) void frontier_emit_kernel(
 // host drivers
 // ---------------------------------------------------------------------------
 constexpr unsigned kBfsDenseGrid = 2048;  // BFS/SSSP sweeps: 8 blocks per CU, every block resident at once
-static constexpr unsigned kSparseGrid = 4096;
+static constexpr unsigned kSparseGrid = 2048;
+static constexpr uint32_t kLevelGroup = 4;  // levels enqueued per read-back of the control block
 
 static float ev_ms(hipEvent_t a, hipEvent_t b)
 {
@@ -1029,6 +1063,8 @@ static int frontier_loop(gphi_graph *gr, uint32_t src, gphi_stats *st)
 	gphi_ctx *ctx = gr->ctx;
 	DevGraph &G = gr->g;
 	hipStream_t s = ctx->stream;
+	if (!G.has_csr || !G.has_tiles)
+		return set_error(ctx, GPHI_E_STATE, "a hybrid traversal needs the CSR and the tile arrays");
 	const uint32_t threshold = (uint32_t)G.m / ctx->t_ratio;
 	StepCtrl *hc = ctx->h_ctrl;
 	hipEvent_t *ev = ctx->ev;
@@ -1045,85 +1081,81 @@ static int frontier_loop(gphi_graph *gr, uint32_t src, gphi_stats *st)
 		fill_u32_kernel<<<grid_for(G.n_pad, kBlock, 4096), kBlock, 0, s>>>(gr->dist, G.n_pad, INT_MAX);
 	}
 	GPHI_HIP(ctx, hipMemsetAsync(gr->cur, 0, (size_t)G.n_words * 4, s));
+	GPHI_HIP(ctx, hipMemsetAsync(gr->live_cnt, 0, kNumXcd * 4, s));
 	init_blocks_kernel<<<grid_for(G.side, kBlock, 65535), kBlock, 0, s>>>(G.n, G.W, G.side,
 	                                                                      gr->row_stamp, gr->col_unvis);
 	init_source_kernel<kBfs><<<1, kBlock, 0, s>>>(src, G.W, G.deg, gr->level, gr->parent, gr->dist,
 	                                               gr->visited, gr->cur, gr->row_stamp, gr->col_unvis,
-	                                               gr->queue, gr->qoff, gr->tile_lo, gr->ctrl);
+	                                               gr->queue, gr->qoff, gr->tile_lo, gr->ctrl, gr->trace);
 	GPHI_HIP(ctx, hipGetLastError());
 	if (timing) GPHI_HIP(ctx, hipEventRecord(ev[1], s));
-	GPHI_HIP(ctx, hipMemcpyAsync(hc, gr->ctrl, sizeof(StepCtrl), hipMemcpyDeviceToHost, s));
-	GPHI_HIP(ctx, hipStreamSynchronize(s));
 
-	uint32_t steps = 0;
-	uint64_t reached = 0, trav = 0;
+	// The level loop runs on the device.  Every kernel of a level reads the
+	// direction, the level and the queue sizes from the control block: the
+	// CSR push of a dense step and the chunk selection and sweep of a sparse
+	// step return at once (the first step is sparse, bfs.cpp:807), and every
+	// kernel of a level past the last one returns at once.  So the host
+	// enqueues kLevelGroup levels at a time and reads the control block back
+	// once per group, not once per level.
 	const unsigned nblk = G.n_pad / kPadVerts;
-	while (!hc->done) {
-		const uint32_t L = steps;
-		const uint32_t dir = (steps == 0) ? 0u : hc->dir;  // first step sparse (bfs.cpp:807)
-		if (st && steps < GPHI_MAX_STEPS) {
-			st->dir[steps] = dir;
-			st->fsize[steps] = hc->fsize;
-			st->fdeg[steps] = hc->fdeg;
-		}
-		reached += hc->fsize;
-		trav += hc->fdeg;
-		const bool ev_step = timing && steps < GPHI_MAX_STEPS - 1;
-		if (dir == 0) {
-			if (!G.has_csr) return set_error(ctx, GPHI_E_STATE, "graph has no CSR arrays");
-			if (hc->fdeg)
-				sparse_push_kernel<kBfs><<<grid_for(hc->fdeg, kSparseTile, kSparseGrid), kBlock, 0, s>>>(
-				    gr->queue, gr->qoff, gr->tile_lo, hc->qsize, hc->fdeg, G.row_off, G.cols, G.csr_w,
-				    gr->visited, gr->flag, gr->parent, gr->dist);
-		} else {
-			if (!G.has_tiles) return set_error(ctx, GPHI_E_STATE, "graph has no tile arrays");
-			const uint32_t list_cap = live_list_cap(G.nchunks);
-			GPHI_HIP(ctx, hipMemsetAsync(gr->live_cnt, 0, kNumXcd * 4, s));
-			const unsigned sgrid = (unsigned)std::max<uint64_t>(1, (G.nchunks + kBlock - 1) / kBlock);
-			if (kBfs)
-				chunk_select_kernel<true><<<sgrid, kBlock, 0, s>>>(G.meta, G.nchunks, gr->row_stamp,
-				                                                 gr->col_unvis, L, gr->live, list_cap,
-				                                                 gr->live_cnt);
-			else
-				chunk_select_kernel<false><<<sgrid, kBlock, 0, s>>>(G.meta, G.nchunks, gr->row_stamp,
-				                                                  gr->col_unvis, L, gr->live, list_cap,
-				                                                  gr->live_cnt);
-			if (ev_step) GPHI_HIP(ctx, hipEventRecord(ev[2 * GPHI_MAX_STEPS + 4 + steps], s));
-			unsigned grid = (unsigned)std::min<uint64_t>(kBfsDenseGrid,
-			                                             (G.nchunks + kNumXcd - 1) / kNumXcd * kNumXcd);
-			grid = std::max<unsigned>(grid / kNumXcd * kNumXcd, kNumXcd);
+	const uint32_t list_cap = live_list_cap(G.nchunks);
+	const unsigned sgrid = (unsigned)std::max<uint64_t>(1, (G.nchunks + kBlock - 1) / kBlock);
+	unsigned grid = (unsigned)std::min<uint64_t>(kBfsDenseGrid, (G.nchunks + kNumXcd - 1) / kNumXcd * kNumXcd);
+	grid = std::max<unsigned>(grid / kNumXcd * kNumXcd, kNumXcd);
+	uint32_t enq = 0;
+	for (;;) {
+		for (uint32_t q = 0; q < kLevelGroup; ++q, ++enq) {
+			const bool ev_step = timing && enq < GPHI_MAX_STEPS - 1;
+			sparse_push_kernel<kBfs><<<kSparseGrid, kBlock, 0, s>>>(gr->queue, gr->qoff, gr->tile_lo, gr->ctrl,
+			                                                     G.row_off, G.cols, G.csr_w, gr->visited,
+			                                                     gr->flag, gr->parent, gr->dist);
+			chunk_select_kernel<kBfs><<<sgrid, kBlock, 0, s>>>(G.meta, G.nchunks, gr->row_stamp, gr->col_unvis,
+			                                                 gr->ctrl, gr->live, list_cap, gr->live_cnt);
+			if (ev_step) GPHI_HIP(ctx, hipEventRecord(ev[2 * GPHI_MAX_STEPS + 4 + enq], s));
 			if (kBfs)
 				bfs_dense_kernel<<<grid, kBlock, 0, s>>>(G.heads, G.tails, gr->live, list_cap, gr->live_cnt,
 				                                       gr->cur, gr->visited, gr->flag, gr->parent);
 			else
 				sssp_dense_kernel<<<grid, kBlock, 0, s>>>(G.heads, G.tails, G.coo_w, gr->live, list_cap,
 				                                        gr->live_cnt, gr->cur, gr->dist, gr->flag);
+			if (ev_step) GPHI_HIP(ctx, hipEventRecord(ev[2 + 2 * enq], s));
+			frontier_mark_kernel<kBfs><<<nblk, kBlock, 0, s>>>(G.W, kLevelFromCtrl, gr->flag, G.deg, gr->level,
+			                                                  gr->visited, gr->cur, gr->row_stamp, gr->col_unvis,
+			                                                  gr->blk_cnt, gr->ctrl);
+			frontier_scan_kernel<<<1, kScanThreads, 0, s>>>(nblk, threshold, gr->blk_cnt, gr->blk_base,
+			                                              gr->live_cnt, gr->ctrl, gr->trace);
+			frontier_emit_kernel<<<nblk, kBlock, 0, s>>>(gr->ctrl, G.deg, gr->cur, gr->blk_cnt, gr->blk_base,
+			                                            gr->queue, gr->qoff, gr->tile_lo);
+			if (ev_step) GPHI_HIP(ctx, hipEventRecord(ev[3 + 2 * enq], s));
 		}
 		GPHI_HIP(ctx, hipGetLastError());
-		if (ev_step) GPHI_HIP(ctx, hipEventRecord(ev[2 + 2 * steps], s));
-		frontier_mark_kernel<kBfs><<<nblk, kBlock, 0, s>>>(G.W, L + 1, gr->flag, G.deg, gr->level, gr->visited,
-		                                                  gr->cur, gr->row_stamp, gr->col_unvis, gr->blk_cnt);
-		frontier_scan_kernel<<<1, kScanThreads, 0, s>>>(nblk, threshold, gr->blk_cnt, gr->blk_base, gr->live_cnt,
-	                                              gr->ctrl);
-		frontier_emit_kernel<<<nblk, kBlock, 0, s>>>(gr->ctrl, G.deg, gr->cur, gr->blk_cnt, gr->blk_base, gr->queue,
-		                                            gr->qoff, gr->tile_lo);
-		GPHI_HIP(ctx, hipGetLastError());
-		if (ev_step) GPHI_HIP(ctx, hipEventRecord(ev[3 + 2 * steps], s));
 		GPHI_HIP(ctx, hipMemcpyAsync(hc, gr->ctrl, sizeof(StepCtrl), hipMemcpyDeviceToHost, s));
 		GPHI_HIP(ctx, hipStreamSynchronize(s));
-		if (st && dir && steps < GPHI_MAX_STEPS) st->chunks[steps] = hc->chunks;
-		++steps;
-		if (steps > G.n + 2) return set_error(ctx, GPHI_E_STATE, "frontier loop did not terminate");
+		if (hc->done) break;
+		if (enq > G.n + 2 + kLevelGroup) return set_error(ctx, GPHI_E_STATE, "frontier loop did not terminate");
 	}
+	const uint32_t steps = hc->level;  // frontiers expanded
 	if (st) {
-		st->steps = std::min<uint32_t>(steps, GPHI_MAX_STEPS);
-		st->reached = reached;
-		st->edges_traversed = trav;
+		const uint32_t ns = std::min<uint32_t>(steps, GPHI_MAX_STEPS);
+		std::vector<uint4> tr(ns);
+		if (ns) {
+			GPHI_HIP(ctx, hipMemcpyAsync(tr.data(), gr->trace, ns * sizeof(uint4), hipMemcpyDeviceToHost, s));
+			GPHI_HIP(ctx, hipStreamSynchronize(s));
+		}
+		st->steps = ns;
+		for (uint32_t k = 0; k < ns; ++k) {
+			st->dir[k] = tr[k].x;
+			st->fsize[k] = tr[k].y;
+			st->fdeg[k] = tr[k].z;
+			st->chunks[k] = tr[k].x ? tr[k].w : 0u;
+		}
+		st->reached = hc->reached;
+		st->edges_traversed = hc->traversed;
 		if (timing) {
 			st->init_ms = ev_ms(ev[0], ev[1]);
 			hipEvent_t prev = ev[1];
-			uint32_t ns = std::min<uint32_t>(steps, GPHI_MAX_STEPS - 1);
-			for (uint32_t k = 0; k < ns; ++k) {
+			const uint32_t nt = std::min<uint32_t>(steps, GPHI_MAX_STEPS - 1);
+			for (uint32_t k = 0; k < nt; ++k) {
 				if (st->dir[k]) {
 					hipEvent_t sel = ev[2 * GPHI_MAX_STEPS + 4 + k];
 					st->select_ms[k] = ev_ms(prev, sel);
@@ -1134,7 +1166,7 @@ static int frontier_loop(gphi_graph *gr, uint32_t src, gphi_stats *st)
 				st->update_ms[k] = ev_ms(ev[2 + 2 * k], ev[3 + 2 * k]);
 				prev = ev[3 + 2 * k];
 			}
-			st->total_ms = ns ? ev_ms(ev[0], ev[1 + 2 * ns]) : st->init_ms;
+			st->total_ms = nt ? ev_ms(ev[0], ev[1 + 2 * nt]) : st->init_ms;
 		}
 	}
 	return GPHI_OK;
@@ -1158,14 +1190,13 @@ void launch_frontier_update_flags(gphi_graph *gr, uint32_t next_level, uint32_t 
 	const unsigned nblk = G.n_pad / kPadVerts;
 	frontier_mark_kernel<false><<<nblk, kBlock, 0, s>>>(G.W, next_level, gr->flag, G.deg, gr->level,
 	                                                   gr->visited, gr->cur, gr->row_stamp, gr->col_unvis,
-	                                                   gr->blk_cnt);
+	                                                   gr->blk_cnt, gr->ctrl);
 	frontier_scan_kernel<<<1, kScanThreads, 0, s>>>(nblk, threshold, gr->blk_cnt, gr->blk_base, gr->live_cnt,
-	                                              gr->ctrl);
+	                                              gr->ctrl, nullptr);
 	frontier_emit_kernel<<<nblk, kBlock, 0, s>>>(gr->ctrl, G.deg, gr->cur, gr->blk_cnt, gr->blk_base,
 	                                            gr->queue, gr->qoff, gr->tile_lo);
 }
 
-// per-XCD live-chunk lists of the rows stamped `level` (no column test)
 int run_bfs(gphi_graph *gr, uint32_t src, int32_t *levels, int32_t *parents, gphi_stats *st)
 {
 	gphi_ctx *ctx = gr->ctx;

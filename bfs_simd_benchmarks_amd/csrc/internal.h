@@ -34,8 +34,10 @@ struct ChunkMeta {
 	uint32_t row_lo, row_hi, col_lo, col_hi;
 };
 
-// Control block of one frontier traversal, read back by the host after each
-// step (64 bytes).
+// Control block of one frontier traversal (128 bytes).  The level loop runs
+// on the device: every kernel of a level reads the direction, the level and
+// the queue sizes here, and the host reads the block back only every few
+// levels to see whether the traversal is done.
 struct StepCtrl {
 	uint32_t fsize;         // |F| of the frontier just produced
 	uint32_t fdeg;          // sum of out-degrees of that frontier
@@ -48,8 +50,12 @@ struct StepCtrl {
 	unsigned long long qpack;  // accumulator: (queue count << 32) | degree sum
 	uint32_t changed;       // hop relaxation: edges that lowered a distance
 	uint32_t chunks;        // chunks on the live lists of the last dense sweep
-	uint32_t pad[4];
+	uint32_t level;         // the step being expanded (device-driven loop)
+	uint32_t pad1;
+	unsigned long long reached, traversed;  // sums of |F| and of sum deg(F) over the steps
+	uint32_t pad[14];
 };
+static_assert(sizeof(StepCtrl) == 128, "StepCtrl is 128 bytes");
 
 struct DevGraph {
 	uint32_t n = 0;
@@ -107,6 +113,7 @@ struct gphi_graph {
 	uint2 *blk_base = nullptr; // per update block: queue and degree prefix
 	uint32_t *live = nullptr;  // per-XCD lists of live chunks of a dense step
 	uint32_t *live_cnt = nullptr;
+	uint4 *trace = nullptr;    // per step: direction, |F|, sum deg(F), chunks swept
 	float *rank = nullptr, *sum = nullptr, *contrib = nullptr;
 	// multi-source BFS (batch.hip): per-vertex source masks and uint8 levels
 	uint32_t batch_words = 0;  // 64-bit words per vertex the buffers were sized for
