@@ -456,6 +456,7 @@ int batch_loop(gphi_graph *gr, const uint32_t *d_srcs, uint32_t k, gphi_stats *s
 	GPHI_HIP(ctx, hipMemsetAsync(gr->b_seen, 0, mask_bytes, s));
 	GPHI_HIP(ctx, hipMemsetAsync(gr->b_next, 0, mask_bytes, s));
 	GPHI_HIP(ctx, hipMemsetAsync(gr->b_level, kUnreached, (size_t)k * G.n_pad, s));
+	GPHI_HIP(ctx, hipMemsetAsync(gr->flag, 0, G.n_pad, s));
 	GPHI_HIP(ctx, hipMemsetAsync(gr->cur, 0, (size_t)G.n_words * 4, s));
 	GPHI_HIP(ctx, hipMemsetAsync(gr->b_live, 0, (size_t)(1 + kLiveSlots) * kLiveStride * 8, s));
 	u64 *live = gr->b_live, *slots = gr->b_live + kLiveStride;

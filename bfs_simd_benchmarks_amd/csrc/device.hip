@@ -331,7 +331,7 @@ int upload_graph(gphi_ctx *ctx, const gphi_graph_desc *d, gphi_graph *gr)
 void free_traversal_state(gphi_graph *gr)
 {
 	void **ps[] = {(void **)&gr->level, (void **)&gr->parent, (void **)&gr->dist, (void **)&gr->flag,
-	               (void **)&gr->visited, (void **)&gr->cur, (void **)&gr->row_stamp, (void **)&gr->col_unvis,
+	               (void **)&gr->cur, (void **)&gr->row_stamp, (void **)&gr->col_unvis,
 	               (void **)&gr->queue, (void **)&gr->qoff, (void **)&gr->tile_lo, (void **)&gr->ctrl,
 	               (void **)&gr->blk_cnt, (void **)&gr->blk_base, (void **)&gr->live, (void **)&gr->live_cnt,
 	               (void **)&gr->trace,
@@ -356,7 +356,7 @@ void free_graph_device(gphi_graph *gr)
 		return;
 	}
 	void *ps[] = {G.row_off, G.cols, G.csr_w, G.deg, G.heads, G.tails, G.coo_w, G.meta,
-	              gr->level, gr->parent, gr->dist, gr->flag, gr->visited, gr->cur,
+	              gr->level, gr->parent, gr->dist, gr->flag, gr->cur,
 	              gr->row_stamp, gr->col_unvis, gr->queue, gr->qoff, gr->ctrl, gr->rank,
 	              gr->sum, gr->contrib, gr->tile_lo, gr->blk_cnt, gr->blk_base, gr->live, gr->live_cnt, gr->trace,
 	              gr->b_front, gr->b_seen, gr->b_next, gr->b_level, gr->b_sums, gr->b_live,
@@ -386,7 +386,6 @@ static int ensure_state(gphi_graph *gr)
 	if ((rc = dmalloc(ctx, (void **)&gr->level, nb, nullptr))) return rc;
 	if ((rc = dmalloc(ctx, (void **)&gr->parent, nb, nullptr))) return rc;
 	if ((rc = dmalloc(ctx, (void **)&gr->flag, G.n_pad, nullptr))) return rc;
-	if ((rc = dmalloc(ctx, (void **)&gr->visited, (size_t)G.n_words * 4, nullptr))) return rc;
 	if ((rc = dmalloc(ctx, (void **)&gr->cur, (size_t)G.n_words * 4, nullptr))) return rc;
 	if ((rc = dmalloc(ctx, (void **)&gr->row_stamp, ((size_t)G.side + 1) * 4, nullptr))) return rc;
 	if ((rc = dmalloc(ctx, (void **)&gr->col_unvis, ((size_t)G.side + 1) * 4, nullptr))) return rc;
@@ -423,7 +422,7 @@ __global__ void init_blocks_kernel(uint32_t n, uint32_t W, uint32_t side, uint32
 template <bool kBfs>
 __global__ void init_source_kernel(uint32_t src, uint32_t W, const uint32_t *__restrict__ deg,
                                    int32_t *level, int32_t *parent, uint32_t *dist,
-                                   uint32_t *visited, uint32_t *cur, uint32_t *row_stamp,
+                                   uint8_t *flag, uint32_t *cur, uint32_t *row_stamp,
                                    uint32_t *col_unvis, uint32_t *queue, uint32_t *qoff,
                                    uint32_t *tile_lo, StepCtrl *ctrl, uint4 *trace)
 {
@@ -434,7 +433,7 @@ __global__ void init_source_kernel(uint32_t src, uint32_t W, const uint32_t *__r
 		if (kBfs) {
 			level[src] = 0;
 			parent[src] = (int32_t)src;
-			visited[src >> 5] |= 1u << (src & 31);
+			flag[src] = kVisited;
 			col_unvis[src / W] -= 1;
 		} else {
 			dist[src] = 0;
@@ -534,16 +533,18 @@ __device__ __forceinline__ uint4 ld_stream(const uint4 *p)
 }
 
 // Software-pipelined walk of this XCD's live-chunk list, 8 edges per lane per
-// chunk.  While chunk k finishes (visited bits, flags, writes), the frontier
-// bits of chunk k+1 are looked up, its tails are requested, and the heads of
-// chunk k+2 are requested, so each iteration waits on about one HBM latency
-// instead of a chain of them.  A wavefront's loads complete in issue order,
-// so within an iteration the loads needed first are issued first, and every
-// load is unconditional (a lane with nothing to fetch reads a cached dummy
-// word) so the compiler can count them.  Tails of a 4-edge quad are fetched
-// only when one of its heads is in the frontier.  The winner of a tail writes
-// parent and flag (benign races: every writer stores a valid frontier head and
-// the same flag).
+// chunk.  While chunk k finishes (state bytes of its tails, writes), the
+// frontier bits of chunk k+1 are looked up, and then its tails and the heads
+// of chunk k+2 are requested, so each iteration waits on one HBM latency and
+// one L2 round trip.  Every load is unconditional (a lane with nothing to
+// fetch reads a cached dummy word) so the compiler can count them.  Tails of
+// a 4-edge quad are fetched only when one of its heads is in the frontier.
+// One byte per vertex says whether a tail is new (state 0): "visited before
+// this step" and "reached in this step" live in the same byte, so a
+// candidate costs one lookup.  Every lane that sees state 0 writes parent and state 1
+// (benign races: every writer stores a valid frontier head and the same
+// state; an earlier step's kVisited is never overwritten, since no lane of
+// this step reads state 0 for such a vertex).
 __device__ __forceinline__ void unpack8(const uint4 a, const uint4 b, uint32_t (&x)[8])
 {
 	x[0] = a.x, x[1] = a.y, x[2] = a.z, x[3] = a.w, x[4] = b.x, x[5] = b.y, x[6] = b.z, x[7] = b.w;
@@ -552,8 +553,7 @@ __device__ __forceinline__ void unpack8(const uint4 a, const uint4 b, uint32_t (
 __global__ __launch_bounds__(kBlock) void bfs_dense_kernel(
     const uint32_t *__restrict__ heads, const uint32_t *__restrict__ tails,
     const uint32_t *__restrict__ lists, uint32_t list_cap, const uint32_t *__restrict__ counts,
-    const uint32_t *__restrict__ cur, const uint32_t *__restrict__ visited,
-    uint8_t *__restrict__ flag, int32_t *__restrict__ parent)
+    const uint32_t *__restrict__ cur, uint8_t *__restrict__ state, int32_t *__restrict__ parent)
 {
 	const uint32_t xcd = blockIdx.x % kNumXcd, per = gridDim.x / kNumXcd;
 	const uint32_t cnt = counts[xcd];
@@ -589,24 +589,20 @@ __global__ __launch_bounds__(kBlock) void bfs_dense_kernel(
 		unpack8(ld_stream((hit & 0x0Fu) ? T + tid : D), ld_stream((hit & 0xF0u) ? T + tid + kBlock : D), t);
 	}
 	for (;;) {
-		// stage A: visited bits of chunk k's candidate tails, frontier bits of chunk k+1
-		uint32_t vw[8], cw[8];
+		// stage A: states of chunk k's frontier-edge tails, frontier bits of chunk k+1
+		uint32_t sv[8], cw[8];
 #pragma unroll
-		for (int j = 0; j < 8; ++j) vw[j] = visited[((hit >> j) & 1u) ? t[j] >> 5 : 0u];
+		for (int j = 0; j < 8; ++j) sv[j] = state[((hit >> j) & 1u) ? t[j] : 0u];
 #pragma unroll
 		for (int j = 0; j < 8; ++j) cw[j] = cur[hn[j] >> 5];
 		uint32_t cand = 0, hit_n = 0;
 #pragma unroll
-		for (int j = 0; j < 8; ++j) cand |= (((vw[j] >> (t[j] & 31)) & 1u) ^ 1u) << j;
+		for (int j = 0; j < 8; ++j) cand |= (sv[j] == 0 ? 1u : 0u) << j;
 		cand &= hit;
 #pragma unroll
 		for (int j = 0; j < 8; ++j) hit_n |= ((cw[j] >> (hn[j] & 31)) & 1u) << j;
 		if (!v1) hit_n = 0;
-		// stage B: flags of chunk k's candidates (needed first), then the
-		// HBM requests of the next iteration: tails of k+1, heads of k+2
-		uint8_t fl[8];
-#pragma unroll
-		for (int j = 0; j < 8; ++j) fl[j] = flag[((cand >> j) & 1u) ? t[j] : 0u];
+		// stage B: the HBM requests of the next iteration: tails of k+1, heads of k+2
 		uint32_t tn[8], h2[8];
 		{
 			const uint4 *T = reinterpret_cast<const uint4 *>(tails + (uint64_t)c1 * kChunk);
@@ -621,9 +617,9 @@ __global__ __launch_bounds__(kBlock) void bfs_dense_kernel(
 		// stage C: chunk k's writes
 #pragma unroll
 		for (int j = 0; j < 8; ++j)
-			if (((cand >> j) & 1u) && !fl[j]) {
+			if ((cand >> j) & 1u) {
 				parent[t[j]] = (int32_t)h[j];
-				flag[t[j]] = 1;
+				state[t[j]] = 1;
 			}
 		if (!v1) break;
 		pos += per;
@@ -758,7 +754,7 @@ __global__ __launch_bounds__(kBlock) void sparse_push_kernel(
     const uint32_t *__restrict__ queue, const uint32_t *__restrict__ qoff,
     const uint32_t *__restrict__ tile_lo, const StepCtrl *__restrict__ ctrl,
     const uint32_t *__restrict__ row_off, const uint32_t *__restrict__ cols,
-    const uint32_t *__restrict__ csr_w, const uint32_t *__restrict__ visited,
+    const uint32_t *__restrict__ csr_w,
     uint8_t *__restrict__ flag, int32_t *__restrict__ parent, uint32_t *dist)
 {
 	__shared__ uint32_t s_off[kSparseTile + 2];
@@ -795,7 +791,7 @@ __global__ __launch_bounds__(kBlock) void sparse_push_kernel(
 			uint32_t idx = s_base[a] + (e - s_off[a]);
 			uint32_t v = cols[idx];
 			if (kBfs) {
-				if (!test_bit(visited, v) && !flag[v]) {
+				if (flag[v] == 0) {
 					parent[v] = (int32_t)u;
 					flag[v] = 1;
 				}
@@ -815,8 +811,9 @@ __global__ __launch_bounds__(kBlock) void sparse_push_kernel(
 // frontier update (update_dense :91-179, update_sparse :467-509, to_dense
 // :54-88, to_sparse :400-465 and the rule :829-834 of apps/bfs_simd/bfs.cpp,
 // fused).  Three launches, no same-address atomics on the hot counters:
-//   mark : one lane per 32-vertex word: flags -> next-frontier word, visited,
-//          levels, row stamps, column counts; per-block (|F|, queue, sum deg)
+//   mark : one lane per 32-vertex word: state bytes -> next-frontier word,
+//          state 1 -> kVisited (BFS) or 0, levels, row stamps, column counts;
+//          per-block (|F|, queue, sum deg)
 //   scan : one block: exclusive scan of the per-block counts, direction rule
 //   emit : one lane per word: the sparse queue in ascending vertex order with
 //          its degree prefix and the tile partition of the next CSR push
@@ -827,7 +824,7 @@ constexpr uint32_t kLevelFromCtrl = UINT32_MAX;  // mark kernel: take the level 
 template <bool kBfs>
 __global__ __launch_bounds__(kBlock) void frontier_mark_kernel(
     uint32_t W, uint32_t next_level, uint8_t *__restrict__ flag, const uint32_t *__restrict__ deg,
-    int32_t *__restrict__ level, uint32_t *__restrict__ visited, uint32_t *__restrict__ cur,
+    int32_t *__restrict__ level, uint32_t *__restrict__ cur,
     uint32_t *__restrict__ row_stamp, uint32_t *__restrict__ col_unvis, uint4 *__restrict__ blk_cnt,
     const StepCtrl *__restrict__ ctrl)
 {
@@ -846,14 +843,20 @@ __global__ __launch_bounds__(kBlock) void frontier_mark_kernel(
 	for (int j = 0; j < 8; ++j)
 #pragma unroll
 		for (int k = 0; k < 4; ++k)
-			if ((fw[j] >> (8 * k)) & 0xFFu) bits |= 1u << (4 * j + k);
+			bits |= ((fw[j] >> (8 * k)) & 1u) << (4 * j + k);
 	uint32_t F = 0, Q = 0, D = 0;
 	cur[w] = bits;
 	const bool row_uniform = (W & 31u) == 0;
 	if (bits) {
-		fp[0] = make_uint4(0, 0, 0, 0);
-		fp[1] = make_uint4(0, 0, 0, 0);
-		if (kBfs) visited[w] |= bits;
+		if (kBfs) {
+			// state 1 (reached this step) -> kVisited; 0 and kVisited stay
+			constexpr uint32_t kLow = 0x01010101u;
+			fp[0] = make_uint4(fa.x + (fa.x & kLow), fa.y + (fa.y & kLow), fa.z + (fa.z & kLow), fa.w + (fa.w & kLow));
+			fp[1] = make_uint4(fb.x + (fb.x & kLow), fb.y + (fb.y & kLow), fb.z + (fb.z & kLow), fb.w + (fb.w & kLow));
+		} else {
+			fp[0] = make_uint4(0, 0, 0, 0);
+			fp[1] = make_uint4(0, 0, 0, 0);
+		}
 		if (row_uniform) row_stamp[v0 / W] = next_level;
 		F = (uint32_t)__popc(bits);
 		// the word's 32 degrees in one 128-byte line (deg is padded to n_pad)
@@ -1076,16 +1079,16 @@ static int frontier_loop(gphi_graph *gr, uint32_t src, gphi_stats *st)
 	if (kBfs) {
 		GPHI_HIP(ctx, hipMemsetAsync(gr->level, 0xFF, nb, s));
 		GPHI_HIP(ctx, hipMemsetAsync(gr->parent, 0xFF, nb, s));
-		GPHI_HIP(ctx, hipMemsetAsync(gr->visited, 0, (size_t)G.n_words * 4, s));
 	} else {
 		fill_u32_kernel<<<grid_for(G.n_pad, kBlock, 4096), kBlock, 0, s>>>(gr->dist, G.n_pad, INT_MAX);
 	}
+	GPHI_HIP(ctx, hipMemsetAsync(gr->flag, 0, G.n_pad, s));
 	GPHI_HIP(ctx, hipMemsetAsync(gr->cur, 0, (size_t)G.n_words * 4, s));
 	GPHI_HIP(ctx, hipMemsetAsync(gr->live_cnt, 0, kNumXcd * 4, s));
 	init_blocks_kernel<<<grid_for(G.side, kBlock, 65535), kBlock, 0, s>>>(G.n, G.W, G.side,
 	                                                                      gr->row_stamp, gr->col_unvis);
 	init_source_kernel<kBfs><<<1, kBlock, 0, s>>>(src, G.W, G.deg, gr->level, gr->parent, gr->dist,
-	                                               gr->visited, gr->cur, gr->row_stamp, gr->col_unvis,
+	                                               gr->flag, gr->cur, gr->row_stamp, gr->col_unvis,
 	                                               gr->queue, gr->qoff, gr->tile_lo, gr->ctrl, gr->trace);
 	GPHI_HIP(ctx, hipGetLastError());
 	if (timing) GPHI_HIP(ctx, hipEventRecord(ev[1], s));
@@ -1107,20 +1110,20 @@ static int frontier_loop(gphi_graph *gr, uint32_t src, gphi_stats *st)
 		for (uint32_t q = 0; q < kLevelGroup; ++q, ++enq) {
 			const bool ev_step = timing && enq < GPHI_MAX_STEPS - 1;
 			sparse_push_kernel<kBfs><<<kSparseGrid, kBlock, 0, s>>>(gr->queue, gr->qoff, gr->tile_lo, gr->ctrl,
-			                                                     G.row_off, G.cols, G.csr_w, gr->visited,
+			                                                     G.row_off, G.cols, G.csr_w,
 			                                                     gr->flag, gr->parent, gr->dist);
 			chunk_select_kernel<kBfs><<<sgrid, kBlock, 0, s>>>(G.meta, G.nchunks, gr->row_stamp, gr->col_unvis,
 			                                                 gr->ctrl, gr->live, list_cap, gr->live_cnt);
 			if (ev_step) GPHI_HIP(ctx, hipEventRecord(ev[2 * GPHI_MAX_STEPS + 4 + enq], s));
 			if (kBfs)
 				bfs_dense_kernel<<<grid, kBlock, 0, s>>>(G.heads, G.tails, gr->live, list_cap, gr->live_cnt,
-				                                       gr->cur, gr->visited, gr->flag, gr->parent);
+				                                       gr->cur, gr->flag, gr->parent);
 			else
 				sssp_dense_kernel<<<grid, kBlock, 0, s>>>(G.heads, G.tails, G.coo_w, gr->live, list_cap,
 				                                        gr->live_cnt, gr->cur, gr->dist, gr->flag);
 			if (ev_step) GPHI_HIP(ctx, hipEventRecord(ev[2 + 2 * enq], s));
 			frontier_mark_kernel<kBfs><<<nblk, kBlock, 0, s>>>(G.W, kLevelFromCtrl, gr->flag, G.deg, gr->level,
-			                                                  gr->visited, gr->cur, gr->row_stamp, gr->col_unvis,
+			                                                  gr->cur, gr->row_stamp, gr->col_unvis,
 			                                                  gr->blk_cnt, gr->ctrl);
 			frontier_scan_kernel<<<1, kScanThreads, 0, s>>>(nblk, threshold, gr->blk_cnt, gr->blk_base,
 			                                              gr->live_cnt, gr->ctrl, gr->trace);
@@ -1189,7 +1192,7 @@ void launch_frontier_update_flags(gphi_graph *gr, uint32_t next_level, uint32_t 
 	hipStream_t s = gr->ctx->stream;
 	const unsigned nblk = G.n_pad / kPadVerts;
 	frontier_mark_kernel<false><<<nblk, kBlock, 0, s>>>(G.W, next_level, gr->flag, G.deg, gr->level,
-	                                                   gr->visited, gr->cur, gr->row_stamp, gr->col_unvis,
+	                                                   gr->cur, gr->row_stamp, gr->col_unvis,
 	                                                   gr->blk_cnt, gr->ctrl);
 	frontier_scan_kernel<<<1, kScanThreads, 0, s>>>(nblk, threshold, gr->blk_cnt, gr->blk_base, gr->live_cnt,
 	                                              gr->ctrl, nullptr);

@@ -21,6 +21,7 @@ constexpr uint32_t kSparseTile = 1024;  // edges per block iteration of the CSR 
 constexpr uint32_t kPadVerts = kBlock * 32;  // vertices per frontier-update block
 constexpr uint32_t kPullLight = 64;     // pull steps: a lane per vertex up to this in-degree,
 constexpr uint32_t kPullPiece = 4096;    // else a wavefront per piece of this many in-edges
+constexpr uint8_t kVisited = 2;          // state byte of a vertex reached in an earlier step
 constexpr int kNumXcd = 8;              // MI355X: 8 XCDs, blocks dispatched round-robin
 
 struct Error {
@@ -104,8 +105,10 @@ struct gphi_graph {
 	int32_t *level = nullptr;  // levels (BFS) ; dist (SSSP/BF) aliases via uint32
 	int32_t *parent = nullptr;
 	uint32_t *dist = nullptr;
+	// per-vertex state byte: 0 not reached, 1 reached in the current step,
+	// kVisited (BFS) reached in an earlier step; SSSP and multi-source use 0/1
 	uint8_t *flag = nullptr;
-	uint32_t *visited = nullptr, *cur = nullptr;
+	uint32_t *cur = nullptr;  // frontier bitmap of the current step
 	uint32_t *row_stamp = nullptr, *col_unvis = nullptr;
 	uint32_t *queue = nullptr, *qoff = nullptr, *tile_lo = nullptr;
 	gphi::StepCtrl *ctrl = nullptr;
