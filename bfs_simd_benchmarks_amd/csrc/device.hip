@@ -532,54 +532,136 @@ __device__ __forceinline__ uint4 ld_stream(const uint4 *p)
 	return make_uint4(v.x, v.y, v.z, v.w);
 }
 
-// Software-pipelined walk of this XCD's live-chunk list, 8 edges per lane per
-// chunk.  While chunk k finishes (state bytes of its tails, writes), the
-// frontier bits of chunk k+1 are looked up, and then its tails and the heads
-// of chunk k+2 are requested, so each iteration waits on one HBM latency and
-// one L2 round trip.  Every load is unconditional (a lane with nothing to
-// fetch reads a cached dummy word) so the compiler can count them.  Tails of
-// a 4-edge quad are fetched only when one of its heads is in the frontier.
+// Frontier-bit lookups of the dense sweep go through LDS.  A chunk's heads
+// lie in its tile-row range [row_lo, row_hi] (a column-stripe chunk stays in
+// one stripe), so the frontier bitmap of an aligned run of R tile rows
+// (kWinWords words) is copied into LDS once and serves every chunk of the
+// block inside it; a chunk outside any such run looks its bits up in global
+// memory.  Gathers cost address-unit cycles per lane, LDS reads do not.
+constexpr uint32_t kWinWords = 2048;
+constexpr uint32_t kDenseRun = 8;  // consecutive live-list entries per block run
+
+// tile rows per window for tile width W (0: no window fits)
+static uint32_t dense_window_rows(uint32_t W)
+{
+	if (W == 0) return 0;
+	if ((W & (W - 1)) == 0 && W <= kWinWords * 32) return kWinWords * 32 / W;
+	return W <= (kWinWords - 4) * 32 ? (kWinWords - 4) * 32 / W : 0;
+}
+
+// window [wlo, wlo + wlen) of bitmap words holding tile rows row_lo..row_hi,
+// or false; wlo is a multiple of 4 and wlen <= kWinWords
+__device__ __forceinline__ bool window_of(uint32_t row_lo, uint32_t row_hi, uint32_t R, uint32_t W,
+                                          uint32_t n_words, uint32_t &wlo, uint32_t &wlen)
+{
+	if (R == 0) return false;
+	const uint32_t r0 = row_lo - row_lo % R;
+	if (row_hi - r0 >= R) return false;
+	const uint64_t b0 = (uint64_t)r0 * W, b1 = (uint64_t)(r0 + R) * W;
+	wlo = (uint32_t)(b0 >> 5) & ~3u;
+	const uint32_t whi = (uint32_t)min((b1 + 31) >> 5, (uint64_t)n_words);
+	wlen = whi - wlo;
+	return true;
+}
+
+// block-wide copy of bitmap words [wlo, wlo + wlen) into LDS (n_words is a
+// multiple of 4, so the rounded-up last uint4 stays inside the bitmap)
+__device__ __forceinline__ void load_window(uint4 *s_win, const uint32_t *cur, uint32_t wlo, uint32_t wlen,
+                                            uint32_t tid)
+{
+	const uint4 *src = reinterpret_cast<const uint4 *>(cur + wlo);
+	const uint32_t nq = (wlen + 3) / 4;
+	// one uint4 in flight per lane: window changes are rare, registers are not
+#pragma nounroll
+	for (uint32_t q = tid; q < nq; q += kBlock) s_win[q] = src[q];
+}
+
+// frontier bits of 8 heads; in the window every real head of the chunk is
+// inside it, a padding head (id n) outside it reads as not in the frontier
+__device__ __forceinline__ uint32_t frontier_bits(const uint32_t (&hh)[8], bool inw, const uint32_t *s_win,
+                                                  uint32_t wlo, uint32_t wlen, const uint32_t *cur)
+{
+	uint32_t cw[8];
+	if (inw) {
+#pragma unroll
+		for (int j = 0; j < 8; ++j) {
+			const uint32_t wi = (hh[j] >> 5) - wlo;
+			cw[j] = wi < wlen ? s_win[wi] : 0u;
+		}
+	} else {
+#pragma unroll
+		for (int j = 0; j < 8; ++j) cw[j] = cur[hh[j] >> 5];
+	}
+	uint32_t hit = 0;
+#pragma unroll
+	for (int j = 0; j < 8; ++j) hit |= ((cw[j] >> (hh[j] & 31)) & 1u) << j;
+	return hit;
+}
+
+// Software-pipelined walk of this XCD's live-chunk list (runs of kDenseRun
+// entries per block), 8 edges per lane per chunk.  While chunk k finishes (state bytes of
+// its tails, writes), the frontier bits of chunk k+1 are looked up, and then
+// its tails and the heads of chunk k+2 are requested, so each iteration waits
+// on one HBM latency and one L2 round trip.  Every global load is
+// unconditional (a lane with nothing to fetch reads a cached dummy word) so
+// the compiler can count them.  Tails of a 4-edge quad are fetched only when
+// one of its heads is in the frontier.
 // One byte per vertex says whether a tail is new (state 0): "visited before
 // this step" and "reached in this step" live in the same byte, so a
-// candidate costs one lookup.  Every lane that sees state 0 writes parent and state 1
-// (benign races: every writer stores a valid frontier head and the same
-// state; an earlier step's kVisited is never overwritten, since no lane of
-// this step reads state 0 for such a vertex).
+// candidate costs one lookup.  Every lane that sees state 0 writes parent and
+// state 1 (benign races: every writer stores a valid frontier head and the
+// same state; an earlier step's kVisited is never overwritten, since no lane
+// of this step reads state 0 for such a vertex).
 __device__ __forceinline__ void unpack8(const uint4 a, const uint4 b, uint32_t (&x)[8])
 {
 	x[0] = a.x, x[1] = a.y, x[2] = a.z, x[3] = a.w, x[4] = b.x, x[5] = b.y, x[6] = b.z, x[7] = b.w;
 }
 
-__global__ __launch_bounds__(kBlock) void bfs_dense_kernel(
-    const uint32_t *__restrict__ heads, const uint32_t *__restrict__ tails,
+__global__ __launch_bounds__(kBlock, 8) void bfs_dense_kernel(
+    const uint32_t *__restrict__ heads, const uint32_t *__restrict__ tails, const ChunkMeta *__restrict__ meta,
     const uint32_t *__restrict__ lists, uint32_t list_cap, const uint32_t *__restrict__ counts,
-    const uint32_t *__restrict__ cur, uint8_t *__restrict__ state, int32_t *__restrict__ parent)
+    const uint32_t *__restrict__ cur, uint32_t n_words, uint32_t W, uint32_t R, uint8_t *__restrict__ state,
+    int32_t *__restrict__ parent)
 {
-	const uint32_t xcd = blockIdx.x % kNumXcd, per = gridDim.x / kNumXcd;
-	const uint32_t cnt = counts[xcd];
+	__shared__ uint4 s_win4[kWinWords / 4];
+	const uint32_t *s_win = reinterpret_cast<const uint32_t *>(s_win4);
+	const uint32_t xcd = blockIdx.x % kNumXcd, nb = gridDim.x / kNumXcd;
+	const uint32_t end = counts[xcd];
+	// runs of kDenseRun consecutive list entries, run r of the XCD to block
+	// r % nb: chunks of a run share a window, while the blocks of an XCD still
+	// sweep one narrow region of the edge arrays at a time (much longer runs
+	// scatter the blocks over the arrays and are slower)
+	const uint32_t skip = (nb - 1) * kDenseRun;
+	auto nxt = [&](uint32_t p) { return ((p + 1) % kDenseRun) ? p + 1 : p + 1 + skip; };
+	uint32_t pos = (blockIdx.x / kNumXcd) * kDenseRun;
+	if (pos >= end) return;
 	const uint32_t *lst = lists + (uint64_t)xcd * list_cap;
-	uint32_t pos = blockIdx.x / kNumXcd;
-	if (pos >= cnt) return;
 	const uint32_t tid = threadIdx.x;
+	uint32_t win_lo = UINT32_MAX;  // first word of the window held in LDS (block-uniform)
 	// list entries: chunks k, k+1, k+2 (an absent one repeats chunk k's id, so
 	// every address stays valid; its results are masked off)
 	const uint32_t c0 = lst[pos];
-	bool v1 = pos + per < cnt, v2 = pos + 2 * per < cnt;
-	uint32_t c1 = v1 ? lst[pos + per] : c0;
-	uint32_t c2 = v2 ? lst[pos + 2 * per] : c0;
+	uint32_t p1 = nxt(pos), p2 = nxt(p1);
+	bool v1 = p1 < end, v2 = p2 < end;
+	uint32_t c1 = v1 ? lst[p1] : c0;
+	uint32_t c2 = v2 ? lst[p2] : c0;
 	// prologue: heads and frontier bits of chunk k, then heads of k+1 and tails of k
 	uint32_t h[8], hn[8], t[8];
 	{
 		const uint4 *H = reinterpret_cast<const uint4 *>(heads + (uint64_t)c0 * kChunk);
 		unpack8(ld_stream(H + tid), ld_stream(H + tid + kBlock), h);
 	}
-	uint32_t hit = 0;
+	uint32_t hit;
 	{
-		uint32_t cw[8];
-#pragma unroll
-		for (int j = 0; j < 8; ++j) cw[j] = cur[h[j] >> 5];
-#pragma unroll
-		for (int j = 0; j < 8; ++j) hit |= ((cw[j] >> (h[j] & 31)) & 1u) << j;
+		const ChunkMeta m0 = meta[c0];
+		uint32_t wlo = 0, wlen = 0;
+		const bool inw = window_of(m0.row_lo, m0.row_hi, R, W, n_words, wlo, wlen);
+		if (inw) {
+			load_window(s_win4, cur, wlo, wlen, tid);
+			__syncthreads();
+			win_lo = wlo;
+		}
+		hit = frontier_bits(h, inw, s_win, wlo, wlen, cur);
 	}
 	{
 		const uint4 *H = reinterpret_cast<const uint4 *>(heads + (uint64_t)c1 * kChunk);
@@ -589,19 +671,28 @@ __global__ __launch_bounds__(kBlock) void bfs_dense_kernel(
 		unpack8(ld_stream((hit & 0x0Fu) ? T + tid : D), ld_stream((hit & 0xF0u) ? T + tid + kBlock : D), t);
 	}
 	for (;;) {
+		// window of chunk k+1 (a block-uniform branch)
+		bool inw_n = false;
+		uint32_t wlo_n = 0, wlen_n = 0;
+		if (v1) {
+			const ChunkMeta mn = meta[c1];
+			inw_n = window_of(mn.row_lo, mn.row_hi, R, W, n_words, wlo_n, wlen_n);
+			if (inw_n && wlo_n != win_lo) {
+				__syncthreads();  // every wave is done with the old window
+				load_window(s_win4, cur, wlo_n, wlen_n, tid);
+				__syncthreads();
+				win_lo = wlo_n;
+			}
+		}
 		// stage A: states of chunk k's frontier-edge tails, frontier bits of chunk k+1
-		uint32_t sv[8], cw[8];
+		uint32_t sv[8];
 #pragma unroll
 		for (int j = 0; j < 8; ++j) sv[j] = state[((hit >> j) & 1u) ? t[j] : 0u];
-#pragma unroll
-		for (int j = 0; j < 8; ++j) cw[j] = cur[hn[j] >> 5];
-		uint32_t cand = 0, hit_n = 0;
+		const uint32_t hit_n = v1 ? frontier_bits(hn, inw_n, s_win, wlo_n, wlen_n, cur) : 0u;
+		uint32_t cand = 0;
 #pragma unroll
 		for (int j = 0; j < 8; ++j) cand |= (sv[j] == 0 ? 1u : 0u) << j;
 		cand &= hit;
-#pragma unroll
-		for (int j = 0; j < 8; ++j) hit_n |= ((cw[j] >> (hn[j] & 31)) & 1u) << j;
-		if (!v1) hit_n = 0;
 		// stage B: the HBM requests of the next iteration: tails of k+1, heads of k+2
 		uint32_t tn[8], h2[8];
 		{
@@ -612,8 +703,9 @@ __global__ __launch_bounds__(kBlock) void bfs_dense_kernel(
 			const uint4 *H = reinterpret_cast<const uint4 *>(heads + (uint64_t)c2 * kChunk);
 			unpack8(ld_stream(H + tid), ld_stream(H + tid + kBlock), h2);
 		}
-		const bool v3 = pos + 3 * per < cnt;
-		const uint32_t c3 = v3 ? lst[pos + 3 * per] : c0;
+		const uint32_t p3 = nxt(p2);
+		const bool v3 = p3 < end;
+		const uint32_t c3 = v3 ? lst[p3] : c0;
 		// stage C: chunk k's writes
 #pragma unroll
 		for (int j = 0; j < 8; ++j)
@@ -622,7 +714,6 @@ __global__ __launch_bounds__(kBlock) void bfs_dense_kernel(
 				state[t[j]] = 1;
 			}
 		if (!v1) break;
-		pos += per;
 #pragma unroll
 		for (int j = 0; j < 8; ++j) {
 			h[j] = hn[j];
@@ -634,6 +725,7 @@ __global__ __launch_bounds__(kBlock) void bfs_dense_kernel(
 		c2 = c3;
 		v1 = v2;
 		v2 = v3;
+		p2 = p3;
 	}
 }
 static_assert(kChunk == 2 * 4 * kBlock, "dense sweep loads two uint4 per array per thread");
@@ -1105,6 +1197,7 @@ static int frontier_loop(gphi_graph *gr, uint32_t src, gphi_stats *st)
 	const unsigned sgrid = (unsigned)std::max<uint64_t>(1, (G.nchunks + kBlock - 1) / kBlock);
 	unsigned grid = (unsigned)std::min<uint64_t>(kBfsDenseGrid, (G.nchunks + kNumXcd - 1) / kNumXcd * kNumXcd);
 	grid = std::max<unsigned>(grid / kNumXcd * kNumXcd, kNumXcd);
+	const uint32_t win_rows = dense_window_rows(G.W);
 	uint32_t enq = 0;
 	for (;;) {
 		for (uint32_t q = 0; q < kLevelGroup; ++q, ++enq) {
@@ -1116,8 +1209,8 @@ static int frontier_loop(gphi_graph *gr, uint32_t src, gphi_stats *st)
 			                                                 gr->ctrl, gr->live, list_cap, gr->live_cnt);
 			if (ev_step) GPHI_HIP(ctx, hipEventRecord(ev[2 * GPHI_MAX_STEPS + 4 + enq], s));
 			if (kBfs)
-				bfs_dense_kernel<<<grid, kBlock, 0, s>>>(G.heads, G.tails, gr->live, list_cap, gr->live_cnt,
-				                                       gr->cur, gr->flag, gr->parent);
+				bfs_dense_kernel<<<grid, kBlock, 0, s>>>(G.heads, G.tails, G.meta, gr->live, list_cap, gr->live_cnt,
+				                                       gr->cur, G.n_words, G.W, win_rows, gr->flag, gr->parent);
 			else
 				sssp_dense_kernel<<<grid, kBlock, 0, s>>>(G.heads, G.tails, G.coo_w, gr->live, list_cap,
 				                                        gr->live_cnt, gr->cur, gr->dist, gr->flag);

@@ -48,7 +48,7 @@ def test_bfs_c1_sources(ctx, c1, src):
     check_bfs(ctx, c1, src)
 
 
-@pytest.mark.parametrize("W,S", [(256, 4), (64, 1), (4096, 16), (1000, 3), (65536, 1)])
+@pytest.mark.parametrize("W,S", [(256, 4), (64, 1), (4096, 16), (1000, 3), (65536, 1), (131072, 1)])
 def test_bfs_c1_tilings(ctx, W, S):
     print(f"case: bfs c1 W={W} S={S}", flush=True)
     n, h, t = O.config_edges("c1")
