@@ -331,7 +331,7 @@ int upload_graph(gphi_ctx *ctx, const gphi_graph_desc *d, gphi_graph *gr)
 void free_traversal_state(gphi_graph *gr)
 {
 	void **ps[] = {(void **)&gr->level, (void **)&gr->parent, (void **)&gr->dist, (void **)&gr->flag,
-	               (void **)&gr->cur, (void **)&gr->row_stamp, (void **)&gr->col_unvis,
+	               (void **)&gr->visited, (void **)&gr->cur, (void **)&gr->row_stamp, (void **)&gr->col_unvis,
 	               (void **)&gr->queue, (void **)&gr->qoff, (void **)&gr->tile_lo, (void **)&gr->ctrl,
 	               (void **)&gr->blk_cnt, (void **)&gr->blk_base, (void **)&gr->live, (void **)&gr->live_cnt,
 	               (void **)&gr->trace,
@@ -356,7 +356,7 @@ void free_graph_device(gphi_graph *gr)
 		return;
 	}
 	void *ps[] = {G.row_off, G.cols, G.csr_w, G.deg, G.heads, G.tails, G.coo_w, G.meta,
-	              gr->level, gr->parent, gr->dist, gr->flag, gr->cur,
+	              gr->level, gr->parent, gr->dist, gr->flag, gr->visited, gr->cur,
 	              gr->row_stamp, gr->col_unvis, gr->queue, gr->qoff, gr->ctrl, gr->rank,
 	              gr->sum, gr->contrib, gr->tile_lo, gr->blk_cnt, gr->blk_base, gr->live, gr->live_cnt, gr->trace,
 	              gr->b_front, gr->b_seen, gr->b_next, gr->b_level, gr->b_sums, gr->b_live,
@@ -386,6 +386,7 @@ static int ensure_state(gphi_graph *gr)
 	if ((rc = dmalloc(ctx, (void **)&gr->level, nb, nullptr))) return rc;
 	if ((rc = dmalloc(ctx, (void **)&gr->parent, nb, nullptr))) return rc;
 	if ((rc = dmalloc(ctx, (void **)&gr->flag, G.n_pad, nullptr))) return rc;
+	if ((rc = dmalloc(ctx, (void **)&gr->visited, (size_t)G.n_words * 4, nullptr))) return rc;
 	if ((rc = dmalloc(ctx, (void **)&gr->cur, (size_t)G.n_words * 4, nullptr))) return rc;
 	if ((rc = dmalloc(ctx, (void **)&gr->row_stamp, ((size_t)G.side + 1) * 4, nullptr))) return rc;
 	if ((rc = dmalloc(ctx, (void **)&gr->col_unvis, ((size_t)G.side + 1) * 4, nullptr))) return rc;
@@ -422,7 +423,7 @@ __global__ void init_blocks_kernel(uint32_t n, uint32_t W, uint32_t side, uint32
 template <bool kBfs>
 __global__ void init_source_kernel(uint32_t src, uint32_t W, const uint32_t *__restrict__ deg,
                                    int32_t *level, int32_t *parent, uint32_t *dist,
-                                   uint8_t *flag, uint32_t *cur, uint32_t *row_stamp,
+                                   uint8_t *flag, uint32_t *visited, uint32_t *cur, uint32_t *row_stamp,
                                    uint32_t *col_unvis, uint32_t *queue, uint32_t *qoff,
                                    uint32_t *tile_lo, StepCtrl *ctrl, uint4 *trace)
 {
@@ -434,6 +435,7 @@ __global__ void init_source_kernel(uint32_t src, uint32_t W, const uint32_t *__r
 			level[src] = 0;
 			parent[src] = (int32_t)src;
 			flag[src] = kVisited;
+			visited[src >> 5] |= 1u << (src & 31);
 			col_unvis[src / W] -= 1;
 		} else {
 			dist[src] = 0;
@@ -620,11 +622,13 @@ __device__ __forceinline__ void unpack8(const uint4 a, const uint4 b, uint32_t (
 __global__ __launch_bounds__(kBlock, 8) void bfs_dense_kernel(
     const uint32_t *__restrict__ heads, const uint32_t *__restrict__ tails, const ChunkMeta *__restrict__ meta,
     const uint32_t *__restrict__ lists, uint32_t list_cap, const uint32_t *__restrict__ counts,
-    const uint32_t *__restrict__ cur, uint32_t n_words, uint32_t W, uint32_t R, uint8_t *__restrict__ state,
-    int32_t *__restrict__ parent)
+    const uint32_t *__restrict__ cur, const uint32_t *__restrict__ visited, uint32_t n_words, uint32_t W,
+    uint32_t R, uint8_t *__restrict__ state, int32_t *__restrict__ parent)
 {
-	__shared__ uint4 s_win4[kWinWords / 4];
+	__shared__ uint4 s_win4[kWinWords / 4];   // frontier bits of chunk k+1's rows
+	__shared__ uint4 s_vwin4[kWinWords / 4];  // visited bits of chunk k's columns
 	const uint32_t *s_win = reinterpret_cast<const uint32_t *>(s_win4);
+	const uint32_t *s_vwin = reinterpret_cast<const uint32_t *>(s_vwin4);
 	const uint32_t xcd = blockIdx.x % kNumXcd, nb = gridDim.x / kNumXcd;
 	const uint32_t end = counts[xcd];
 	// runs of kDenseRun consecutive list entries, run r of the XCD to block
@@ -637,7 +641,8 @@ __global__ __launch_bounds__(kBlock, 8) void bfs_dense_kernel(
 	if (pos >= end) return;
 	const uint32_t *lst = lists + (uint64_t)xcd * list_cap;
 	const uint32_t tid = threadIdx.x;
-	uint32_t win_lo = UINT32_MAX;  // first word of the window held in LDS (block-uniform)
+	// first words of the windows held in LDS (block-uniform)
+	uint32_t win_lo = UINT32_MAX, vwin_lo = UINT32_MAX;
 	// list entries: chunks k, k+1, k+2 (an absent one repeats chunk k's id, so
 	// every address stays valid; its results are masked off)
 	const uint32_t c0 = lst[pos];
@@ -670,30 +675,59 @@ __global__ __launch_bounds__(kBlock, 8) void bfs_dense_kernel(
 		const uint4 *D = reinterpret_cast<const uint4 *>(tails);  // dummy: first 16 bytes
 		unpack8(ld_stream((hit & 0x0Fu) ? T + tid : D), ld_stream((hit & 0xF0u) ? T + tid + kBlock : D), t);
 	}
+	uint32_t ck = c0;  // chunk k
 	for (;;) {
-		// window of chunk k+1 (a block-uniform branch)
+		// windows (block-uniform): frontier rows of chunk k+1, visited columns of chunk k
 		bool inw_n = false;
-		uint32_t wlo_n = 0, wlen_n = 0;
+		uint32_t wlo_n = 0, wlen_n = 0, vwlo = 0, vwlen = 0;
 		if (v1) {
 			const ChunkMeta mn = meta[c1];
 			inw_n = window_of(mn.row_lo, mn.row_hi, R, W, n_words, wlo_n, wlen_n);
-			if (inw_n && wlo_n != win_lo) {
-				__syncthreads();  // every wave is done with the old window
+		}
+		const ChunkMeta mk = meta[ck];
+		const bool vin = window_of(mk.col_lo, mk.col_hi, R, W, n_words, vwlo, vwlen);
+		const bool ld_c = inw_n && wlo_n != win_lo, ld_v = vin && vwlo != vwin_lo;
+		if (ld_c || ld_v) {
+			__syncthreads();  // every wave is done with the old windows
+			if (ld_c) {
 				load_window(s_win4, cur, wlo_n, wlen_n, tid);
-				__syncthreads();
 				win_lo = wlo_n;
 			}
+			if (ld_v) {
+				load_window(s_vwin4, visited, vwlo, vwlen, tid);
+				vwin_lo = vwlo;
+			}
+			__syncthreads();
 		}
-		// stage A: states of chunk k's frontier-edge tails, frontier bits of chunk k+1
+		// stage A: states of chunk k's new frontier-edge tails, frontier bits of
+		// chunk k+1.  A tail visited in an earlier step (its bit in the window)
+		// needs no state lookup.
+		uint32_t need = hit;
+		if (vin) {
+#pragma unroll
+			for (int j = 0; j < 8; ++j) {
+				const uint32_t wi = (t[j] >> 5) - vwlo;
+				const uint32_t vb = wi < vwlen ? (s_vwin[wi] >> (t[j] & 31)) & 1u : 0u;
+				need &= ~(vb << j);
+			}
+		}
 		uint32_t sv[8];
 #pragma unroll
-		for (int j = 0; j < 8; ++j) sv[j] = state[((hit >> j) & 1u) ? t[j] : 0u];
+		for (int j = 0; j < 8; ++j) sv[j] = state[((need >> j) & 1u) ? t[j] : 0u];
 		const uint32_t hit_n = v1 ? frontier_bits(hn, inw_n, s_win, wlo_n, wlen_n, cur) : 0u;
 		uint32_t cand = 0;
 #pragma unroll
 		for (int j = 0; j < 8; ++j) cand |= (sv[j] == 0 ? 1u : 0u) << j;
-		cand &= hit;
-		// stage B: the HBM requests of the next iteration: tails of k+1, heads of k+2
+		cand &= need;
+		// stage B: chunk k's writes (few; issued first, so chunk k's heads and
+		// tails are dead before the next requests take their registers)
+#pragma unroll
+		for (int j = 0; j < 8; ++j)
+			if ((cand >> j) & 1u) {
+				parent[t[j]] = (int32_t)h[j];
+				state[t[j]] = 1;
+			}
+		// stage C: the HBM requests of the next iteration: tails of k+1, heads of k+2
 		uint32_t tn[8], h2[8];
 		{
 			const uint4 *T = reinterpret_cast<const uint4 *>(tails + (uint64_t)c1 * kChunk);
@@ -706,13 +740,6 @@ __global__ __launch_bounds__(kBlock, 8) void bfs_dense_kernel(
 		const uint32_t p3 = nxt(p2);
 		const bool v3 = p3 < end;
 		const uint32_t c3 = v3 ? lst[p3] : c0;
-		// stage C: chunk k's writes
-#pragma unroll
-		for (int j = 0; j < 8; ++j)
-			if ((cand >> j) & 1u) {
-				parent[t[j]] = (int32_t)h[j];
-				state[t[j]] = 1;
-			}
 		if (!v1) break;
 #pragma unroll
 		for (int j = 0; j < 8; ++j) {
@@ -721,6 +748,7 @@ __global__ __launch_bounds__(kBlock, 8) void bfs_dense_kernel(
 			hn[j] = h2[j];
 		}
 		hit = hit_n;
+		ck = c1;
 		c1 = c2;
 		c2 = c3;
 		v1 = v2;
@@ -916,7 +944,7 @@ constexpr uint32_t kLevelFromCtrl = UINT32_MAX;  // mark kernel: take the level 
 template <bool kBfs>
 __global__ __launch_bounds__(kBlock) void frontier_mark_kernel(
     uint32_t W, uint32_t next_level, uint8_t *__restrict__ flag, const uint32_t *__restrict__ deg,
-    int32_t *__restrict__ level, uint32_t *__restrict__ cur,
+    int32_t *__restrict__ level, uint32_t *__restrict__ visited, uint32_t *__restrict__ cur,
     uint32_t *__restrict__ row_stamp, uint32_t *__restrict__ col_unvis, uint4 *__restrict__ blk_cnt,
     const StepCtrl *__restrict__ ctrl)
 {
@@ -949,6 +977,7 @@ __global__ __launch_bounds__(kBlock) void frontier_mark_kernel(
 			fp[0] = make_uint4(0, 0, 0, 0);
 			fp[1] = make_uint4(0, 0, 0, 0);
 		}
+		if (kBfs) visited[w] |= bits;
 		if (row_uniform) row_stamp[v0 / W] = next_level;
 		F = (uint32_t)__popc(bits);
 		// the word's 32 degrees in one 128-byte line (deg is padded to n_pad)
@@ -1175,12 +1204,13 @@ static int frontier_loop(gphi_graph *gr, uint32_t src, gphi_stats *st)
 		fill_u32_kernel<<<grid_for(G.n_pad, kBlock, 4096), kBlock, 0, s>>>(gr->dist, G.n_pad, INT_MAX);
 	}
 	GPHI_HIP(ctx, hipMemsetAsync(gr->flag, 0, G.n_pad, s));
+	if (kBfs) GPHI_HIP(ctx, hipMemsetAsync(gr->visited, 0, (size_t)G.n_words * 4, s));
 	GPHI_HIP(ctx, hipMemsetAsync(gr->cur, 0, (size_t)G.n_words * 4, s));
 	GPHI_HIP(ctx, hipMemsetAsync(gr->live_cnt, 0, kNumXcd * 4, s));
 	init_blocks_kernel<<<grid_for(G.side, kBlock, 65535), kBlock, 0, s>>>(G.n, G.W, G.side,
 	                                                                      gr->row_stamp, gr->col_unvis);
 	init_source_kernel<kBfs><<<1, kBlock, 0, s>>>(src, G.W, G.deg, gr->level, gr->parent, gr->dist,
-	                                               gr->flag, gr->cur, gr->row_stamp, gr->col_unvis,
+	                                               gr->flag, gr->visited, gr->cur, gr->row_stamp, gr->col_unvis,
 	                                               gr->queue, gr->qoff, gr->tile_lo, gr->ctrl, gr->trace);
 	GPHI_HIP(ctx, hipGetLastError());
 	if (timing) GPHI_HIP(ctx, hipEventRecord(ev[1], s));
@@ -1210,13 +1240,14 @@ static int frontier_loop(gphi_graph *gr, uint32_t src, gphi_stats *st)
 			if (ev_step) GPHI_HIP(ctx, hipEventRecord(ev[2 * GPHI_MAX_STEPS + 4 + enq], s));
 			if (kBfs)
 				bfs_dense_kernel<<<grid, kBlock, 0, s>>>(G.heads, G.tails, G.meta, gr->live, list_cap, gr->live_cnt,
-				                                       gr->cur, G.n_words, G.W, win_rows, gr->flag, gr->parent);
+				                                       gr->cur, gr->visited, G.n_words, G.W, win_rows, gr->flag,
+				                                       gr->parent);
 			else
 				sssp_dense_kernel<<<grid, kBlock, 0, s>>>(G.heads, G.tails, G.coo_w, gr->live, list_cap,
 				                                        gr->live_cnt, gr->cur, gr->dist, gr->flag);
 			if (ev_step) GPHI_HIP(ctx, hipEventRecord(ev[2 + 2 * enq], s));
 			frontier_mark_kernel<kBfs><<<nblk, kBlock, 0, s>>>(G.W, kLevelFromCtrl, gr->flag, G.deg, gr->level,
-			                                                  gr->cur, gr->row_stamp, gr->col_unvis,
+			                                                  gr->visited, gr->cur, gr->row_stamp, gr->col_unvis,
 			                                                  gr->blk_cnt, gr->ctrl);
 			frontier_scan_kernel<<<1, kScanThreads, 0, s>>>(nblk, threshold, gr->blk_cnt, gr->blk_base,
 			                                              gr->live_cnt, gr->ctrl, gr->trace);
@@ -1285,7 +1316,7 @@ void launch_frontier_update_flags(gphi_graph *gr, uint32_t next_level, uint32_t 
 	hipStream_t s = gr->ctx->stream;
 	const unsigned nblk = G.n_pad / kPadVerts;
 	frontier_mark_kernel<false><<<nblk, kBlock, 0, s>>>(G.W, next_level, gr->flag, G.deg, gr->level,
-	                                                   gr->cur, gr->row_stamp, gr->col_unvis,
+	                                                   gr->visited, gr->cur, gr->row_stamp, gr->col_unvis,
 	                                                   gr->blk_cnt, gr->ctrl);
 	frontier_scan_kernel<<<1, kScanThreads, 0, s>>>(nblk, threshold, gr->blk_cnt, gr->blk_base, gr->live_cnt,
 	                                              gr->ctrl, nullptr);

@@ -108,7 +108,8 @@ struct gphi_graph {
 	// per-vertex state byte: 0 not reached, 1 reached in the current step,
 	// kVisited (BFS) reached in an earlier step; SSSP and multi-source use 0/1
 	uint8_t *flag = nullptr;
-	uint32_t *cur = nullptr;  // frontier bitmap of the current step
+	uint32_t *visited = nullptr;  // BFS: bitmap of the vertices reached in earlier steps
+	uint32_t *cur = nullptr;      // frontier bitmap of the current step
 	uint32_t *row_stamp = nullptr, *col_unvis = nullptr;
 	uint32_t *queue = nullptr, *qoff = nullptr, *tile_lo = nullptr;
 	gphi::StepCtrl *ctrl = nullptr;
