@@ -126,8 +126,72 @@ static unsigned grid_for(uint64_t items, unsigned per_block, unsigned cap)
 	return (unsigned)g;
 }
 
+// A window: an aligned run of R tile rows (or columns) whose R*W vertices fit
+// kWinWords bitmap words, i.e. 16-bit offsets (R*W <= 65536).
+constexpr uint32_t kWinWords = 2048;
+
+// tile rows per window for tile width W (0: no window fits)
+static uint32_t dense_window_rows(uint32_t W)
+{
+	if (W == 0) return 0;
+	if ((W & (W - 1)) == 0 && W <= kWinWords * 32) return kWinWords * 32 / W;
+	return W <= (kWinWords - 4) * 32 ? (kWinWords - 4) * 32 / W : 0;
+}
+
+// first vertex of the window that holds tile row (column) lo
+__host__ __device__ __forceinline__ uint32_t window_base(uint32_t lo, uint32_t R, uint32_t W)
+{
+	return (uint32_t)((uint64_t)(lo - lo % R) * W);
+}
+
+// 16-bit forms of the heads and tails of each chunk (offsets from the
+// window bases of its first row and column) and the flags saying which are
+// exact.  One block per chunk.
+__global__ __launch_bounds__(kBlock) void pack_chunks_kernel(const uint32_t *__restrict__ heads,
+                                                             const uint32_t *__restrict__ tails,
+                                                             const ChunkMeta *__restrict__ meta, uint64_t nchunks,
+                                                             uint32_t R, uint32_t W, uint16_t *__restrict__ h16,
+                                                             uint16_t *__restrict__ t16, ChunkPack *__restrict__ pack)
+{
+	__shared__ uint32_t s_ok[kBlock / 64];
+	for (uint64_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
+		const ChunkMeta md = meta[c];
+		const bool real = md.row_lo <= md.row_hi;
+		const uint32_t hb = real ? window_base(md.row_lo, R, W) : 0u;
+		const uint32_t tb = real ? window_base(md.col_lo, R, W) : 0u;
+		uint32_t ok = real ? 3u : 0u;
+		for (uint32_t i = threadIdx.x; i < kChunk; i += kBlock) {
+			const uint64_t e = c * kChunk + i;
+			const uint32_t ho = heads[e] - hb, to = tails[e] - tb;
+			if (ho > 0xFFFFu) ok &= ~1u;
+			if (to > 0xFFFFu) ok &= ~2u;
+			h16[e] = (uint16_t)ho;
+			t16[e] = (uint16_t)to;
+		}
+		for (int o = 32; o > 0; o >>= 1) ok &= (uint32_t)__shfl_xor((int)ok, o, 64);
+		if (lane_id() == 0) s_ok[threadIdx.x >> 6] = ok;
+		__syncthreads();
+		if (threadIdx.x == 0) {
+			for (int k = 1; k < (int)(kBlock / 64); ++k) ok &= s_ok[k];
+			pack[c] = ChunkPack{hb, tb, ok, 0u};
+		}
+		__syncthreads();
+	}
+}
+
+void free_chunk_summaries(DevGraph &G)
+{
+	if (G.meta) G.device_bytes -= G.nchunks * sizeof(ChunkMeta) + 16, (void)hipFree(G.meta);
+	if (G.pack) G.device_bytes -= G.nchunks * sizeof(ChunkPack) + 16, (void)hipFree(G.pack);
+	if (G.heads16) G.device_bytes -= G.m_pad * 2, (void)hipFree(G.heads16);
+	if (G.tails16) G.device_bytes -= G.m_pad * 2, (void)hipFree(G.tails16);
+	G.meta = nullptr;
+	G.pack = nullptr;
+	G.heads16 = G.tails16 = nullptr;
+}
+
 // Pads the tile arrays to whole chunks with sentinel edges and builds the
-// chunk summaries.
+// chunk summaries and the 16-bit tile forms.
 int finish_tiles(gphi_ctx *ctx, gphi_graph *gr)
 {
 	DevGraph &G = gr->g;
@@ -144,6 +208,16 @@ int finish_tiles(gphi_ctx *ctx, gphi_graph *gr)
 	if (G.nchunks)
 		chunk_meta_kernel<<<grid_for(G.nchunks, 1, 65536), kBlock, 0, s>>>(G.heads, G.tails, G.n, G.W,
 		                                                                  G.nchunks, G.meta);
+	if ((rc = dmalloc(ctx, (void **)&G.pack, (size_t)G.nchunks * sizeof(ChunkPack) + 16, &G.device_bytes)))
+		return rc;
+	GPHI_HIP(ctx, hipMemsetAsync(G.pack, 0, (size_t)G.nchunks * sizeof(ChunkPack) + 16, s));
+	const uint32_t R = dense_window_rows(G.W);
+	if (R && G.nchunks) {
+		if ((rc = dmalloc(ctx, (void **)&G.heads16, G.m_pad * 2, &G.device_bytes))) return rc;
+		if ((rc = dmalloc(ctx, (void **)&G.tails16, G.m_pad * 2, &G.device_bytes))) return rc;
+		pack_chunks_kernel<<<grid_for(G.nchunks, 1, 65536), kBlock, 0, s>>>(G.heads, G.tails, G.meta, G.nchunks, R,
+		                                                                   G.W, G.heads16, G.tails16, G.pack);
+	}
 	GPHI_HIP(ctx, hipGetLastError());
 	GPHI_HIP(ctx, hipStreamSynchronize(s));
 	return GPHI_OK;
@@ -355,7 +429,7 @@ void free_graph_device(gphi_graph *gr)
 			if (p) (void)hipFree(p);
 		return;
 	}
-	void *ps[] = {G.row_off, G.cols, G.csr_w, G.deg, G.heads, G.tails, G.coo_w, G.meta,
+	void *ps[] = {G.row_off, G.cols, G.csr_w, G.deg, G.heads, G.tails, G.coo_w, G.meta, G.pack, G.heads16, G.tails16,
 	              gr->level, gr->parent, gr->dist, gr->flag, gr->visited, gr->cur,
 	              gr->row_stamp, gr->col_unvis, gr->queue, gr->qoff, gr->ctrl, gr->rank,
 	              gr->sum, gr->contrib, gr->tile_lo, gr->blk_cnt, gr->blk_base, gr->live, gr->live_cnt, gr->trace,
@@ -540,16 +614,7 @@ __device__ __forceinline__ uint4 ld_stream(const uint4 *p)
 // (kWinWords words) is copied into LDS once and serves every chunk of the
 // block inside it; a chunk outside any such run looks its bits up in global
 // memory.  Gathers cost address-unit cycles per lane, LDS reads do not.
-constexpr uint32_t kWinWords = 2048;
 constexpr uint32_t kDenseRun = 8;  // consecutive live-list entries per block run
-
-// tile rows per window for tile width W (0: no window fits)
-static uint32_t dense_window_rows(uint32_t W)
-{
-	if (W == 0) return 0;
-	if ((W & (W - 1)) == 0 && W <= kWinWords * 32) return kWinWords * 32 / W;
-	return W <= (kWinWords - 4) * 32 ? (kWinWords - 4) * 32 / W : 0;
-}
 
 // window [wlo, wlo + wlen) of bitmap words holding tile rows row_lo..row_hi,
 // or false; wlo is a multiple of 4 and wlen <= kWinWords
@@ -601,13 +666,16 @@ __device__ __forceinline__ uint32_t frontier_bits(const uint32_t (&hh)[8], bool 
 }
 
 // Software-pipelined walk of this XCD's live-chunk list (runs of kDenseRun
-// entries per block), 8 edges per lane per chunk.  While chunk k finishes (state bytes of
-// its tails, writes), the frontier bits of chunk k+1 are looked up, and then
-// its tails and the heads of chunk k+2 are requested, so each iteration waits
-// on one HBM latency and one L2 round trip.  Every global load is
-// unconditional (a lane with nothing to fetch reads a cached dummy word) so
-// the compiler can count them.  Tails of a 4-edge quad are fetched only when
-// one of its heads is in the frontier.
+// entries per block), 8 edges per lane per chunk.  While chunk k finishes
+// (state bytes of its tails, writes), the frontier bits of chunk k+1 are
+// looked up, and then its tails and the heads of chunk k+2 are requested, so
+// each iteration waits on one HBM latency and one L2 round trip.  Every
+// global load is unconditional (a lane with nothing to fetch reads a cached
+// dummy word) so the compiler can count them.  Tails of a 4-edge quad are
+// fetched only when one of its heads is in the frontier.
+// Heads and tails come in their 16-bit forms wherever those are exact (half
+// the bytes and memory requests); lane tid always owns edges 4tid..4tid+3
+// and 1024+4tid..1024+4tid+3 of the chunk, in either form.
 // One byte per vertex says whether a tail is new (state 0): "visited before
 // this step" and "reached in this step" live in the same byte, so a
 // candidate costs one lookup.  Every lane that sees state 0 writes parent and
@@ -619,8 +687,47 @@ __device__ __forceinline__ void unpack8(const uint4 a, const uint4 b, uint32_t (
 	x[0] = a.x, x[1] = a.y, x[2] = a.z, x[3] = a.w, x[4] = b.x, x[5] = b.y, x[6] = b.z, x[7] = b.w;
 }
 
-__global__ __launch_bounds__(kBlock, 8) void bfs_dense_kernel(
-    const uint32_t *__restrict__ heads, const uint32_t *__restrict__ tails, const ChunkMeta *__restrict__ meta,
+typedef unsigned int u32x2_t __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint2 ld_stream8(const uint2 *p)
+{
+	const u32x2_t v = __builtin_nontemporal_load(reinterpret_cast<const u32x2_t *>(p));
+	return make_uint2(v.x, v.y);
+}
+
+// Raw edge words of one array of chunk c as loaded: 8 ids, or (16-bit form)
+// 4 words of two offsets each.  q0 / q1: fetch the first / second quad (a
+// lane that skips one reads the array's first bytes instead).
+__device__ __forceinline__ void load_edges(bool p16, const uint32_t *__restrict__ a32,
+                                           const uint16_t *__restrict__ a16, uint64_t c, uint32_t tid, bool q0,
+                                           bool q1, uint32_t (&raw)[8])
+{
+	if (p16) {
+		const uint2 *A = reinterpret_cast<const uint2 *>(a16 + c * kChunk);
+		const uint2 *D = reinterpret_cast<const uint2 *>(a16);
+		const uint2 x = ld_stream8(q0 ? A + tid : D), y = ld_stream8(q1 ? A + tid + kBlock : D);
+		raw[0] = x.x, raw[1] = x.y, raw[2] = y.x, raw[3] = y.y;
+	} else {
+		const uint4 *A = reinterpret_cast<const uint4 *>(a32 + c * kChunk);
+		const uint4 *D = reinterpret_cast<const uint4 *>(a32);
+		unpack8(ld_stream(q0 ? A + tid : D), ld_stream(q1 ? A + tid + kBlock : D), raw);
+	}
+}
+
+// raw edge words -> 8 vertex ids (in place)
+__device__ __forceinline__ void decode_edges(bool p16, uint32_t base, uint32_t (&x)[8])
+{
+	if (p16) {
+		const uint32_t w0 = x[0], w1 = x[1], w2 = x[2], w3 = x[3];
+		x[0] = base + (w0 & 0xFFFFu), x[1] = base + (w0 >> 16);
+		x[2] = base + (w1 & 0xFFFFu), x[3] = base + (w1 >> 16);
+		x[4] = base + (w2 & 0xFFFFu), x[5] = base + (w2 >> 16);
+		x[6] = base + (w3 & 0xFFFFu), x[7] = base + (w3 >> 16);
+	}
+}
+
+__global__ __launch_bounds__(kBlock) void bfs_dense_kernel(
+    const uint32_t *__restrict__ heads, const uint32_t *__restrict__ tails, const uint16_t *__restrict__ heads16,
+    const uint16_t *__restrict__ tails16, const ChunkMeta *__restrict__ meta, const ChunkPack *__restrict__ pack,
     const uint32_t *__restrict__ lists, uint32_t list_cap, const uint32_t *__restrict__ counts,
     const uint32_t *__restrict__ cur, const uint32_t *__restrict__ visited, uint32_t n_words, uint32_t W,
     uint32_t R, uint8_t *__restrict__ state, int32_t *__restrict__ parent)
@@ -651,11 +758,10 @@ __global__ __launch_bounds__(kBlock, 8) void bfs_dense_kernel(
 	uint32_t c1 = v1 ? lst[p1] : c0;
 	uint32_t c2 = v2 ? lst[p2] : c0;
 	// prologue: heads and frontier bits of chunk k, then heads of k+1 and tails of k
+	ChunkPack pk = pack[c0], pk1 = pack[c1];
 	uint32_t h[8], hn[8], t[8];
-	{
-		const uint4 *H = reinterpret_cast<const uint4 *>(heads + (uint64_t)c0 * kChunk);
-		unpack8(ld_stream(H + tid), ld_stream(H + tid + kBlock), h);
-	}
+	load_edges(pk.flags & 1u, heads, heads16, c0, tid, true, true, h);
+	decode_edges(pk.flags & 1u, pk.hbase, h);
 	uint32_t hit;
 	{
 		const ChunkMeta m0 = meta[c0];
@@ -668,13 +774,8 @@ __global__ __launch_bounds__(kBlock, 8) void bfs_dense_kernel(
 		}
 		hit = frontier_bits(h, inw, s_win, wlo, wlen, cur);
 	}
-	{
-		const uint4 *H = reinterpret_cast<const uint4 *>(heads + (uint64_t)c1 * kChunk);
-		unpack8(ld_stream(H + tid), ld_stream(H + tid + kBlock), hn);
-		const uint4 *T = reinterpret_cast<const uint4 *>(tails + (uint64_t)c0 * kChunk);
-		const uint4 *D = reinterpret_cast<const uint4 *>(tails);  // dummy: first 16 bytes
-		unpack8(ld_stream((hit & 0x0Fu) ? T + tid : D), ld_stream((hit & 0xF0u) ? T + tid + kBlock : D), t);
-	}
+	load_edges(pk1.flags & 1u, heads, heads16, c1, tid, true, true, hn);
+	load_edges(pk.flags & 2u, tails, tails16, c0, tid, hit & 0x0Fu, hit & 0xF0u, t);
 	uint32_t ck = c0;  // chunk k
 	for (;;) {
 		// windows (block-uniform): frontier rows of chunk k+1, visited columns of chunk k
@@ -699,6 +800,8 @@ __global__ __launch_bounds__(kBlock, 8) void bfs_dense_kernel(
 			}
 			__syncthreads();
 		}
+		decode_edges(pk.flags & 2u, pk.tbase, t);
+		decode_edges(pk1.flags & 1u, pk1.hbase, hn);
 		// stage A: states of chunk k's new frontier-edge tails, frontier bits of
 		// chunk k+1.  A tail visited in an earlier step (its bit in the window)
 		// needs no state lookup.
@@ -728,15 +831,10 @@ __global__ __launch_bounds__(kBlock, 8) void bfs_dense_kernel(
 				state[t[j]] = 1;
 			}
 		// stage C: the HBM requests of the next iteration: tails of k+1, heads of k+2
+		const ChunkPack pk2 = pack[c2];
 		uint32_t tn[8], h2[8];
-		{
-			const uint4 *T = reinterpret_cast<const uint4 *>(tails + (uint64_t)c1 * kChunk);
-			const uint4 *D = reinterpret_cast<const uint4 *>(tails);
-			unpack8(ld_stream((hit_n & 0x0Fu) ? T + tid : D), ld_stream((hit_n & 0xF0u) ? T + tid + kBlock : D),
-			        tn);
-			const uint4 *H = reinterpret_cast<const uint4 *>(heads + (uint64_t)c2 * kChunk);
-			unpack8(ld_stream(H + tid), ld_stream(H + tid + kBlock), h2);
-		}
+		load_edges(pk1.flags & 2u, tails, tails16, c1, tid, hit_n & 0x0Fu, hit_n & 0xF0u, tn);
+		load_edges(pk2.flags & 1u, heads, heads16, c2, tid, true, true, h2);
 		const uint32_t p3 = nxt(p2);
 		const bool v3 = p3 < end;
 		const uint32_t c3 = v3 ? lst[p3] : c0;
@@ -751,6 +849,8 @@ __global__ __launch_bounds__(kBlock, 8) void bfs_dense_kernel(
 		ck = c1;
 		c1 = c2;
 		c2 = c3;
+		pk = pk1;
+		pk1 = pk2;
 		v1 = v2;
 		v2 = v3;
 		p2 = p3;
@@ -1171,6 +1271,18 @@ __global__ __launch_bounds__(kBlock) void frontier_emit_kernel(
 // host drivers
 // ---------------------------------------------------------------------------
 constexpr unsigned kBfsDenseGrid = 2048;  // BFS/SSSP sweeps: 8 blocks per CU, every block resident at once
+
+// blocks of kBlock threads of a kernel that fit the device at once, rounded
+// down to a multiple of the XCD count (kBfsDenseGrid if the query fails)
+static unsigned resident_grid(const void *kernel)
+{
+	int dev = 0, ncu = 0, per = 0;
+	if (hipGetDevice(&dev) != hipSuccess ||
+	    hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+	    hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, kBlock, 0) != hipSuccess || ncu <= 0 || per <= 0)
+		return kBfsDenseGrid;
+	return std::max<unsigned>((unsigned)(ncu * per) / kNumXcd * kNumXcd, kNumXcd);
+}
 static constexpr unsigned kSparseGrid = 2048;
 static constexpr uint32_t kLevelGroup = 4;  // levels enqueued per read-back of the control block
 
@@ -1225,7 +1337,11 @@ static int frontier_loop(gphi_graph *gr, uint32_t src, gphi_stats *st)
 	const unsigned nblk = G.n_pad / kPadVerts;
 	const uint32_t list_cap = live_list_cap(G.nchunks);
 	const unsigned sgrid = (unsigned)std::max<uint64_t>(1, (G.nchunks + kBlock - 1) / kBlock);
-	unsigned grid = (unsigned)std::min<uint64_t>(kBfsDenseGrid, (G.nchunks + kNumXcd - 1) / kNumXcd * kNumXcd);
+	// every block of the sweep resident at once (blocks of a run-interleaved
+	// list walk all have the same work), a multiple of the XCD count
+	static const unsigned resident = resident_grid(reinterpret_cast<const void *>(bfs_dense_kernel));
+	const unsigned dense_cap = kBfs ? resident : kBfsDenseGrid;
+	unsigned grid = (unsigned)std::min<uint64_t>(dense_cap, (G.nchunks + kNumXcd - 1) / kNumXcd * kNumXcd);
 	grid = std::max<unsigned>(grid / kNumXcd * kNumXcd, kNumXcd);
 	const uint32_t win_rows = dense_window_rows(G.W);
 	uint32_t enq = 0;
@@ -1239,7 +1355,8 @@ static int frontier_loop(gphi_graph *gr, uint32_t src, gphi_stats *st)
 			                                                 gr->ctrl, gr->live, list_cap, gr->live_cnt);
 			if (ev_step) GPHI_HIP(ctx, hipEventRecord(ev[2 * GPHI_MAX_STEPS + 4 + enq], s));
 			if (kBfs)
-				bfs_dense_kernel<<<grid, kBlock, 0, s>>>(G.heads, G.tails, G.meta, gr->live, list_cap, gr->live_cnt,
+				bfs_dense_kernel<<<grid, kBlock, 0, s>>>(G.heads, G.tails, G.heads16, G.tails16, G.meta, G.pack,
+				                                       gr->live, list_cap, gr->live_cnt,
 				                                       gr->cur, gr->visited, G.n_words, G.W, win_rows, gr->flag,
 				                                       gr->parent);
 			else

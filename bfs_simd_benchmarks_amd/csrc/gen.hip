@@ -614,7 +614,7 @@ int retile_graph(gphi_graph *gr, uint32_t W, uint32_t S, int tile_order)
 	if (G.heads) G.device_bytes -= arr, (void)hipFree(G.heads);
 	if (G.tails) G.device_bytes -= arr, (void)hipFree(G.tails);
 	if (G.coo_w) G.device_bytes -= arr, (void)hipFree(G.coo_w);
-	if (G.meta) G.device_bytes -= G.nchunks * sizeof(ChunkMeta) + 16, (void)hipFree(G.meta);
+	free_chunk_summaries(G);
 	G.device_bytes += arr * (nw ? 3 : 2);
 	G.heads = nh;
 	G.tails = nt;
@@ -622,7 +622,6 @@ int retile_graph(gphi_graph *gr, uint32_t W, uint32_t S, int tile_order)
 	sc.keep(nh);
 	sc.keep(nt);
 	if (nw) sc.keep(nw);
-	G.meta = nullptr;
 	G.W = W;
 	G.S = S;
 	G.side = side;

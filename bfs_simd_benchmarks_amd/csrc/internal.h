@@ -35,6 +35,14 @@ struct ChunkMeta {
 	uint32_t row_lo, row_hi, col_lo, col_hi;
 };
 
+// 16-bit forms of a chunk's heads and tails: offsets from the first vertex of
+// the aligned run of tile rows (columns) that holds the chunk's rows
+// (columns); flags bit 0 / bit 1: every head / tail of the chunk, padding
+// included, fits in 16 bits, so the 16-bit form is valid
+struct ChunkPack {
+	uint32_t hbase, tbase, flags, pad;
+};
+
 // Control block of one frontier traversal (128 bytes).  The level loop runs
 // on the device: every kernel of a level reads the direction, the level and
 // the queue sizes here, and the host reads the block back only every few
@@ -77,6 +85,8 @@ struct DevGraph {
 	// tiled COO (padded)
 	uint32_t *heads = nullptr, *tails = nullptr, *coo_w = nullptr;
 	ChunkMeta *meta = nullptr;
+	uint16_t *heads16 = nullptr, *tails16 = nullptr;  // or null when no window fits (W too wide)
+	ChunkPack *pack = nullptr;
 	uint64_t device_bytes = 0;
 };
 
@@ -150,6 +160,7 @@ int run_bellman_ford(gphi_graph *gr, uint32_t src, uint32_t max_sweeps, uint32_t
                      uint32_t *sweeps_run, float *ms);
 
 int finish_tiles(gphi_ctx *ctx, gphi_graph *gr);
+void free_chunk_summaries(DevGraph &G);  // meta and the 16-bit tile forms
 int count_active_tiles(gphi_graph *gr, const int32_t *levels, const std::vector<uint32_t> &dense_steps,
                        std::vector<double> &e_act, std::vector<double> &t_act);
 int tile_offsets(gphi_graph *gr, uint32_t *out);
