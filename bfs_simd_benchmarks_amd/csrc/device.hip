@@ -733,9 +733,9 @@ __global__ __launch_bounds__(kBlock) void bfs_dense_kernel(
     uint32_t R, uint8_t *__restrict__ state, int32_t *__restrict__ parent)
 {
 	__shared__ uint4 s_win4[kWinWords / 4];   // frontier bits of chunk k+1's rows
-	__shared__ uint4 s_vwin4[kWinWords / 4];  // visited bits of chunk k's columns
+	__shared__ uint4 s_vwin4[kWinWords / 4];  // visited / claimed bits of chunk k's columns
 	const uint32_t *s_win = reinterpret_cast<const uint32_t *>(s_win4);
-	const uint32_t *s_vwin = reinterpret_cast<const uint32_t *>(s_vwin4);
+	uint32_t *s_vwin = reinterpret_cast<uint32_t *>(s_vwin4);
 	const uint32_t xcd = blockIdx.x % kNumXcd, nb = gridDim.x / kNumXcd;
 	const uint32_t end = counts[xcd];
 	// runs of kDenseRun consecutive list entries, run r of the XCD to block
@@ -803,15 +803,16 @@ __global__ __launch_bounds__(kBlock) void bfs_dense_kernel(
 		decode_edges(pk.flags & 2u, pk.tbase, t);
 		decode_edges(pk1.flags & 1u, pk1.hbase, hn);
 		// stage A: states of chunk k's new frontier-edge tails, frontier bits of
-		// chunk k+1.  A tail visited in an earlier step (its bit in the window)
-		// needs no state lookup.
+		// chunk k+1.  In the column window each frontier edge claims its tail's
+		// bit: a tail visited in an earlier step (bit set when the window was
+		// loaded) or already claimed by another edge of this block in this step
+		// needs no state lookup, so a block looks each new tail up once.
 		uint32_t need = hit;
 		if (vin) {
 #pragma unroll
 			for (int j = 0; j < 8; ++j) {
-				const uint32_t wi = (t[j] >> 5) - vwlo;
-				const uint32_t vb = wi < vwlen ? (s_vwin[wi] >> (t[j] & 31)) & 1u : 0u;
-				need &= ~(vb << j);
+				const uint32_t wi = (t[j] >> 5) - vwlo, b = 1u << (t[j] & 31);
+				if (((hit >> j) & 1u) && wi < vwlen && (atomicOr(s_vwin + wi, b) & b)) need &= ~(1u << j);
 			}
 		}
 		uint32_t sv[8];
