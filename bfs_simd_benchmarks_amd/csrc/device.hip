@@ -144,36 +144,72 @@ __host__ __device__ __forceinline__ uint32_t window_base(uint32_t lo, uint32_t R
 	return (uint32_t)((uint64_t)(lo - lo % R) * W);
 }
 
-// 16-bit forms of the heads and tails of each chunk (offsets from the
-// window bases of its first row and column) and the flags saying which are
-// exact.  One block per chunk.
+// 16-bit forms of the heads and tails of each chunk (head offsets from the
+// window base of its first row, tail offsets from the smallest tail of each
+// group of kPackGroup edges) and the flags saying which are exact.  One block
+// per chunk; thread i handles edges i + kBlock * r, r = 0..7, so round r is
+// group r.
+static_assert(kPackGroup == kBlock && kChunk == 8 * kPackGroup, "one group per round of the block");
 __global__ __launch_bounds__(kBlock) void pack_chunks_kernel(const uint32_t *__restrict__ heads,
                                                              const uint32_t *__restrict__ tails,
                                                              const ChunkMeta *__restrict__ meta, uint64_t nchunks,
                                                              uint32_t R, uint32_t W, uint16_t *__restrict__ h16,
                                                              uint16_t *__restrict__ t16, ChunkPack *__restrict__ pack)
 {
-	__shared__ uint32_t s_ok[kBlock / 64];
+	__shared__ uint32_t s_lo[8][kBlock / 64], s_hi[8][kBlock / 64], s_ok[kBlock / 64];
+	__shared__ uint32_t s_base[8];
+	const uint32_t wid = threadIdx.x >> 6;
 	for (uint64_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
 		const ChunkMeta md = meta[c];
 		const bool real = md.row_lo <= md.row_hi;
 		const uint32_t hb = real ? window_base(md.row_lo, R, W) : 0u;
-		const uint32_t tb = real ? window_base(md.col_lo, R, W) : 0u;
-		uint32_t ok = real ? 3u : 0u;
-		for (uint32_t i = threadIdx.x; i < kChunk; i += kBlock) {
-			const uint64_t e = c * kChunk + i;
-			const uint32_t ho = heads[e] - hb, to = tails[e] - tb;
-			if (ho > 0xFFFFu) ok &= ~1u;
-			if (to > 0xFFFFu) ok &= ~2u;
+		uint32_t ok = real ? 1u : 0u;
+		uint32_t t[8];
+#pragma unroll
+		for (int r = 0; r < 8; ++r) {
+			const uint64_t e = c * kChunk + threadIdx.x + r * kBlock;
+			const uint32_t ho = heads[e] - hb;
+			if (ho > 0xFFFFu) ok = 0u;
 			h16[e] = (uint16_t)ho;
-			t16[e] = (uint16_t)to;
+			t[r] = tails[e];
+			uint32_t lo = t[r], hi = t[r];
+			for (int o = 32; o > 0; o >>= 1) {
+				lo = min(lo, (uint32_t)__shfl_xor((int)lo, o, 64));
+				hi = max(hi, (uint32_t)__shfl_xor((int)hi, o, 64));
+			}
+			if (lane_id() == 0) {
+				s_lo[r][wid] = lo;
+				s_hi[r][wid] = hi;
+			}
 		}
 		for (int o = 32; o > 0; o >>= 1) ok &= (uint32_t)__shfl_xor((int)ok, o, 64);
-		if (lane_id() == 0) s_ok[threadIdx.x >> 6] = ok;
+		if (lane_id() == 0) s_ok[wid] = ok;
 		__syncthreads();
+		if (threadIdx.x < 8) {
+			const uint32_t r = threadIdx.x;
+			uint32_t lo = s_lo[r][0], hi = s_hi[r][0];
+			for (int k = 1; k < (int)(kBlock / 64); ++k) {
+				lo = min(lo, s_lo[r][k]);
+				hi = max(hi, s_hi[r][k]);
+			}
+			s_base[r] = lo;
+			s_hi[r][0] = hi - lo;  // span of the group
+		}
+		__syncthreads();
+#pragma unroll
+		for (int r = 0; r < 8; ++r) t16[c * kChunk + threadIdx.x + r * kBlock] = (uint16_t)(t[r] - s_base[r]);
 		if (threadIdx.x == 0) {
-			for (int k = 1; k < (int)(kBlock / 64); ++k) ok &= s_ok[k];
-			pack[c] = ChunkPack{hb, tb, ok, 0u};
+			uint32_t okh = s_ok[0], okt = real ? 1u : 0u;
+			for (int k = 1; k < (int)(kBlock / 64); ++k) okh &= s_ok[k];
+			ChunkPack pk;
+			pk.hbase = hb;
+			pk.pad0 = pk.pad1 = 0u;
+			for (int r = 0; r < 8; ++r) {
+				pk.tbase[r] = s_base[r];
+				if (s_hi[r][0] > 0xFFFFu) okt = 0u;
+			}
+			pk.flags = okh | (okt << 1);
+			pack[c] = pk;
 		}
 		__syncthreads();
 	}
@@ -675,7 +711,8 @@ __device__ __forceinline__ uint32_t frontier_bits(const uint32_t (&hh)[8], bool 
 // fetched only when one of its heads is in the frontier.
 // Heads and tails come in their 16-bit forms wherever those are exact (half
 // the bytes and memory requests); lane tid always owns edges 4tid..4tid+3
-// and 1024+4tid..1024+4tid+3 of the chunk, in either form.
+// and 1024+4tid..1024+4tid+3 of the chunk, in either form, so the tails of
+// a wavefront's two quads decode from its two tail groups.
 // One byte per vertex says whether a tail is new (state 0): "visited before
 // this step" and "reached in this step" live in the same byte, so a
 // candidate costs one lookup.  Every lane that sees state 0 writes parent and
@@ -714,15 +751,26 @@ __device__ __forceinline__ void load_edges(bool p16, const uint32_t *__restrict_
 }
 
 // raw edge words -> 8 vertex ids (in place)
-__device__ __forceinline__ void decode_edges(bool p16, uint32_t base, uint32_t (&x)[8])
+// (base0 for the lane's first quad, base1 for its second)
+__device__ __forceinline__ void decode_edges(bool p16, uint32_t base0, uint32_t base1, uint32_t (&x)[8])
 {
 	if (p16) {
 		const uint32_t w0 = x[0], w1 = x[1], w2 = x[2], w3 = x[3];
-		x[0] = base + (w0 & 0xFFFFu), x[1] = base + (w0 >> 16);
-		x[2] = base + (w1 & 0xFFFFu), x[3] = base + (w1 >> 16);
-		x[4] = base + (w2 & 0xFFFFu), x[5] = base + (w2 >> 16);
-		x[6] = base + (w3 & 0xFFFFu), x[7] = base + (w3 >> 16);
+		x[0] = base0 + (w0 & 0xFFFFu), x[1] = base0 + (w0 >> 16);
+		x[2] = base0 + (w1 & 0xFFFFu), x[3] = base0 + (w1 >> 16);
+		x[4] = base1 + (w2 & 0xFFFFu), x[5] = base1 + (w2 >> 16);
+		x[6] = base1 + (w3 & 0xFFFFu), x[7] = base1 + (w3 >> 16);
 	}
+}
+
+// what the sweep needs of a chunk's ChunkPack: head base, flags, and the
+// tail bases of this wavefront's two edge groups (wv and 4 + wv)
+struct PackSlot {
+	uint32_t hbase, flags, tb0, tb1;
+};
+__device__ __forceinline__ PackSlot pack_slot(const ChunkPack *__restrict__ pack, uint32_t c, uint32_t wv)
+{
+	return PackSlot{pack[c].hbase, pack[c].flags, pack[c].tbase[wv], pack[c].tbase[4 + wv]};
 }
 
 __global__ __launch_bounds__(kBlock) void bfs_dense_kernel(
@@ -748,6 +796,8 @@ __global__ __launch_bounds__(kBlock) void bfs_dense_kernel(
 	if (pos >= end) return;
 	const uint32_t *lst = lists + (uint64_t)xcd * list_cap;
 	const uint32_t tid = threadIdx.x;
+	const uint32_t wv = (uint32_t)__builtin_amdgcn_readfirstlane((int)(tid >> 6));  // wavefront in the block
+	static_assert(kPackGroup == 4 * 64, "a wavefront's quad row is one tail group");
 	// first words of the windows held in LDS (block-uniform)
 	uint32_t win_lo = UINT32_MAX, vwin_lo = UINT32_MAX;
 	// list entries: chunks k, k+1, k+2 (an absent one repeats chunk k's id, so
@@ -758,10 +808,10 @@ __global__ __launch_bounds__(kBlock) void bfs_dense_kernel(
 	uint32_t c1 = v1 ? lst[p1] : c0;
 	uint32_t c2 = v2 ? lst[p2] : c0;
 	// prologue: heads and frontier bits of chunk k, then heads of k+1 and tails of k
-	ChunkPack pk = pack[c0], pk1 = pack[c1];
+	PackSlot pk = pack_slot(pack, c0, wv), pk1 = pack_slot(pack, c1, wv);
 	uint32_t h[8], hn[8], t[8];
 	load_edges(pk.flags & 1u, heads, heads16, c0, tid, true, true, h);
-	decode_edges(pk.flags & 1u, pk.hbase, h);
+	decode_edges(pk.flags & 1u, pk.hbase, pk.hbase, h);
 	uint32_t hit;
 	{
 		const ChunkMeta m0 = meta[c0];
@@ -800,8 +850,8 @@ __global__ __launch_bounds__(kBlock) void bfs_dense_kernel(
 			}
 			__syncthreads();
 		}
-		decode_edges(pk.flags & 2u, pk.tbase, t);
-		decode_edges(pk1.flags & 1u, pk1.hbase, hn);
+		decode_edges(pk.flags & 2u, pk.tb0, pk.tb1, t);
+		decode_edges(pk1.flags & 1u, pk1.hbase, pk1.hbase, hn);
 		// stage A: states of chunk k's new frontier-edge tails, frontier bits of
 		// chunk k+1.  In the column window each frontier edge claims its tail's
 		// bit: a tail visited in an earlier step (bit set when the window was
@@ -832,7 +882,7 @@ __global__ __launch_bounds__(kBlock) void bfs_dense_kernel(
 				state[t[j]] = 1;
 			}
 		// stage C: the HBM requests of the next iteration: tails of k+1, heads of k+2
-		const ChunkPack pk2 = pack[c2];
+		const PackSlot pk2 = pack_slot(pack, c2, wv);
 		uint32_t tn[8], h2[8];
 		load_edges(pk1.flags & 2u, tails, tails16, c1, tid, hit_n & 0x0Fu, hit_n & 0xF0u, tn);
 		load_edges(pk2.flags & 1u, heads, heads16, c2, tid, true, true, h2);

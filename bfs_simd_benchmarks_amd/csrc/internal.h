@@ -35,12 +35,16 @@ struct ChunkMeta {
 	uint32_t row_lo, row_hi, col_lo, col_hi;
 };
 
-// 16-bit forms of a chunk's heads and tails: offsets from the first vertex of
-// the aligned run of tile rows (columns) that holds the chunk's rows
-// (columns); flags bit 0 / bit 1: every head / tail of the chunk, padding
-// included, fits in 16 bits, so the 16-bit form is valid
+// 16-bit forms of a chunk's heads and tails.  Heads: offsets from the first
+// vertex of the aligned run of tile rows that holds the chunk's rows.  Tails:
+// offsets from the smallest tail of each group of kPackGroup consecutive
+// edges (a wavefront's quad row in the dense sweep).  flags bit 0 / bit 1:
+// every head / tail of the chunk, padding included, fits in 16 bits, so the
+// 16-bit form is valid.
+constexpr uint32_t kPackGroup = 256;
 struct ChunkPack {
-	uint32_t hbase, tbase, flags, pad;
+	uint32_t hbase, flags, pad0, pad1;
+	uint32_t tbase[8];  // per group of kPackGroup edges
 };
 
 // Control block of one frontier traversal (128 bytes).  The level loop runs
