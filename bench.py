@@ -124,11 +124,15 @@ def main() -> int:
         g.bfs(int(s), parents=False, levels=False)
     timed = srcs[args.warmup:]
     stats = []
+    # the timed BFS run with per-level event timing off (their kernels back to
+    # back, total device time only); the roofline pass below re-runs them with it on
+    ctx.set_level_timing(False)
     t0 = time.perf_counter()
     for s in timed:
         _, _, st = g.bfs(int(s), parents=False, levels=False)
         stats.append(st)
     wall = time.perf_counter() - t0
+    ctx.set_level_timing(True)
     edges = sum(int(st.edges_traversed) for st in stats)
     gteps = edges / wall / 1e9
     print(f"[bench] {len(timed)} BFS in {wall * 1e3:.1f} ms wall, "
@@ -136,7 +140,7 @@ def main() -> int:
           file=sys.stderr, flush=True)
 
     # roofline of the dense sweep: bytes from the (deterministic) levels of
-    # each timed source, durations from the timed runs' HIP events
+    # each timed source, durations from the HIP events of a second run of it
     dense_ms, dense_bytes, dense_launches, select_ms = 0.0, 0.0, 0, 0.0
     step_bytes_total, step_ms_total = 0.0, 0.0
     levels0 = None
@@ -146,11 +150,11 @@ def main() -> int:
             levels0 = lv
         tr = st2.trace()
         ab = g.alg_bytes(lv, st2)
-        sw, sel = st.sweep_times(), st.select_times()
-        for k in range(st.steps):
-            f_next = float(tr[k + 1][1]) if k + 1 < st.steps else 0.0
+        sw, sel, up = st2.sweep_times(), st2.select_times(), st2.update_times()
+        for k in range(st2.steps):
+            f_next = float(tr[k + 1][1]) if k + 1 < st2.steps else 0.0
             step_bytes_total += ab[k]
-            step_ms_total += float(sw[k]) + float(sel[k]) + float(st.update_times()[k])
+            step_ms_total += float(sw[k]) + float(sel[k]) + float(up[k])
             select_ms += float(sel[k])
             if tr[k][0] == 1:
                 # e_act/t_act are folded into ab; recover the sweep share

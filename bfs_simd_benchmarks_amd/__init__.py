@@ -119,6 +119,7 @@ def _declare(lib):
         "gphi_version": (C.c_char_p, []),
         "gphi_open": (i32, [C.POINTER(Opts), C.POINTER(_P)]),
         "gphi_close": (None, [_P]),
+        "gphi_set_level_timing": (i32, [_P, i32]),
         "gphi_last_error": (C.c_char_p, [_P]),
         "gphi_graph_from_desc": (i32, [_P, C.POINTER(GraphDesc), C.POINTER(_P)]),
         "gphi_graph_load_text": (i32, [_P, C.c_char_p, u32, u32, i32, C.POINTER(_P)]),
@@ -287,6 +288,12 @@ class Context:
     def check(self, rc: int):
         if rc:
             raise GphiError(rc, self.lib.gphi_last_error(self.h).decode())
+
+    def set_level_timing(self, on: bool):
+        """Per-level HIP-event timing of traversals (Stats sweep/select/update
+        times); on by default.  Off, only total_ms is recorded and the levels'
+        kernels run back to back."""
+        self.check(self.lib.gphi_set_level_timing(self.h, 1 if on else 0))
 
     def close(self):
         if self.h:

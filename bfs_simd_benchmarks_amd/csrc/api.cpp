@@ -75,6 +75,14 @@ int gphi_open(const gphi_opts *opts, gphi_ctx **out)
 	return GPHI_OK;
 }
 
+int gphi_set_level_timing(gphi_ctx *ctx, int on)
+{
+	if (!ctx) return set_error(nullptr, GPHI_E_ARG, "gphi_set_level_timing: ctx is NULL");
+	std::lock_guard<std::mutex> lk(ctx->mu);
+	ctx->level_timing = on != 0;
+	return GPHI_OK;
+}
+
 void gphi_close(gphi_ctx *ctx)
 {
 	if (!ctx) return;

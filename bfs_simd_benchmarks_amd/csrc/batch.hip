@@ -449,9 +449,11 @@ int batch_loop(gphi_graph *gr, const uint32_t *d_srcs, uint32_t k, gphi_stats *s
 	const uint32_t threshold = (uint32_t)G.m / ctx->t_ratio;
 	StepCtrl *hc = ctx->h_ctrl;
 	hipEvent_t *ev = ctx->ev;
-	const bool timing = st != nullptr && ctx->ev_ok;
+	// total time from two events; per-step events only with level timing on
+	const bool total_timing = st != nullptr && ctx->ev_ok;
+	const bool timing = total_timing && ctx->level_timing;
 	const size_t mask_bytes = (size_t)G.n_pad * NW * 8;
-	if (timing) GPHI_HIP(ctx, hipEventRecord(ev[0], s));
+	if (total_timing) GPHI_HIP(ctx, hipEventRecord(ev[0], s));
 	GPHI_HIP(ctx, hipMemsetAsync(gr->b_front, 0, mask_bytes, s));
 	GPHI_HIP(ctx, hipMemsetAsync(gr->b_seen, 0, mask_bytes, s));
 	GPHI_HIP(ctx, hipMemsetAsync(gr->b_next, 0, mask_bytes, s));
@@ -510,7 +512,7 @@ int batch_loop(gphi_graph *gr, const uint32_t *d_srcs, uint32_t k, gphi_stats *s
 	GPHI_HIP(ctx, hipMemsetAsync(gr->b_sums, 0, 16, s));
 	batch_sums_kernel<NW><<<std::min<unsigned>(grid_n(G.n), 4096), kBlock, 0, s>>>(G.n, gr->b_seen, G.deg,
 	                                                                               gr->b_sums);
-	if (timing) GPHI_HIP(ctx, hipEventRecord(ev[2 * GPHI_MAX_STEPS + 2], s));
+	if (total_timing) GPHI_HIP(ctx, hipEventRecord(ev[2 * GPHI_MAX_STEPS + 2], s));
 	GPHI_HIP(ctx, hipMemcpyAsync(sums_host, gr->b_sums, 16, hipMemcpyDeviceToHost, s));
 	GPHI_HIP(ctx, hipStreamSynchronize(s));
 	if (st) {
@@ -524,8 +526,12 @@ int batch_loop(gphi_graph *gr, const uint32_t *d_srcs, uint32_t k, gphi_stats *s
 				st->update_ms[q] = ev_ms(ev[2 + 2 * q], ev[3 + 2 * q]);
 				prev = ev[3 + 2 * q];
 			}
-			st->total_ms = ev_ms(ev[0], ev[2 * GPHI_MAX_STEPS + 2]);
+		} else {
+			memset(st->sweep_ms, 0, sizeof(st->sweep_ms));
+			memset(st->update_ms, 0, sizeof(st->update_ms));
+			st->init_ms = 0.f;
 		}
+		st->total_ms = total_timing ? ev_ms(ev[0], ev[2 * GPHI_MAX_STEPS + 2]) : 0.f;
 	}
 	return GPHI_OK;
 }

@@ -1355,8 +1355,11 @@ static int frontier_loop(gphi_graph *gr, uint32_t src, gphi_stats *st)
 	const uint32_t threshold = (uint32_t)G.m / ctx->t_ratio;
 	StepCtrl *hc = ctx->h_ctrl;
 	hipEvent_t *ev = ctx->ev;
-	const bool timing = st != nullptr && ctx->ev_ok;
-	if (timing) GPHI_HIP(ctx, hipEventRecord(ev[0], s));
+	// total time from two events; per-level events only with level timing on
+	const bool total_timing = st != nullptr && ctx->ev_ok;
+	const bool timing = total_timing && ctx->level_timing;
+	constexpr uint32_t kEvEnd = 3 * GPHI_MAX_STEPS + 3;  // end of the last group (no level timing)
+	if (total_timing) GPHI_HIP(ctx, hipEventRecord(ev[0], s));
 
 	// state initialisation (the reference's untimed set-up, bfs.cpp:771-797)
 	size_t nb = (size_t)G.n_pad * 4;
@@ -1424,6 +1427,7 @@ static int frontier_loop(gphi_graph *gr, uint32_t src, gphi_stats *st)
 			if (ev_step) GPHI_HIP(ctx, hipEventRecord(ev[3 + 2 * enq], s));
 		}
 		GPHI_HIP(ctx, hipGetLastError());
+		if (total_timing && !timing) GPHI_HIP(ctx, hipEventRecord(ev[kEvEnd], s));
 		GPHI_HIP(ctx, hipMemcpyAsync(hc, gr->ctrl, sizeof(StepCtrl), hipMemcpyDeviceToHost, s));
 		GPHI_HIP(ctx, hipStreamSynchronize(s));
 		if (hc->done) break;
@@ -1462,6 +1466,12 @@ static int frontier_loop(gphi_graph *gr, uint32_t src, gphi_stats *st)
 				prev = ev[3 + 2 * k];
 			}
 			st->total_ms = nt ? ev_ms(ev[0], ev[1 + 2 * nt]) : st->init_ms;
+		} else {
+			memset(st->sweep_ms, 0, sizeof(st->sweep_ms));
+			memset(st->select_ms, 0, sizeof(st->select_ms));
+			memset(st->update_ms, 0, sizeof(st->update_ms));
+			st->init_ms = 0.f;
+			st->total_ms = total_timing ? ev_ms(ev[0], ev[kEvEnd]) : 0.f;
 		}
 	}
 	return GPHI_OK;

@@ -109,6 +109,7 @@ struct gphi_ctx {
 	// done, [2*MAX+4+k] chunk selection of dense step k done
 	hipEvent_t ev[3 * GPHI_MAX_STEPS + 4];
 	bool ev_ok = false;
+	bool level_timing = true;  // gphi_set_level_timing
 };
 
 struct gphi_graph {

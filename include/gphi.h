@@ -84,6 +84,13 @@ typedef struct gphi_stats {
 
 /* ---- context: replaces the globals NNODES..T_RATIO (apps/bfs_simd/bfs.cpp:17-26) ---- */
 int gphi_open(const gphi_opts *opts, gphi_ctx **out);
+/* Per-level timing of the traversals on this context (gphi_stats sweep_ms,
+ * select_ms, update_ms, init_ms from HIP events around each level's kernels):
+ * on (1) by default.  Off (0), a traversal records only its total_ms, and the
+ * kernels of consecutive levels run back to back (the per-level events cost
+ * about 0.1 ms of a 2.6 ms scale-25 BFS).  The reference's counterpart is its
+ * per-phase timers, apps/bfs_simd/bfs.cpp:733-751. */
+int gphi_set_level_timing(gphi_ctx *ctx, int on);
 void gphi_close(gphi_ctx *ctx);
 const char *gphi_last_error(gphi_ctx *ctx);
 const char *gphi_version(void);

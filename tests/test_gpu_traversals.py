@@ -150,3 +150,22 @@ def test_bellman_ford_unbounded_sweeps(ctx, c1):
     finally:
         gg.free()
     assert np.array_equal(d, O.bellman_ford_hops(c1.n, c1.rh, c1.rt, 0, 99)) and run < 99
+
+
+def test_bfs_level_timing_off(ctx, c1):
+    # per-level event timing off: same levels and trace, only the total time
+    gg = upload(ctx, c1)
+    try:
+        lv_on, _, st_on = gg.bfs(0)
+        ctx.set_level_timing(False)
+        try:
+            lv_off, _, st_off = gg.bfs(0)
+        finally:
+            ctx.set_level_timing(True)
+        assert np.array_equal(lv_on, lv_off)
+        assert st_on.steps == st_off.steps and np.array_equal(st_on.trace(), st_off.trace())
+        assert st_off.edges_traversed == st_on.edges_traversed
+        assert not any(st_off.sweep_times()) and not any(st_off.update_times())
+        assert st_off.total_ms > 0 and any(st_on.sweep_times())
+    finally:
+        gg.free()
