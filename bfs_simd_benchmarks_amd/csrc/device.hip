@@ -1361,14 +1361,11 @@ static int frontier_loop(gphi_graph *gr, uint32_t src, gphi_stats *st)
 	constexpr uint32_t kEvEnd = 3 * GPHI_MAX_STEPS + 3;  // end of the last group (no level timing)
 	if (total_timing) GPHI_HIP(ctx, hipEventRecord(ev[0], s));
 
-	// state initialisation (the reference's untimed set-up, bfs.cpp:771-797)
-	size_t nb = (size_t)G.n_pad * 4;
-	if (kBfs) {
-		GPHI_HIP(ctx, hipMemsetAsync(gr->level, 0xFF, nb, s));
-		GPHI_HIP(ctx, hipMemsetAsync(gr->parent, 0xFF, nb, s));
-	} else {
-		fill_u32_kernel<<<grid_for(G.n_pad, kBlock, 4096), kBlock, 0, s>>>(gr->dist, G.n_pad, INT_MAX);
-	}
+	// state initialisation (the reference's untimed set-up, bfs.cpp:771-797).
+	// BFS levels and parents are written for reached vertices only and never
+	// read during the traversal: the -1 of unreached ones is filled in by
+	// bfs_finalize_kernel when they are copied out.
+	if (!kBfs) fill_u32_kernel<<<grid_for(G.n_pad, kBlock, 4096), kBlock, 0, s>>>(gr->dist, G.n_pad, INT_MAX);
 	GPHI_HIP(ctx, hipMemsetAsync(gr->flag, 0, G.n_pad, s));
 	if (kBfs) GPHI_HIP(ctx, hipMemsetAsync(gr->visited, 0, (size_t)G.n_words * 4, s));
 	GPHI_HIP(ctx, hipMemsetAsync(gr->cur, 0, (size_t)G.n_words * 4, s));
@@ -1502,6 +1499,19 @@ void launch_frontier_update_flags(gphi_graph *gr, uint32_t next_level, uint32_t 
 	                                            gr->queue, gr->qoff, gr->tile_lo);
 }
 
+// level and parent -1 for the vertices a BFS did not reach (state 0; every
+// reached vertex ends in kVisited)
+__global__ __launch_bounds__(kBlock) void bfs_finalize_kernel(uint32_t n, const uint8_t *__restrict__ state,
+                                                              int32_t *__restrict__ level,
+                                                              int32_t *__restrict__ parent)
+{
+	for (uint32_t v = blockIdx.x * kBlock + threadIdx.x; v < n; v += gridDim.x * kBlock)
+		if (state[v] == 0) {
+			if (level) level[v] = -1;
+			if (parent) parent[v] = -1;
+		}
+}
+
 int run_bfs(gphi_graph *gr, uint32_t src, int32_t *levels, int32_t *parents, gphi_stats *st)
 {
 	gphi_ctx *ctx = gr->ctx;
@@ -1512,6 +1522,9 @@ int run_bfs(gphi_graph *gr, uint32_t src, int32_t *levels, int32_t *parents, gph
 	if (st) memset(st, 0, sizeof(*st));
 	if ((rc = frontier_loop<true>(gr, src, st))) return rc;
 	hipStream_t s = ctx->stream;
+	if (levels || parents)
+		bfs_finalize_kernel<<<grid_for(G.n, kBlock, 8192), kBlock, 0, s>>>(
+		    G.n, gr->flag, levels ? gr->level : nullptr, parents ? gr->parent : nullptr);
 	if (levels) GPHI_HIP(ctx, hipMemcpyAsync(levels, gr->level, (size_t)G.n * 4, hipMemcpyDeviceToHost, s));
 	if (parents) GPHI_HIP(ctx, hipMemcpyAsync(parents, gr->parent, (size_t)G.n * 4, hipMemcpyDeviceToHost, s));
 	GPHI_HIP(ctx, hipStreamSynchronize(s));
