@@ -397,6 +397,23 @@ __global__ void batch_seed_kernel(const uint32_t *__restrict__ srcs, uint32_t k,
 	}
 }
 
+// level 255 for every (source, vertex) pair the traversal did not reach: the
+// level bytes are written for reached pairs only, so this runs only when the
+// levels are copied out.  One lane per vertex, one coalesced store per source.
+template <int NW>
+__global__ __launch_bounds__(kBlock) void batch_finalize_kernel(uint32_t n, uint32_t n_pitch, uint32_t k,
+                                                                const u64 *__restrict__ seen, uint8_t *level)
+{
+	const uint32_t v = blockIdx.x * kBlock + threadIdx.x;
+	if (v >= n) return;
+#pragma unroll
+	for (int i = 0; i < NW; ++i) {
+		const u64 sn = seen[(uint64_t)v * NW + i];
+		for (uint32_t b = 0; b < 64 && i * 64 + b < k; ++b)
+			if (!((sn >> b) & 1ull)) level[(uint64_t)(i * 64 + b) * n_pitch + v] = kUnreached;
+	}
+}
+
 // totals: sum over v of popcount(seen[v]) and of deg[v] * popcount(seen[v])
 template <int NW>
 __global__ __launch_bounds__(kBlock) void batch_sums_kernel(uint32_t n, const u64 *__restrict__ seen,
@@ -457,7 +474,6 @@ int batch_loop(gphi_graph *gr, const uint32_t *d_srcs, uint32_t k, gphi_stats *s
 	GPHI_HIP(ctx, hipMemsetAsync(gr->b_front, 0, mask_bytes, s));
 	GPHI_HIP(ctx, hipMemsetAsync(gr->b_seen, 0, mask_bytes, s));
 	GPHI_HIP(ctx, hipMemsetAsync(gr->b_next, 0, mask_bytes, s));
-	GPHI_HIP(ctx, hipMemsetAsync(gr->b_level, kUnreached, (size_t)k * G.n_pad, s));
 	GPHI_HIP(ctx, hipMemsetAsync(gr->flag, 0, G.n_pad, s));
 	GPHI_HIP(ctx, hipMemsetAsync(gr->cur, 0, (size_t)G.n_words * 4, s));
 	GPHI_HIP(ctx, hipMemsetAsync(gr->b_live, 0, (size_t)(1 + kLiveSlots) * kLiveStride * 8, s));
@@ -563,7 +579,7 @@ int run_bfs_batch(gphi_graph *gr, const uint32_t *srcs, uint32_t k, uint8_t *lev
 		if (e == hipSuccess) e = hipMalloc((void **)&gr->b_seen, mb);
 		if (e == hipSuccess) e = hipMalloc((void **)&gr->b_next, mb);
 		if (e == hipSuccess) e = hipMalloc((void **)&gr->b_level, (size_t)nw * 64 * G.n_pad);
-		if (e == hipSuccess) e = hipMalloc((void **)&gr->b_sums, 16);
+		if (e == hipSuccess) e = hipMalloc((void **)&gr->b_sums, 16 + 256 * 4);  // totals, then the sources
 		if (e == hipSuccess) e = hipMalloc((void **)&gr->b_live, (size_t)(1 + kLiveSlots) * kLiveStride * 8);
 		if (e != hipSuccess)
 			return set_error(ctx, GPHI_E_OOM, "batched BFS buffers (%zu MB): %s",
@@ -571,13 +587,8 @@ int run_bfs_batch(gphi_graph *gr, const uint32_t *srcs, uint32_t k, uint8_t *lev
 		gr->batch_words = nw;
 	}
 	hipStream_t s = ctx->stream;
-	uint32_t *d_srcs = nullptr;
-	GPHI_HIP(ctx, hipMalloc((void **)&d_srcs, (size_t)k * 4));
-	hipError_t e = hipMemcpy(d_srcs, srcs, (size_t)k * 4, hipMemcpyHostToDevice);
-	if (e != hipSuccess) {
-		(void)hipFree(d_srcs);
-		return hip_fail(ctx, e, "copy sources", __FILE__, __LINE__);
-	}
+	uint32_t *d_srcs = reinterpret_cast<uint32_t *>(gr->b_sums + 2);
+	GPHI_HIP(ctx, hipMemcpyAsync(d_srcs, srcs, (size_t)k * 4, hipMemcpyHostToDevice, s));
 	if (st) memset(st, 0, sizeof(*st));
 	u64 sums[2] = {0, 0};
 	// the masks are sized for gr->batch_words >= nw words; use that stride
@@ -586,13 +597,20 @@ int run_bfs_batch(gphi_graph *gr, const uint32_t *srcs, uint32_t k, uint8_t *lev
 	case 2: rc = batch_loop<2>(gr, d_srcs, k, st, sums); break;
 	default: rc = batch_loop<4>(gr, d_srcs, k, st, sums); break;
 	}
-	(void)hipFree(d_srcs);
 	if (rc) return rc;
 	if (st) {
 		st->reached = sums[0];
 		st->edges_traversed = sums[1];
 	}
 	if (levels) {
+		const unsigned fg = grid_n(G.n);
+		if (gr->batch_words == 1)
+			batch_finalize_kernel<1><<<fg, kBlock, 0, s>>>(G.n, G.n_pad, k, gr->b_seen, gr->b_level);
+		else if (gr->batch_words == 2)
+			batch_finalize_kernel<2><<<fg, kBlock, 0, s>>>(G.n, G.n_pad, k, gr->b_seen, gr->b_level);
+		else
+			batch_finalize_kernel<4><<<fg, kBlock, 0, s>>>(G.n, G.n_pad, k, gr->b_seen, gr->b_level);
+		GPHI_HIP(ctx, hipGetLastError());
 		GPHI_HIP(ctx, hipMemcpy2DAsync(levels, G.n, gr->b_level, G.n_pad, G.n, k, hipMemcpyDeviceToHost, s));
 		GPHI_HIP(ctx, hipStreamSynchronize(s));
 	}
