@@ -1131,22 +1131,27 @@ __global__ __launch_bounds__(kBlock) void frontier_mark_kernel(
 		if (kBfs) visited[w] |= bits;
 		if (row_uniform) row_stamp[v0 / W] = next_level;
 		F = (uint32_t)__popc(bits);
-		// the word's 32 degrees in one 128-byte line (deg is padded to n_pad)
-		const uint4 *dp = reinterpret_cast<const uint4 *>(deg + v0);
-		uint4 dq[8];
-#pragma unroll
-		for (int j = 0; j < 8; ++j) dq[j] = dp[j];
-#pragma unroll
-		for (int j = 0; j < 8; ++j) {
-			const uint32_t dd[4] = {dq[j].x, dq[j].y, dq[j].z, dq[j].w};
-#pragma unroll
-			for (int k = 0; k < 4; ++k) {
-				const uint32_t i = 4 * j + k;
-				if (bits >> i & 1u) {
-					if (kBfs) level[v0 + i] = (int32_t)next_level;
-					if (!row_uniform) row_stamp[(v0 + i) / W] = next_level;
-					D += dd[k];
-					Q += dd[k] > 0;
+	}
+	// Levels and degrees of the new vertices, vertex-major over the wave's
+	// 2048 vertices: in round i lane l takes vertex 64i + l, whose bit is in
+	// word 2i + l/32 of the wave, so every level store and degree load of a
+	// round covers 64 consecutive vertices (a lane per word would scatter them
+	// 128 bytes apart).  Rounds without a new vertex are skipped (uniform).
+	{
+		const uint32_t lane = lane_id();
+		const uint32_t wv0 = (w - lane) * 32;
+		const unsigned long long any = __ballot(bits != 0);
+		if (any) {
+			for (uint32_t i = 0; i < 32; ++i) {
+				const uint32_t wb = (uint32_t)__shfl((int)bits, (int)(2 * i + (lane >> 5)), 64);
+				if (!((any >> (2 * i)) & 3ull)) continue;
+				if ((wb >> (lane & 31)) & 1u) {
+					const uint32_t v = wv0 + 64 * i + lane;
+					if (kBfs) level[v] = (int32_t)next_level;
+					if (!row_uniform) row_stamp[v / W] = next_level;
+					const uint32_t d = deg[v];
+					D += d;
+					Q += d > 0;
 				}
 			}
 		}
