@@ -147,9 +147,9 @@ __host__ __device__ __forceinline__ uint32_t window_base(uint32_t lo, uint32_t R
 // 16-bit forms of the heads and tails of each chunk (head offsets from the
 // window base of its first row, tail offsets from the smallest tail of each
 // group of kPackGroup edges) and the flags saying which are exact.  One block
-// per chunk; thread i handles edges i + kBlock * r, r = 0..7, so round r is
+// per chunk; thread i handles edges i + kBlock * r, r = 0..7, so pass r is
 // group r.
-static_assert(kPackGroup == kBlock && kChunk == 8 * kPackGroup, "one group per round of the block");
+static_assert(kPackGroup == kBlock && kChunk == 8 * kPackGroup, "one group per pass of the block");
 __global__ __launch_bounds__(kBlock) void pack_chunks_kernel(const uint32_t *__restrict__ heads,
                                                              const uint32_t *__restrict__ tails,
                                                              const ChunkMeta *__restrict__ meta, uint64_t nchunks,
@@ -479,8 +479,8 @@ void free_graph_device(gphi_graph *gr)
 // groups of 64 (a wavefront's), group g to XCD g % 8.
 static uint32_t live_list_cap(uint64_t nchunks)
 {
-	const uint64_t groups = (nchunks + 63) / 64;
-	return (uint32_t)((groups + kNumXcd - 1) / kNumXcd * 64);
+	const uint64_t groups = (nchunks + kBlock - 1) / kBlock;
+	return (uint32_t)((groups + kNumXcd - 1) / kNumXcd * kBlock);
 }
 
 // ---------------------------------------------------------------------------
@@ -618,21 +618,28 @@ __global__ __launch_bounds__(kBlock) void chunk_select_kernel(
 			}
 		}
 	}
-	// the 64 chunks of wavefront w go to XCD w % 8: every region of the edge
+	// the 256 chunks of block b go to XCD b % 8: every region of the edge
 	// order is shared by all XCDs, so the lists stay balanced however the live
 	// chunks cluster (each XCD still walks its chunks in edge order), and a
-	// wavefront appends with one atomic
-	const uint32_t x = (uint32_t)((c / 64) % kNumXcd);
-	for (uint32_t xx = 0; xx < (uint32_t)kNumXcd; ++xx) {
-		const bool mine = live && x == xx;
-		const unsigned long long m = __ballot(mine);
-		if (!m) continue;
-		uint32_t base = 0;
-		const uint32_t leader = (uint32_t)__ffsll((long long)m) - 1;
-		if (lane_id() == leader) base = atomicAdd(counts + xx, (uint32_t)__popcll(m));
-		base = __shfl(base, leader, 64);
-		if (mine) lists[(uint64_t)xx * list_cap + base + __popcll(m & ((1ull << lane_id()) - 1ull))] = (uint32_t)c;
+	// block appends with one atomic
+	__shared__ uint32_t s_cnt[kBlock / 64 + 1];
+	const unsigned long long m = __ballot(live);
+	const uint32_t wid = threadIdx.x >> 6;
+	if (lane_id() == 0) s_cnt[wid] = (uint32_t)__popcll(m);
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		uint32_t tot = 0;
+		for (uint32_t k = 0; k < kBlock / 64; ++k) {
+			const uint32_t n = s_cnt[k];
+			s_cnt[k] = tot;
+			tot += n;
+		}
+		s_cnt[kBlock / 64] = tot ? atomicAdd(counts + blockIdx.x % kNumXcd, tot) : 0u;
 	}
+	__syncthreads();
+	if (live)
+		lists[(uint64_t)(blockIdx.x % kNumXcd) * list_cap + s_cnt[kBlock / 64] + s_cnt[wid] +
+		      __popcll(m & ((1ull << lane_id()) - 1ull))] = (uint32_t)c;
 }
 
 typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
@@ -1133,10 +1140,11 @@ __global__ __launch_bounds__(kBlock) void frontier_mark_kernel(
 		F = (uint32_t)__popc(bits);
 	}
 	// Levels and degrees of the new vertices, vertex-major over the wave's
-	// 2048 vertices: in round i lane l takes vertex 64i + l, whose bit is in
+	// 2048 vertices: in iteration i lane l takes vertex 64i + l, whose bit is in
 	// word 2i + l/32 of the wave, so every level store and degree load of a
-	// round covers 64 consecutive vertices (a lane per word would scatter them
-	// 128 bytes apart).  Rounds without a new vertex are skipped (uniform).
+	// iteration covers 64 consecutive vertices (a lane per word would scatter
+	// them 128 bytes apart).  Iterations without a new vertex are skipped
+	// (uniform).
 	{
 		const uint32_t lane = lane_id();
 		const uint32_t wv0 = (w - lane) * 32;
