@@ -475,6 +475,11 @@ void free_graph_device(gphi_graph *gr)
 		if (p) (void)hipFree(p);
 }
 
+// The per-XCD live-list counters, one 128-byte line apart so that their
+// atomics do not queue on one line.
+constexpr uint32_t kLiveCntStride = 32;
+constexpr size_t kLiveCntBytes = (size_t)kNumXcd * kLiveCntStride * 4;
+
 // Capacity of one XCD's live-chunk list: chunk selection hands out chunks in
 // groups of 64 (a wavefront's), group g to XCD g % 8.
 static uint32_t live_list_cap(uint64_t nchunks)
@@ -506,14 +511,14 @@ static int ensure_state(gphi_graph *gr)
 	if ((rc = dmalloc(ctx, (void **)&gr->ctrl, sizeof(StepCtrl), nullptr))) return rc;
 	const size_t list_cap = live_list_cap(G.nchunks);
 	if ((rc = dmalloc(ctx, (void **)&gr->live, list_cap * kNumXcd * 4 + 4, nullptr))) return rc;
-	if ((rc = dmalloc(ctx, (void **)&gr->live_cnt, kNumXcd * 4, nullptr))) return rc;
+	if ((rc = dmalloc(ctx, (void **)&gr->live_cnt, kLiveCntBytes, nullptr))) return rc;
 	if ((rc = dmalloc(ctx, (void **)&gr->trace, (GPHI_MAX_STEPS + 1) * sizeof(uint4), nullptr))) return rc;
 	const size_t nblk = G.n_pad / kPadVerts;
 	if ((rc = dmalloc(ctx, (void **)&gr->blk_cnt, nblk * sizeof(uint4), nullptr))) return rc;
 	if ((rc = dmalloc(ctx, (void **)&gr->blk_base, nblk * sizeof(uint2), nullptr))) return rc;
 	GPHI_HIP(ctx, hipMemsetAsync(gr->flag, 0, G.n_pad, ctx->stream));
 	GPHI_HIP(ctx, hipMemsetAsync(gr->ctrl, 0, sizeof(StepCtrl), ctx->stream));
-	GPHI_HIP(ctx, hipMemsetAsync(gr->live_cnt, 0, kNumXcd * 4, ctx->stream));
+	GPHI_HIP(ctx, hipMemsetAsync(gr->live_cnt, 0, kLiveCntBytes, ctx->stream));
 	return GPHI_OK;
 }
 
@@ -634,7 +639,7 @@ __global__ __launch_bounds__(kBlock) void chunk_select_kernel(
 			s_cnt[k] = tot;
 			tot += n;
 		}
-		s_cnt[kBlock / 64] = tot ? atomicAdd(counts + blockIdx.x % kNumXcd, tot) : 0u;
+		s_cnt[kBlock / 64] = tot ? atomicAdd(counts + (blockIdx.x % kNumXcd) * kLiveCntStride, tot) : 0u;
 	}
 	__syncthreads();
 	if (live)
@@ -792,7 +797,7 @@ __global__ __launch_bounds__(kBlock) void bfs_dense_kernel(
 	const uint32_t *s_win = reinterpret_cast<const uint32_t *>(s_win4);
 	uint32_t *s_vwin = reinterpret_cast<uint32_t *>(s_vwin4);
 	const uint32_t xcd = blockIdx.x % kNumXcd, nb = gridDim.x / kNumXcd;
-	const uint32_t end = counts[xcd];
+	const uint32_t end = counts[xcd * kLiveCntStride];
 	// runs of kDenseRun consecutive list entries, run r of the XCD to block
 	// r % nb: chunks of a run share a window, while the blocks of an XCD still
 	// sweep one narrow region of the edge arrays at a time (much longer runs
@@ -937,7 +942,7 @@ __global__ __launch_bounds__(kBlock) void sssp_dense_kernel(
     uint8_t *__restrict__ flag)
 {
 	const uint32_t xcd = blockIdx.x % kNumXcd, per = gridDim.x / kNumXcd;
-	const uint32_t cnt = counts[xcd];
+	const uint32_t cnt = counts[xcd * kLiveCntStride];
 	const uint32_t *lst = lists + (uint64_t)xcd * list_cap;
 	uint32_t pos = blockIdx.x / kNumXcd;
 	if (pos >= cnt) return;
@@ -1256,7 +1261,7 @@ __global__ __launch_bounds__(kScanThreads) void frontier_scan_kernel(uint32_t nb
 		ctrl->dir = (uint32_t)(fall + dall) > threshold ? 1u : 0u;
 		ctrl->done = fall == 0;
 		uint32_t chunks = 0;  // chunks the step's dense sweep took (its live lists)
-		for (int x = 0; x < kNumXcd; ++x) chunks += live_cnt[x];
+		for (int x = 0; x < kNumXcd; ++x) chunks += live_cnt[x * kLiveCntStride];
 		ctrl->chunks = chunks;
 		if (trace) {
 			// device-driven loop: record the step, open the next one, and clear
@@ -1267,7 +1272,7 @@ __global__ __launch_bounds__(kScanThreads) void frontier_scan_kernel(uint32_t nb
 			ctrl->level = L + 1;
 			ctrl->reached += fall;
 			ctrl->traversed += dall;
-			for (int x = 0; x < kNumXcd; ++x) live_cnt[x] = 0;
+			for (int x = 0; x < kNumXcd; ++x) live_cnt[x * kLiveCntStride] = 0;
 		}
 	}
 }
@@ -1382,7 +1387,7 @@ static int frontier_loop(gphi_graph *gr, uint32_t src, gphi_stats *st)
 	GPHI_HIP(ctx, hipMemsetAsync(gr->flag, 0, G.n_pad, s));
 	if (kBfs) GPHI_HIP(ctx, hipMemsetAsync(gr->visited, 0, (size_t)G.n_words * 4, s));
 	GPHI_HIP(ctx, hipMemsetAsync(gr->cur, 0, (size_t)G.n_words * 4, s));
-	GPHI_HIP(ctx, hipMemsetAsync(gr->live_cnt, 0, kNumXcd * 4, s));
+	GPHI_HIP(ctx, hipMemsetAsync(gr->live_cnt, 0, kLiveCntBytes, s));
 	init_blocks_kernel<<<grid_for(G.side, kBlock, 65535), kBlock, 0, s>>>(G.n, G.W, G.side,
 	                                                                      gr->row_stamp, gr->col_unvis);
 	init_source_kernel<kBfs><<<1, kBlock, 0, s>>>(src, G.W, G.deg, gr->level, gr->parent, gr->dist,
