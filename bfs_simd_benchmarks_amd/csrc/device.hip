@@ -1037,7 +1037,7 @@ __global__ __launch_bounds__(kBlock) void sparse_push_kernel(
     const uint32_t *__restrict__ queue, const uint32_t *__restrict__ qoff,
     const uint32_t *__restrict__ tile_lo, const StepCtrl *__restrict__ ctrl,
     const uint32_t *__restrict__ row_off, const uint32_t *__restrict__ cols,
-    const uint32_t *__restrict__ csr_w,
+    const uint32_t *__restrict__ csr_w, const uint32_t *__restrict__ visited,
     uint8_t *__restrict__ flag, int32_t *__restrict__ parent, uint32_t *dist)
 {
 	__shared__ uint32_t s_off[kSparseTile + 2];
@@ -1074,7 +1074,9 @@ __global__ __launch_bounds__(kBlock) void sparse_push_kernel(
 			uint32_t idx = s_base[a] + (e - s_off[a]);
 			uint32_t v = cols[idx];
 			if (kBfs) {
-				if (flag[v] == 0) {
+				// the visited bitmap (4 MiB, cache-resident) first: late sparse
+				// steps mostly reach vertices visited in earlier steps
+				if (!((visited[v >> 5] >> (v & 31)) & 1u) && flag[v] == 0) {
 					parent[v] = (int32_t)u;
 					flag[v] = 1;
 				}
@@ -1418,7 +1420,7 @@ static int frontier_loop(gphi_graph *gr, uint32_t src, gphi_stats *st)
 		for (uint32_t q = 0; q < kLevelGroup; ++q, ++enq) {
 			const bool ev_step = timing && enq < GPHI_MAX_STEPS - 1;
 			sparse_push_kernel<kBfs><<<kSparseGrid, kBlock, 0, s>>>(gr->queue, gr->qoff, gr->tile_lo, gr->ctrl,
-			                                                     G.row_off, G.cols, G.csr_w,
+			                                                     G.row_off, G.cols, G.csr_w, gr->visited,
 			                                                     gr->flag, gr->parent, gr->dist);
 			chunk_select_kernel<kBfs><<<sgrid, kBlock, 0, s>>>(G.meta, G.nchunks, gr->row_stamp, gr->col_unvis,
 			                                                 gr->ctrl, gr->live, list_cap, gr->live_cnt);
