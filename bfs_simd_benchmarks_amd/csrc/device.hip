@@ -1692,7 +1692,7 @@ int run_pagerank(gphi_graph *gr, uint32_t iters, float damping, float init, floa
 // block, so the distance reads hit cache; only an edge that lowers dist[t]
 // issues an atomicMin.  Two chunks per block iteration with every load issued
 // before any is used; blocks of XCD x (blockIdx % 8) walk the chunks c with
-// c / 64 % 8 == x.  changed counts wavefronts that lowered a distance.
+// c / 64 % 8 == x.  changed is set to 1 when some edge lowered a distance.
 __global__ __launch_bounds__(kBlock) void hop_sweep_kernel(const uint32_t *__restrict__ heads,
                                                            const uint32_t *__restrict__ tails,
                                                            uint64_t nchunks, uint32_t n, uint32_t *dist,
@@ -1743,7 +1743,9 @@ __global__ __launch_bounds__(kBlock) void hop_sweep_kernel(const uint32_t *__res
 			if (e && (int32_t)nd < (int32_t)dt[j] && nd < atomicMin(dist + t[j], nd)) lowered = true;
 		}
 	}
-	if (__ballot(lowered) && lane_id() == 0) atomicAdd(changed, 1u);
+	// a plain store of 1, not an atomic: every writer stores the same value,
+	// and thousands of wavefronts would queue on one counter
+	if (__ballot(lowered) && lane_id() == 0) *changed = 1u;
 }
 
 int run_bellman_ford(gphi_graph *gr, uint32_t src, uint32_t max_sweeps, uint32_t *dist,
