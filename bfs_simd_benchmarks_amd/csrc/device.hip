@@ -215,8 +215,26 @@ __global__ __launch_bounds__(kBlock) void pack_chunks_kernel(const uint32_t *__r
 	}
 }
 
+// largest weight, then the 8-bit copy of the tile-order weights
+__global__ __launch_bounds__(kBlock) void max_u32_kernel(const uint32_t *__restrict__ a, uint64_t n, uint32_t *out)
+{
+	uint32_t m = 0;
+	for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kBlock)
+		m = max(m, a[i]);
+	for (int o = 32; o > 0; o >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, o, 64));
+	if (lane_id() == 0 && m) atomicMax(out, m);
+}
+__global__ __launch_bounds__(kBlock) void narrow_u8_kernel(const uint32_t *__restrict__ a, uint64_t n,
+                                                           uint8_t *__restrict__ b)
+{
+	for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kBlock)
+		b[i] = (uint8_t)a[i];
+}
+
 void free_chunk_summaries(DevGraph &G)
 {
+	if (G.coo_w8) G.device_bytes -= G.m_pad, (void)hipFree(G.coo_w8);
+	G.coo_w8 = nullptr;
 	if (G.meta) G.device_bytes -= G.nchunks * sizeof(ChunkMeta) + 16, (void)hipFree(G.meta);
 	if (G.pack) G.device_bytes -= G.nchunks * sizeof(ChunkPack) + 16, (void)hipFree(G.pack);
 	if (G.heads16) G.device_bytes -= G.m_pad * 2, (void)hipFree(G.heads16);
@@ -253,6 +271,20 @@ int finish_tiles(gphi_ctx *ctx, gphi_graph *gr)
 		if ((rc = dmalloc(ctx, (void **)&G.tails16, G.m_pad * 2, &G.device_bytes))) return rc;
 		pack_chunks_kernel<<<grid_for(G.nchunks, 1, 65536), kBlock, 0, s>>>(G.heads, G.tails, G.meta, G.nchunks, R,
 		                                                                   G.W, G.heads16, G.tails16, G.pack);
+	}
+	if (G.coo_w && G.nchunks) {
+		// SSSP sweeps read 1-byte weights when every weight fits
+		uint32_t *d_max = nullptr, h_max = 0;
+		if ((rc = dmalloc(ctx, (void **)&d_max, 4, nullptr))) return rc;
+		GPHI_HIP(ctx, hipMemsetAsync(d_max, 0, 4, s));
+		max_u32_kernel<<<grid_for(G.m_pad, kBlock, 4096), kBlock, 0, s>>>(G.coo_w, G.m_pad, d_max);
+		GPHI_HIP(ctx, hipMemcpyAsync(&h_max, d_max, 4, hipMemcpyDeviceToHost, s));
+		GPHI_HIP(ctx, hipStreamSynchronize(s));
+		(void)hipFree(d_max);
+		if (h_max < 256) {
+			if ((rc = dmalloc(ctx, (void **)&G.coo_w8, G.m_pad, &G.device_bytes))) return rc;
+			narrow_u8_kernel<<<grid_for(G.m_pad, kBlock, 8192), kBlock, 0, s>>>(G.coo_w, G.m_pad, G.coo_w8);
+		}
 	}
 	GPHI_HIP(ctx, hipGetLastError());
 	GPHI_HIP(ctx, hipStreamSynchronize(s));
@@ -465,7 +497,7 @@ void free_graph_device(gphi_graph *gr)
 			if (p) (void)hipFree(p);
 		return;
 	}
-	void *ps[] = {G.row_off, G.cols, G.csr_w, G.deg, G.heads, G.tails, G.coo_w, G.meta, G.pack, G.heads16, G.tails16,
+	void *ps[] = {G.row_off, G.cols, G.csr_w, G.deg, G.heads, G.tails, G.coo_w, G.coo_w8, G.meta, G.pack, G.heads16, G.tails16,
 	              gr->level, gr->parent, gr->dist, gr->flag, gr->visited, gr->cur,
 	              gr->row_stamp, gr->col_unvis, gr->queue, gr->qoff, gr->ctrl, gr->rank,
 	              gr->sum, gr->contrib, gr->tile_lo, gr->blk_cnt, gr->blk_base, gr->live, gr->live_cnt, gr->trace,
@@ -922,109 +954,150 @@ __global__ __launch_bounds__(kBlock) void bfs_dense_kernel(
 static_assert(kChunk == 2 * 4 * kBlock, "dense sweep loads two uint4 per array per thread");
 
 // SSSP relax (apps/sssp_simd/sssp.cpp:344-445): if mask[h] and dist[h]+w <
-// dist[t] then lower dist[t] and mark t for the next frontier.  The same
-// software pipeline as bfs_dense_kernel: while chunk k is relaxed, the
-// frontier bits of chunk k+1 are looked up and its tails and weights (only
-// for 4-edge quads with a frontier head) and the heads of chunk k+2 are
-// requested.  A lane that sees dist[h]+w below the dist[t] it read issues an
-// atomicMin (no return) and marks t: dist[t] then ends the step below the
-// value read during it, i.e. t improved this step, which is exactly the
-// condition for t to be in the next frontier.
-__device__ __forceinline__ void unpack8u(const uint4 a, const uint4 b, uint32_t (&x)[8])
+// dist[t] then lower dist[t] and mark t for the next frontier.  The walk of
+// bfs_dense_kernel: runs of kDenseRun list entries per block, frontier bits
+// from the LDS row window, heads and tails in their 16-bit forms where
+// exact, and a three-chunk pipeline: while chunk k is relaxed, the frontier
+// bits of chunk k+1 are looked up and its tails and weights (only for 4-edge
+// quads with a frontier head) and the heads of chunk k+2 are requested.  A
+// lane that sees dist[h]+w below the dist[t] it read issues an atomicMin (no
+// return) and marks t: dist[t] then ends the step below the value read
+// during it, i.e. t improved this step, which is exactly the condition for t
+// to be in the next frontier.
+// a lane's 8 weights of chunk c: 32-bit (8 registers) or, with kW8, the
+// 1-byte copy (2 registers, 4 weights each); quads without a frontier head
+// read the array's first bytes
+template <bool kW8>
+__device__ __forceinline__ void load_weights(const uint32_t *__restrict__ w32, const uint8_t *__restrict__ w8,
+                                             uint64_t c, uint32_t tid, bool q0, bool q1,
+                                             uint32_t (&w)[kW8 ? 2 : 8])
 {
-	x[0] = a.x, x[1] = a.y, x[2] = a.z, x[3] = a.w, x[4] = b.x, x[5] = b.y, x[6] = b.z, x[7] = b.w;
+	if constexpr (kW8) {
+		const uint32_t *A = reinterpret_cast<const uint32_t *>(w8 + c * kChunk);
+		const uint32_t *D = reinterpret_cast<const uint32_t *>(w8);
+		w[0] = q0 ? A[tid] : D[0];
+		w[1] = q1 ? A[tid + kBlock] : D[0];
+	} else {
+		load_edges(false, w32, nullptr, c, tid, q0, q1, w);
+	}
+}
+template <bool kW8>
+__device__ __forceinline__ uint32_t weight_of(const uint32_t (&w)[kW8 ? 2 : 8], int j)
+{
+	if constexpr (kW8)
+		return (w[j >> 2] >> (8 * (j & 3))) & 0xFFu;
+	else
+		return w[j];
 }
 
+template <bool kW8>
 __global__ __launch_bounds__(kBlock) void sssp_dense_kernel(
-    const uint32_t *__restrict__ heads, const uint32_t *__restrict__ tails,
-    const uint32_t *__restrict__ wts, const uint32_t *__restrict__ lists, uint32_t list_cap,
-    const uint32_t *__restrict__ counts, const uint32_t *__restrict__ cur, uint32_t *dist,
-    uint8_t *__restrict__ flag)
+    const uint32_t *__restrict__ heads, const uint32_t *__restrict__ tails, const uint16_t *__restrict__ heads16,
+    const uint16_t *__restrict__ tails16, const uint32_t *__restrict__ wts, const uint8_t *__restrict__ wts8,
+    const ChunkMeta *__restrict__ meta,
+    const ChunkPack *__restrict__ pack, const uint32_t *__restrict__ lists, uint32_t list_cap,
+    const uint32_t *__restrict__ counts, const uint32_t *__restrict__ cur, uint32_t n_words, uint32_t W, uint32_t R,
+    uint32_t *dist, uint8_t *__restrict__ flag)
 {
-	const uint32_t xcd = blockIdx.x % kNumXcd, per = gridDim.x / kNumXcd;
-	const uint32_t cnt = counts[xcd * kLiveCntStride];
+	__shared__ uint4 s_win4[kWinWords / 4];  // frontier bits of chunk k+1's rows
+	const uint32_t *s_win = reinterpret_cast<const uint32_t *>(s_win4);
+	const uint32_t xcd = blockIdx.x % kNumXcd, nb = gridDim.x / kNumXcd;
+	const uint32_t end = counts[xcd * kLiveCntStride];
+	const uint32_t skip = (nb - 1) * kDenseRun;
+	auto nxt = [&](uint32_t p) { return ((p + 1) % kDenseRun) ? p + 1 : p + 1 + skip; };
+	const uint32_t first = (blockIdx.x / kNumXcd) * kDenseRun;
+	if (first >= end) return;
 	const uint32_t *lst = lists + (uint64_t)xcd * list_cap;
-	uint32_t pos = blockIdx.x / kNumXcd;
-	if (pos >= cnt) return;
 	const uint32_t tid = threadIdx.x;
-	const uint4 *D = reinterpret_cast<const uint4 *>(tails);  // dummy: first 16 bytes
-	const uint32_t c0 = lst[pos];
-	bool v1 = pos + per < cnt, v2 = pos + 2 * per < cnt;
-	uint32_t c1 = v1 ? lst[pos + per] : c0;
-	uint32_t c2 = v2 ? lst[pos + 2 * per] : c0;
-	uint32_t h[8], hn[8], t[8], w[8];
+	const uint32_t wv = (uint32_t)__builtin_amdgcn_readfirstlane((int)(tid >> 6));
+	uint32_t win_lo = UINT32_MAX;  // first word of the window held in LDS (block-uniform)
+	const uint32_t c0 = lst[first];
+	uint32_t p2 = nxt(nxt(first));
+	bool v1 = nxt(first) < end, v2 = p2 < end;
+	uint32_t c1 = v1 ? lst[nxt(first)] : c0, c2 = v2 ? lst[p2] : c0;
+	PackSlot pk = pack_slot(pack, c0, wv), pk1 = pack_slot(pack, c1, wv);
+	uint32_t h[8], hn[8], t[8], w[kW8 ? 2 : 8];
+	load_edges(pk.flags & 1u, heads, heads16, c0, tid, true, true, h);
+	decode_edges(pk.flags & 1u, pk.hbase, pk.hbase, h);
+	uint32_t hit;
 	{
-		const uint4 *H = reinterpret_cast<const uint4 *>(heads + (uint64_t)c0 * kChunk);
-		unpack8u(ld_stream(H + tid), ld_stream(H + tid + kBlock), h);
+		const ChunkMeta m0 = meta[c0];
+		uint32_t wlo = 0, wlen = 0;
+		const bool inw = window_of(m0.row_lo, m0.row_hi, R, W, n_words, wlo, wlen);
+		if (inw) {
+			load_window(s_win4, cur, wlo, wlen, tid);
+			__syncthreads();
+			win_lo = wlo;
+		}
+		hit = frontier_bits(h, inw, s_win, wlo, wlen, cur);
 	}
-	uint32_t hit = 0;
 	{
-		uint32_t cw[8];
-#pragma unroll
-		for (int j = 0; j < 8; ++j) cw[j] = cur[h[j] >> 5];
-#pragma unroll
-		for (int j = 0; j < 8; ++j) hit |= ((cw[j] >> (h[j] & 31)) & 1u) << j;
-	}
-	{
-		const uint4 *H = reinterpret_cast<const uint4 *>(heads + (uint64_t)c1 * kChunk);
-		unpack8u(ld_stream(H + tid), ld_stream(H + tid + kBlock), hn);
-		const uint4 *T = reinterpret_cast<const uint4 *>(tails + (uint64_t)c0 * kChunk);
-		const uint4 *Wt = reinterpret_cast<const uint4 *>(wts + (uint64_t)c0 * kChunk);
-		const bool q0 = hit & 0x0Fu, q1 = hit & 0xF0u;
-		unpack8u(ld_stream(q0 ? T + tid : D), ld_stream(q1 ? T + tid + kBlock : D), t);
-		unpack8u(ld_stream(q0 ? Wt + tid : D), ld_stream(q1 ? Wt + tid + kBlock : D), w);
+		load_edges(pk1.flags & 1u, heads, heads16, c1, tid, true, true, hn);
+		load_edges(pk.flags & 2u, tails, tails16, c0, tid, hit & 0x0Fu, hit & 0xF0u, t);
+		load_weights<kW8>(wts, wts8, c0, tid, hit & 0x0Fu, hit & 0xF0u, w);
 	}
 	for (;;) {
+		// window of chunk k+1 (a block-uniform branch)
+		bool inw_n = false;
+		uint32_t wlo_n = 0, wlen_n = 0;
+		if (v1) {
+			const ChunkMeta mn = meta[c1];
+			inw_n = window_of(mn.row_lo, mn.row_hi, R, W, n_words, wlo_n, wlen_n);
+			if (inw_n && wlo_n != win_lo) {
+				__syncthreads();  // every wave is done with the old window
+				load_window(s_win4, cur, wlo_n, wlen_n, tid);
+				__syncthreads();
+				win_lo = wlo_n;
+			}
+		}
+		decode_edges(pk.flags & 2u, pk.tb0, pk.tb1, t);
+		decode_edges(pk1.flags & 1u, pk1.hbase, pk1.hbase, hn);
 		// stage A: distances at both ends of chunk k's frontier edges, frontier bits of chunk k+1
-		uint32_t dh[8], dt[8], cw[8];
+		uint32_t dh[8], dt[8];
 #pragma unroll
 		for (int j = 0; j < 8; ++j) {
 			const bool on = (hit >> j) & 1u;
 			dh[j] = dist[on ? h[j] : 0u];
 			dt[j] = dist[on ? t[j] : 0u];
 		}
+		const uint32_t hit_n = v1 ? frontier_bits(hn, inw_n, s_win, wlo_n, wlen_n, cur) : 0u;
+		uint32_t better = 0;
 #pragma unroll
-		for (int j = 0; j < 8; ++j) cw[j] = cur[hn[j] >> 5];
-		uint32_t better = 0, hit_n = 0;
-#pragma unroll
-		for (int j = 0; j < 8; ++j) better |= (uint32_t)(dh[j] + w[j] < dt[j]) << j;
+		for (int j = 0; j < 8; ++j) better |= (uint32_t)(dh[j] + weight_of<kW8>(w, j) < dt[j]) << j;
 		better &= hit;
-#pragma unroll
-		for (int j = 0; j < 8; ++j) hit_n |= ((cw[j] >> (hn[j] & 31)) & 1u) << j;
-		if (!v1) hit_n = 0;
-		// stage B: chunk k's relaxations, then the requests of the next iteration
+		// stage B: chunk k's relaxations
 #pragma unroll
 		for (int j = 0; j < 8; ++j)
 			if ((better >> j) & 1u) {
-				atomicMin(dist + t[j], dh[j] + w[j]);
+				atomicMin(dist + t[j], dh[j] + weight_of<kW8>(w, j));
 				flag[t[j]] = 1;
 			}
-		uint32_t tn[8], wn[8], h2[8];
-		{
-			const uint4 *T = reinterpret_cast<const uint4 *>(tails + (uint64_t)c1 * kChunk);
-			const uint4 *Wt = reinterpret_cast<const uint4 *>(wts + (uint64_t)c1 * kChunk);
-			const bool q0 = hit_n & 0x0Fu, q1 = hit_n & 0xF0u;
-			unpack8u(ld_stream(q0 ? T + tid : D), ld_stream(q1 ? T + tid + kBlock : D), tn);
-			unpack8u(ld_stream(q0 ? Wt + tid : D), ld_stream(q1 ? Wt + tid + kBlock : D), wn);
-			const uint4 *H = reinterpret_cast<const uint4 *>(heads + (uint64_t)c2 * kChunk);
-			unpack8u(ld_stream(H + tid), ld_stream(H + tid + kBlock), h2);
-		}
-		const bool v3 = pos + 3 * per < cnt;
-		const uint32_t c3 = v3 ? lst[pos + 3 * per] : c0;
+		// stage C: the requests of the next iteration
+		const PackSlot pk2 = pack_slot(pack, c2, wv);
+		uint32_t tn[8], wn[kW8 ? 2 : 8], h2[8];
+		load_edges(pk1.flags & 2u, tails, tails16, c1, tid, hit_n & 0x0Fu, hit_n & 0xF0u, tn);
+		load_weights<kW8>(wts, wts8, c1, tid, hit_n & 0x0Fu, hit_n & 0xF0u, wn);
+		load_edges(pk2.flags & 1u, heads, heads16, c2, tid, true, true, h2);
+		const uint32_t p3 = nxt(p2);
+		const bool v3 = p3 < end;
+		const uint32_t c3 = v3 ? lst[p3] : c0;
 		if (!v1) break;
-		pos += per;
 #pragma unroll
 		for (int j = 0; j < 8; ++j) {
 			h[j] = hn[j];
 			t[j] = tn[j];
-			w[j] = wn[j];
 			hn[j] = h2[j];
 		}
+#pragma unroll
+		for (int j = 0; j < (kW8 ? 2 : 8); ++j) w[j] = wn[j];
 		hit = hit_n;
 		c1 = c2;
 		c2 = c3;
+		pk = pk1;
+		pk1 = pk2;
 		v1 = v2;
 		v2 = v3;
+		p2 = p3;
 	}
 }
 
@@ -1411,7 +1484,9 @@ static int frontier_loop(gphi_graph *gr, uint32_t src, gphi_stats *st)
 	// every block of the sweep resident at once (blocks of a run-interleaved
 	// list walk all have the same work), a multiple of the XCD count
 	static const unsigned resident = resident_grid(reinterpret_cast<const void *>(bfs_dense_kernel));
-	const unsigned dense_cap = kBfs ? resident : kBfsDenseGrid;
+	static const unsigned resident_s8 = resident_grid(reinterpret_cast<const void *>(sssp_dense_kernel<true>));
+	static const unsigned resident_s32 = resident_grid(reinterpret_cast<const void *>(sssp_dense_kernel<false>));
+	const unsigned dense_cap = kBfs ? resident : G.coo_w8 ? resident_s8 : resident_s32;
 	unsigned grid = (unsigned)std::min<uint64_t>(dense_cap, (G.nchunks + kNumXcd - 1) / kNumXcd * kNumXcd);
 	grid = std::max<unsigned>(grid / kNumXcd * kNumXcd, kNumXcd);
 	const uint32_t win_rows = dense_window_rows(G.W);
@@ -1431,8 +1506,16 @@ static int frontier_loop(gphi_graph *gr, uint32_t src, gphi_stats *st)
 				                                       gr->cur, gr->visited, G.n_words, G.W, win_rows, gr->flag,
 				                                       gr->parent);
 			else
-				sssp_dense_kernel<<<grid, kBlock, 0, s>>>(G.heads, G.tails, G.coo_w, gr->live, list_cap,
-				                                        gr->live_cnt, gr->cur, gr->dist, gr->flag);
+			{
+				if (G.coo_w8)
+					sssp_dense_kernel<true><<<grid, kBlock, 0, s>>>(
+					    G.heads, G.tails, G.heads16, G.tails16, G.coo_w, G.coo_w8, G.meta, G.pack, gr->live, list_cap,
+					    gr->live_cnt, gr->cur, G.n_words, G.W, win_rows, gr->dist, gr->flag);
+				else
+					sssp_dense_kernel<false><<<grid, kBlock, 0, s>>>(
+					    G.heads, G.tails, G.heads16, G.tails16, G.coo_w, G.coo_w8, G.meta, G.pack, gr->live, list_cap,
+					    gr->live_cnt, gr->cur, G.n_words, G.W, win_rows, gr->dist, gr->flag);
+			}
 			if (ev_step) GPHI_HIP(ctx, hipEventRecord(ev[2 + 2 * enq], s));
 			frontier_mark_kernel<kBfs><<<nblk, kBlock, 0, s>>>(G.W, kLevelFromCtrl, gr->flag, G.deg, gr->level,
 			                                                  gr->visited, gr->cur, gr->row_stamp, gr->col_unvis,

@@ -91,6 +91,7 @@ struct DevGraph {
 	ChunkMeta *meta = nullptr;
 	uint16_t *heads16 = nullptr, *tails16 = nullptr;  // or null when no window fits (W too wide)
 	ChunkPack *pack = nullptr;
+	uint8_t *coo_w8 = nullptr;  // 8-bit tile-order weights when every weight is below 256
 	uint64_t device_bytes = 0;
 };
 
