@@ -1154,10 +1154,13 @@ __global__ __launch_bounds__(kBlock) void sparse_push_kernel(
 					flag[v] = 1;
 				}
 			} else {
-				uint32_t nd = dist[u] + csr_w[idx];
+				// as in the dense sweep: an edge that beats the dist[v] it read
+				// lowers it (no return needed) and marks v, which then ends the
+				// step below a value read during it, i.e. improved
+				const uint32_t nd = dist[u] + csr_w[idx];
 				if (nd < dist[v]) {
-					uint32_t old = atomicMin(dist + v, nd);
-					if (nd < old && !flag[v]) flag[v] = 1;
+					atomicMin(dist + v, nd);
+					flag[v] = 1;
 				}
 			}
 		}
