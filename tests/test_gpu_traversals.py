@@ -152,6 +152,23 @@ def test_bellman_ford_unbounded_sweeps(ctx, c1):
     assert np.array_equal(d, O.bellman_ford_hops(c1.n, c1.rh, c1.rt, 0, 99)) and run < 99
 
 
+@pytest.mark.parametrize("src", [0, 999])
+def test_sssp_wide_weights(ctx, c3, src):
+    # weights above 255: the sweep reads the 32-bit weights (no 1-byte copy)
+    import dataclasses
+    scale = np.uint32(4099)
+    g = dataclasses.replace(c3, cw=c3.cw * scale, sw=c3.sw * scale, rw=c3.rw * scale)
+    print(f"case: sssp c3 x{int(scale)} src={src}", flush=True)
+    gg = upload(ctx, g, weighted=True)
+    try:
+        d, st = gg.sssp(src)
+    finally:
+        gg.free()
+    ref = O.sssp_dijkstra(g.n, g.deg, g.cols, g.cw, src)
+    assert int(g.cw.max()) > 255
+    assert np.array_equal(d, ref), f"dist differ at {np.flatnonzero(d != ref)[:10]}"
+
+
 def test_bfs_level_timing_off(ctx, c1):
     # per-level event timing off: same levels and trace, only the total time
     gg = upload(ctx, c1)
