@@ -1431,7 +1431,7 @@ static unsigned resident_grid(const void *kernel)
 	return std::max<unsigned>((unsigned)(ncu * per) / kNumXcd * kNumXcd, kNumXcd);
 }
 static constexpr unsigned kSparseGrid = 2048;
-static constexpr uint32_t kLevelGroup = 4;  // levels enqueued per read-back of the control block
+static constexpr uint32_t kLevelGroup = 8;  // levels enqueued per read-back of the control block
 
 static float ev_ms(hipEvent_t a, hipEvent_t b)
 {
