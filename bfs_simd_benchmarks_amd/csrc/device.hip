@@ -554,6 +554,33 @@ static int ensure_state(gphi_graph *gr)
 	return GPHI_OK;
 }
 
+// One launch for the per-traversal clears: state bytes, visited and frontier
+// bitmaps, live-list counters, and the tile-row stamps / column counts
+// (col_unvis[c] = vertices of tail block c; row_stamp[r] = none).
+__global__ __launch_bounds__(kBlock) void init_state_kernel(uint32_t n, uint32_t W, uint32_t side, uint32_t n_pad,
+                                                            uint32_t n_words, uint4 *__restrict__ flag16,
+                                                            uint32_t *__restrict__ visited, uint32_t *__restrict__ cur,
+                                                            uint32_t *__restrict__ live_cnt,
+                                                            uint32_t *__restrict__ row_stamp,
+                                                            uint32_t *__restrict__ col_unvis)
+{
+	const uint32_t nf = n_pad / 16;  // n_pad is a multiple of 8192
+	const uint32_t nl = (uint32_t)(kLiveCntBytes / 4);
+	for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < max(max(nf, n_words), max(side, nl));
+	     i += gridDim.x * kBlock) {
+		if (i < nf) flag16[i] = make_uint4(0, 0, 0, 0);
+		if (i < n_words) {
+			cur[i] = 0;
+			if (visited) visited[i] = 0;
+		}
+		if (i < side) {
+			row_stamp[i] = UINT32_MAX;
+			col_unvis[i] = (i + 1 == side) ? n - i * W : W;
+		}
+		if (i < nl) live_cnt[i] = 0;
+	}
+}
+
 // col_unvis[c] = vertices of tail block c; row_stamp[r] = none
 __global__ void init_blocks_kernel(uint32_t n, uint32_t W, uint32_t side, uint32_t *row_stamp,
                                    uint32_t *col_unvis)
@@ -1462,12 +1489,12 @@ static int frontier_loop(gphi_graph *gr, uint32_t src, gphi_stats *st)
 	// read during the traversal: the -1 of unreached ones is filled in by
 	// bfs_finalize_kernel when they are copied out.
 	if (!kBfs) fill_u32_kernel<<<grid_for(G.n_pad, kBlock, 4096), kBlock, 0, s>>>(gr->dist, G.n_pad, INT_MAX);
-	GPHI_HIP(ctx, hipMemsetAsync(gr->flag, 0, G.n_pad, s));
-	if (kBfs) GPHI_HIP(ctx, hipMemsetAsync(gr->visited, 0, (size_t)G.n_words * 4, s));
-	GPHI_HIP(ctx, hipMemsetAsync(gr->cur, 0, (size_t)G.n_words * 4, s));
-	GPHI_HIP(ctx, hipMemsetAsync(gr->live_cnt, 0, kLiveCntBytes, s));
-	init_blocks_kernel<<<grid_for(G.side, kBlock, 65535), kBlock, 0, s>>>(G.n, G.W, G.side,
-	                                                                      gr->row_stamp, gr->col_unvis);
+	{
+		const uint64_t items = std::max<uint64_t>({G.n_pad / 16, G.n_words, G.side, kLiveCntBytes / 4});
+		init_state_kernel<<<grid_for(items, kBlock, 8192), kBlock, 0, s>>>(
+		    G.n, G.W, G.side, G.n_pad, G.n_words, reinterpret_cast<uint4 *>(gr->flag), kBfs ? gr->visited : nullptr,
+		    gr->cur, gr->live_cnt, gr->row_stamp, gr->col_unvis);
+	}
 	init_source_kernel<kBfs><<<1, kBlock, 0, s>>>(src, G.W, G.deg, gr->level, gr->parent, gr->dist,
 	                                               gr->flag, gr->visited, gr->cur, gr->row_stamp, gr->col_unvis,
 	                                               gr->queue, gr->qoff, gr->tile_lo, gr->ctrl, gr->trace);
