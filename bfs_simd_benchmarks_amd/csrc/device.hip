@@ -475,7 +475,7 @@ void free_traversal_state(gphi_graph *gr)
 	void **ps[] = {(void **)&gr->level, (void **)&gr->parent, (void **)&gr->dist, (void **)&gr->flag,
 	               (void **)&gr->visited, (void **)&gr->cur, (void **)&gr->row_stamp, (void **)&gr->col_unvis,
 	               (void **)&gr->queue, (void **)&gr->qoff, (void **)&gr->tile_lo, (void **)&gr->ctrl,
-	               (void **)&gr->blk_cnt, (void **)&gr->blk_base, (void **)&gr->live, (void **)&gr->live_cnt,
+	               (void **)&gr->blk_cnt, (void **)&gr->blk_base, (void **)&gr->wave_cnt, (void **)&gr->live, (void **)&gr->live_cnt,
 	               (void **)&gr->trace,
 	               (void **)&gr->rank, (void **)&gr->sum, (void **)&gr->contrib, (void **)&gr->b_front,
 	               (void **)&gr->b_seen, (void **)&gr->b_next, (void **)&gr->b_level, (void **)&gr->b_sums,
@@ -500,7 +500,7 @@ void free_graph_device(gphi_graph *gr)
 	void *ps[] = {G.row_off, G.cols, G.csr_w, G.deg, G.heads, G.tails, G.coo_w, G.coo_w8, G.meta, G.pack, G.heads16, G.tails16,
 	              gr->level, gr->parent, gr->dist, gr->flag, gr->visited, gr->cur,
 	              gr->row_stamp, gr->col_unvis, gr->queue, gr->qoff, gr->ctrl, gr->rank,
-	              gr->sum, gr->contrib, gr->tile_lo, gr->blk_cnt, gr->blk_base, gr->live, gr->live_cnt, gr->trace,
+	              gr->sum, gr->contrib, gr->tile_lo, gr->blk_cnt, gr->blk_base, gr->wave_cnt, gr->live, gr->live_cnt, gr->trace,
 	              gr->b_front, gr->b_seen, gr->b_next, gr->b_level, gr->b_sums, gr->b_live,
 	              gr->in_off, gr->in_src, gr->pieces};
 	for (void *p : ps)
@@ -548,6 +548,7 @@ static int ensure_state(gphi_graph *gr)
 	const size_t nblk = G.n_pad / kPadVerts;
 	if ((rc = dmalloc(ctx, (void **)&gr->blk_cnt, nblk * sizeof(uint4), nullptr))) return rc;
 	if ((rc = dmalloc(ctx, (void **)&gr->blk_base, nblk * sizeof(uint2), nullptr))) return rc;
+	if ((rc = dmalloc(ctx, (void **)&gr->wave_cnt, nblk * (kBlock / 64) * sizeof(uint2), nullptr))) return rc;
 	GPHI_HIP(ctx, hipMemsetAsync(gr->flag, 0, G.n_pad, ctx->stream));
 	GPHI_HIP(ctx, hipMemsetAsync(gr->ctrl, 0, sizeof(StepCtrl), ctx->stream));
 	GPHI_HIP(ctx, hipMemsetAsync(gr->live_cnt, 0, kLiveCntBytes, ctx->stream));
@@ -1214,7 +1215,7 @@ __global__ __launch_bounds__(kBlock) void frontier_mark_kernel(
     uint32_t W, uint32_t next_level, uint8_t *__restrict__ flag, const uint32_t *__restrict__ deg,
     int32_t *__restrict__ level, uint32_t *__restrict__ visited, uint32_t *__restrict__ cur,
     uint32_t *__restrict__ row_stamp, uint32_t *__restrict__ col_unvis, uint4 *__restrict__ blk_cnt,
-    const StepCtrl *__restrict__ ctrl)
+    uint2 *__restrict__ wave_cnt, const StepCtrl *__restrict__ ctrl)
 {
 	__shared__ uint32_t s_red[3][kBlock / 64];
 	if (next_level == kLevelFromCtrl) {  // device-driven loop
@@ -1298,6 +1299,7 @@ __global__ __launch_bounds__(kBlock) void frontier_mark_kernel(
 		s_red[0][wid] = F;
 		s_red[1][wid] = Q;
 		s_red[2][wid] = D;
+		wave_cnt[blockIdx.x * (kBlock / 64) + wid] = make_uint2(Q, D);
 	}
 	__syncthreads();
 	if (threadIdx.x == 0) {
@@ -1384,60 +1386,47 @@ __global__ __launch_bounds__(kScanThreads) void frontier_scan_kernel(uint32_t nb
 
 __global__ __launch_bounds__(kBlock) void frontier_emit_kernel(
     const StepCtrl *__restrict__ ctrl, const uint32_t *__restrict__ deg, const uint32_t *__restrict__ cur,
-    const uint4 *__restrict__ blk_cnt, const uint2 *__restrict__ blk_base,
+    const uint4 *__restrict__ blk_cnt, const uint2 *__restrict__ blk_base, const uint2 *__restrict__ wave_cnt,
     uint32_t *__restrict__ queue, uint32_t *__restrict__ qoff, uint32_t *__restrict__ tile_lo)
 {
-	__shared__ uint32_t sq[kBlock / 64], sd[kBlock / 64];
 	// the queue only feeds a sparse step; empty blocks have nothing to emit
 	if (ctrl->done || ctrl->dir != 0 || blk_cnt[blockIdx.x].y == 0) return;
 	const uint32_t w = blockIdx.x * kBlock + threadIdx.x;
-	const uint32_t v0 = w * 32;
+	const uint32_t lane = lane_id(), wid = threadIdx.x >> 6;
 	const uint32_t bits = cur[w];
-	uint32_t q = 0, d = 0;
-	uint32_t dd[32];
-	if (bits) {
-		const uint4 *dp = reinterpret_cast<const uint4 *>(deg + v0);
-#pragma unroll
-		for (int j = 0; j < 8; ++j) {
-			const uint4 x = dp[j];
-			dd[4 * j] = x.x;
-			dd[4 * j + 1] = x.y;
-			dd[4 * j + 2] = x.z;
-			dd[4 * j + 3] = x.w;
-		}
-#pragma unroll
-		for (int i = 0; i < 32; ++i) {
-			const uint32_t dv = (bits >> i & 1u) ? dd[i] : 0u;
-			q += dv > 0;
-			d += dv;
-		}
-	}
-	uint32_t qt, dt;
-	uint32_t qx = wave_excl_scan(q, &qt), dx = wave_excl_scan(d, &dt);
-	const uint32_t wid = threadIdx.x >> 6;
-	if (lane_id() == 0) {
-		sq[wid] = qt;
-		sd[wid] = dt;
-	}
-	__syncthreads();
+	// this wavefront's first queue entry and degree prefix: the block's, plus
+	// the totals the mark kernel recorded for the block's earlier wavefronts
+	uint2 base = blk_base[blockIdx.x];
 	for (uint32_t k = 0; k < wid; ++k) {
-		qx += sq[k];
-		dx += sd[k];
+		const uint2 c = wave_cnt[blockIdx.x * (kBlock / 64) + k];
+		base.x += c.x;
+		base.y += c.y;
 	}
-	const uint2 base = blk_base[blockIdx.x];
-	uint32_t pos = base.x + qx, off = base.y + dx;
-#pragma unroll
-	for (int i = 0; i < 32; ++i) {
-		if (!(bits >> i & 1u)) continue;
-		const uint32_t dv = dd[i];
-		if (!dv) continue;
-		queue[pos] = v0 + i;
-		qoff[pos] = off;
-		// tiles of the next push whose first edge falls in [off, off+dv)
-		for (uint32_t k = (off + kSparseTile - 1) / kSparseTile; (uint64_t)k * kSparseTile < (uint64_t)off + dv; ++k)
-			tile_lo[k] = pos;
-		++pos;
-		off += dv;
+	// vertex-major over the wave's 2048 vertices, as in the mark kernel: in
+	// iteration i lane l takes vertex 64i + l, so the entries come out in
+	// ascending vertex order and each store covers consecutive entries
+	const uint32_t wv0 = (w - lane) * 32;
+	const unsigned long long any = __ballot(bits != 0);
+	const unsigned long long below = (1ull << lane) - 1ull;
+	uint32_t pos = base.x, off = base.y;
+	for (uint32_t i = 0; i < 32; ++i) {
+		const uint32_t wb = (uint32_t)__shfl((int)bits, (int)(2 * i + (lane >> 5)), 64);
+		if (!((any >> (2 * i)) & 3ull)) continue;
+		const uint32_t v = wv0 + 64 * i + lane;
+		const uint32_t dv = ((wb >> (lane & 31)) & 1u) ? deg[v] : 0u;
+		const unsigned long long qm = __ballot(dv > 0);
+		uint32_t dtot;
+		const uint32_t dx = wave_excl_scan(dv, &dtot);
+		if (dv) {
+			const uint32_t p = pos + (uint32_t)__popcll(qm & below), o = off + dx;
+			queue[p] = v;
+			qoff[p] = o;
+			// tiles of the next push whose first edge falls in [o, o+dv)
+			for (uint32_t k = (o + kSparseTile - 1) / kSparseTile; (uint64_t)k * kSparseTile < (uint64_t)o + dv; ++k)
+				tile_lo[k] = p;
+		}
+		pos += (uint32_t)__popcll(qm);
+		off += dtot;
 	}
 }
 
@@ -1549,11 +1538,11 @@ static int frontier_loop(gphi_graph *gr, uint32_t src, gphi_stats *st)
 			if (ev_step) GPHI_HIP(ctx, hipEventRecord(ev[2 + 2 * enq], s));
 			frontier_mark_kernel<kBfs><<<nblk, kBlock, 0, s>>>(G.W, kLevelFromCtrl, gr->flag, G.deg, gr->level,
 			                                                  gr->visited, gr->cur, gr->row_stamp, gr->col_unvis,
-			                                                  gr->blk_cnt, gr->ctrl);
+			                                                  gr->blk_cnt, gr->wave_cnt, gr->ctrl);
 			frontier_scan_kernel<<<1, kScanThreads, 0, s>>>(nblk, threshold, gr->blk_cnt, gr->blk_base,
 			                                              gr->live_cnt, gr->ctrl, gr->trace);
 			frontier_emit_kernel<<<nblk, kBlock, 0, s>>>(gr->ctrl, G.deg, gr->cur, gr->blk_cnt, gr->blk_base,
-			                                            gr->queue, gr->qoff, gr->tile_lo);
+			                                            gr->wave_cnt, gr->queue, gr->qoff, gr->tile_lo);
 			if (ev_step) GPHI_HIP(ctx, hipEventRecord(ev[3 + 2 * enq], s));
 		}
 		GPHI_HIP(ctx, hipGetLastError());
@@ -1625,11 +1614,11 @@ void launch_frontier_update_flags(gphi_graph *gr, uint32_t next_level, uint32_t 
 	const unsigned nblk = G.n_pad / kPadVerts;
 	frontier_mark_kernel<false><<<nblk, kBlock, 0, s>>>(G.W, next_level, gr->flag, G.deg, gr->level,
 	                                                   gr->visited, gr->cur, gr->row_stamp, gr->col_unvis,
-	                                                   gr->blk_cnt, gr->ctrl);
+	                                                   gr->blk_cnt, gr->wave_cnt, gr->ctrl);
 	frontier_scan_kernel<<<1, kScanThreads, 0, s>>>(nblk, threshold, gr->blk_cnt, gr->blk_base, gr->live_cnt,
 	                                              gr->ctrl, nullptr);
 	frontier_emit_kernel<<<nblk, kBlock, 0, s>>>(gr->ctrl, G.deg, gr->cur, gr->blk_cnt, gr->blk_base,
-	                                            gr->queue, gr->qoff, gr->tile_lo);
+	                                            gr->wave_cnt, gr->queue, gr->qoff, gr->tile_lo);
 }
 
 // level and parent -1 for the vertices a BFS did not reach (state 0; every

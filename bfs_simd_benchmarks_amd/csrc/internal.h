@@ -131,6 +131,7 @@ struct gphi_graph {
 	gphi::StepCtrl *ctrl = nullptr;
 	uint4 *blk_cnt = nullptr;  // per update block: |F|, queue entries, degree sum
 	uint2 *blk_base = nullptr; // per update block: queue and degree prefix
+	uint2 *wave_cnt = nullptr; // per update wavefront: queue entries, degree sum
 	uint32_t *live = nullptr;  // per-XCD lists of live chunks of a dense step
 	uint32_t *live_cnt = nullptr;
 	uint4 *trace = nullptr;    // per step: direction, |F|, sum deg(F), chunks swept
