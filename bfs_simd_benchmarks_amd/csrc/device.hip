@@ -1794,12 +1794,17 @@ int run_pagerank(gphi_graph *gr, uint32_t iters, float damping, float init, floa
 // block, so the distance reads hit cache; only an edge that lowers dist[t]
 // issues an atomicMin.  Two chunks per block iteration with every load issued
 // before any is used; blocks of XCD x (blockIdx % 8) walk the chunks c with
-// c / 64 % 8 == x.  changed is set to 1 when some edge lowered a distance.
+// c / 64 % 8 == x.  Heads and tails are read in their 16-bit forms where
+// exact.  changed is set to 1 when some edge lowered a distance.
 __global__ __launch_bounds__(kBlock) void hop_sweep_kernel(const uint32_t *__restrict__ heads,
                                                            const uint32_t *__restrict__ tails,
+                                                           const uint16_t *__restrict__ heads16,
+                                                           const uint16_t *__restrict__ tails16,
+                                                           const ChunkPack *__restrict__ pack,
                                                            uint64_t nchunks, uint32_t n, uint32_t *dist,
                                                            uint32_t *changed)
 {
+	const uint32_t wv = (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
 	const uint32_t xcd = blockIdx.x % kNumXcd, slot = blockIdx.x / kNumXcd, per = gridDim.x / kNumXcd;
 	const uint64_t groups = (nchunks + 63) / 64;
 	const uint64_t mine = groups / kNumXcd + (xcd < groups % kNumXcd ? 1 : 0);  // groups g = xcd + 8i
@@ -1817,13 +1822,15 @@ __global__ __launch_bounds__(kBlock) void hop_sweep_kernel(const uint32_t *__res
 		uint32_t h[16], t[16];
 #pragma unroll
 		for (int q = 0; q < 2; ++q) {
+			// heads and tails in their 16-bit forms where exact (pack is null
+			// when the graph has none)
 			const uint64_t c = ok[q] ? cc[q] : 0;
-			const uint4 *H = reinterpret_cast<const uint4 *>(heads + c * kChunk);
-			const uint4 *T = reinterpret_cast<const uint4 *>(tails + c * kChunk);
-			const uint4 a = ld_stream(H + threadIdx.x), b = ld_stream(H + threadIdx.x + kBlock);
-			const uint4 x = ld_stream(T + threadIdx.x), y = ld_stream(T + threadIdx.x + kBlock);
-			const uint32_t hv[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
-			const uint32_t tv[8] = {x.x, x.y, x.z, x.w, y.x, y.y, y.z, y.w};
+			const PackSlot pk = pack ? pack_slot(pack, (uint32_t)c, wv) : PackSlot{0u, 0u, 0u, 0u};
+			uint32_t hv[8], tv[8];
+			load_edges(pk.flags & 1u, heads, heads16, c, threadIdx.x, true, true, hv);
+			load_edges(pk.flags & 2u, tails, tails16, c, threadIdx.x, true, true, tv);
+			decode_edges(pk.flags & 1u, pk.hbase, pk.hbase, hv);
+			decode_edges(pk.flags & 2u, pk.tb0, pk.tb1, tv);
 #pragma unroll
 			for (int j = 0; j < 8; ++j) {
 				// sentinel edges (head n) and absent chunks read vertex 0 and are masked
@@ -1878,7 +1885,9 @@ int run_bellman_ford(gphi_graph *gr, uint32_t src, uint32_t max_sweeps, uint32_t
 		const uint32_t nb = std::min<uint32_t>(kBatch, max_sweeps - run);
 		hipError_t e = hipMemsetAsync(counts, 0, kBatch * 4, s);
 		for (uint32_t k = 0; k < nb && e == hipSuccess; ++k) {
-			hop_sweep_kernel<<<grid, kBlock, 0, s>>>(G.heads, G.tails, G.nchunks, G.n, gr->dist, counts + k);
+			hop_sweep_kernel<<<grid, kBlock, 0, s>>>(G.heads, G.tails, G.heads16, G.tails16,
+			                                       G.heads16 ? G.pack : nullptr, G.nchunks, G.n, gr->dist,
+			                                       counts + k);
 			e = hipGetLastError();
 		}
 		if (e == hipSuccess) e = hipMemcpyAsync(hcnt, counts, nb * 4, hipMemcpyDeviceToHost, s);
