@@ -1161,7 +1161,46 @@ __global__ __launch_bounds__(kBlock) void sparse_push_kernel(
 		__syncthreads();
 		uint32_t e0 = k * kSparseTile;
 		uint32_t e1 = min(e0 + kSparseTile, qdeg);
-		for (uint32_t e = e0 + threadIdx.x; e < e1; e += kBlock) {
+		if (kBfs) {
+			// the tile's 4 edges per lane in batches of dependent loads (tails,
+			// then visited bits, then state bytes), so a lane waits on three
+			// round trips per tile instead of twelve; the visited bitmap
+			// (4 MiB, cache-resident) first: late sparse steps mostly reach
+			// vertices visited in earlier steps
+			constexpr int kPer = kSparseTile / kBlock;
+			static_assert(kSparseTile % kBlock == 0, "whole edges per lane");
+			uint32_t uu[kPer], vv[kPer], vw[kPer], sv[kPer];
+			bool on[kPer];
+#pragma unroll
+			for (int r = 0; r < kPer; ++r) {
+				const uint32_t e = e0 + threadIdx.x + r * kBlock;
+				on[r] = e < e1;
+				uint32_t a = 0, b = cnt;  // largest j with s_off[j] <= e
+				while (b - a > 1) {
+					const uint32_t mid = (a + b) >> 1;
+					if (s_off[mid] <= e)
+						a = mid;
+					else
+						b = mid;
+				}
+				uu[r] = s_u[a];
+				vv[r] = on[r] ? cols[s_base[a] + (e - s_off[a])] : 0u;
+			}
+#pragma unroll
+			for (int r = 0; r < kPer; ++r) vw[r] = on[r] ? visited[vv[r] >> 5] : 0xFFFFFFFFu;
+#pragma unroll
+			for (int r = 0; r < kPer; ++r) {
+				on[r] = on[r] && !((vw[r] >> (vv[r] & 31)) & 1u);
+				sv[r] = on[r] ? flag[vv[r]] : 1u;
+			}
+#pragma unroll
+			for (int r = 0; r < kPer; ++r)
+				if (on[r] && sv[r] == 0) {
+					parent[vv[r]] = (int32_t)uu[r];
+					flag[vv[r]] = 1;
+				}
+		}
+		for (uint32_t e = e0 + threadIdx.x; !kBfs && e < e1; e += kBlock) {
 			// largest j with s_off[j] <= e
 			uint32_t a = 0, b = cnt;
 			while (b - a > 1) {
@@ -1174,14 +1213,7 @@ __global__ __launch_bounds__(kBlock) void sparse_push_kernel(
 			uint32_t u = s_u[a];
 			uint32_t idx = s_base[a] + (e - s_off[a]);
 			uint32_t v = cols[idx];
-			if (kBfs) {
-				// the visited bitmap (4 MiB, cache-resident) first: late sparse
-				// steps mostly reach vertices visited in earlier steps
-				if (!((visited[v >> 5] >> (v & 31)) & 1u) && flag[v] == 0) {
-					parent[v] = (int32_t)u;
-					flag[v] = 1;
-				}
-			} else {
+			{
 				// as in the dense sweep: an edge that beats the dist[v] it read
 				// lowers it (no return needed) and marks v, which then ends the
 				// step below a value read during it, i.e. improved
