@@ -645,8 +645,8 @@ __global__ void init_source_kernel(uint32_t src, uint32_t W, const uint32_t *__r
 // vertex.  Live chunks are appended to one list per XCD (the XCD that runs
 // blocks b with b % 8 == x), 64 consecutive chunks at a time.
 template <bool kColSkip>
-__global__ __launch_bounds__(kBlock) void chunk_select_kernel(
-    const ChunkMeta *__restrict__ meta, uint64_t nchunks, const uint32_t *__restrict__ row_stamp,
+__device__ __forceinline__ void chunk_select_block(
+    uint32_t bid, const ChunkMeta *__restrict__ meta, uint64_t nchunks, const uint32_t *__restrict__ row_stamp,
     const uint32_t *__restrict__ col_unvis, const StepCtrl *__restrict__ ctrl, uint32_t *__restrict__ lists,
     uint32_t list_cap, uint32_t *__restrict__ counts)
 {
@@ -654,7 +654,7 @@ __global__ __launch_bounds__(kBlock) void chunk_select_kernel(
 	// stay 0, so the dense sweep launched after it returns at once
 	if (ctrl->done || ctrl->dir != 1) return;
 	const uint32_t level = ctrl->level;
-	const uint64_t c = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+	const uint64_t c = (uint64_t)bid * kBlock + threadIdx.x;
 	bool live = false;
 	if (c < nchunks) {
 		const ChunkMeta md = meta[c];
@@ -699,11 +699,11 @@ __global__ __launch_bounds__(kBlock) void chunk_select_kernel(
 			s_cnt[k] = tot;
 			tot += n;
 		}
-		s_cnt[kBlock / 64] = tot ? atomicAdd(counts + (blockIdx.x % kNumXcd) * kLiveCntStride, tot) : 0u;
+		s_cnt[kBlock / 64] = tot ? atomicAdd(counts + (bid % kNumXcd) * kLiveCntStride, tot) : 0u;
 	}
 	__syncthreads();
 	if (live)
-		lists[(uint64_t)(blockIdx.x % kNumXcd) * list_cap + s_cnt[kBlock / 64] + s_cnt[wid] +
+		lists[(uint64_t)(bid % kNumXcd) * list_cap + s_cnt[kBlock / 64] + s_cnt[wid] +
 		      __popcll(m & ((1ull << lane_id()) - 1ull))] = (uint32_t)c;
 }
 
@@ -1134,8 +1134,8 @@ __global__ __launch_bounds__(kBlock) void sssp_dense_kernel(
 // queue entry (tile_lo) was written by the previous frontier update.
 // ---------------------------------------------------------------------------
 template <bool kBfs>
-__global__ __launch_bounds__(kBlock) void sparse_push_kernel(
-    const uint32_t *__restrict__ queue, const uint32_t *__restrict__ qoff,
+__device__ __forceinline__ void sparse_push_block(
+    uint32_t bid, uint32_t nbid, const uint32_t *__restrict__ queue, const uint32_t *__restrict__ qoff,
     const uint32_t *__restrict__ tile_lo, const StepCtrl *__restrict__ ctrl,
     const uint32_t *__restrict__ row_off, const uint32_t *__restrict__ cols,
     const uint32_t *__restrict__ csr_w, const uint32_t *__restrict__ visited,
@@ -1147,7 +1147,7 @@ __global__ __launch_bounds__(kBlock) void sparse_push_kernel(
 	if (ctrl->done || ctrl->dir != 0) return;  // a dense step, or a finished traversal
 	const uint32_t qsize = ctrl->qsize, qdeg = ctrl->fdeg;
 	uint32_t ntile = (qdeg + kSparseTile - 1) / kSparseTile;
-	for (uint32_t k = blockIdx.x; k < ntile; k += gridDim.x) {
+	for (uint32_t k = bid; k < ntile; k += nbid) {
 		uint32_t lo = tile_lo[k];
 		uint32_t hi = (k + 1 < ntile) ? tile_lo[k + 1] : qsize - 1;
 		if (hi > qsize - 1) hi = qsize - 1;
@@ -1225,6 +1225,30 @@ __global__ __launch_bounds__(kBlock) void sparse_push_kernel(
 			}
 		}
 		__syncthreads();
+	}
+}
+
+
+// The first kernel of a level: the CSR push of a sparse step on blocks
+// [0, npush), or the chunk selection of a dense step on blocks [0, nsel)
+// (block-uniform branches), so a level does not launch the one it does not
+// need.
+template <bool kBfs>
+__global__ __launch_bounds__(kBlock) void level_front_kernel(
+    uint32_t npush, uint32_t nsel, const uint32_t *__restrict__ queue, const uint32_t *__restrict__ qoff,
+    const uint32_t *__restrict__ tile_lo, const StepCtrl *__restrict__ ctrl, const uint32_t *__restrict__ row_off,
+    const uint32_t *__restrict__ cols, const uint32_t *__restrict__ csr_w, const uint32_t *__restrict__ visited,
+    uint8_t *__restrict__ flag, int32_t *__restrict__ parent, uint32_t *dist, const ChunkMeta *__restrict__ meta,
+    uint64_t nchunks, const uint32_t *__restrict__ row_stamp, const uint32_t *__restrict__ col_unvis,
+    uint32_t *__restrict__ lists, uint32_t list_cap, uint32_t *__restrict__ counts)
+{
+	if (ctrl->done) return;
+	if (ctrl->dir == 0) {
+		if (blockIdx.x < npush)
+			sparse_push_block<kBfs>(blockIdx.x, npush, queue, qoff, tile_lo, ctrl, row_off, cols, csr_w, visited,
+			                        flag, parent, dist);
+	} else if (blockIdx.x < nsel) {
+		chunk_select_block<kBfs>(blockIdx.x, meta, nchunks, row_stamp, col_unvis, ctrl, lists, list_cap, counts);
 	}
 }
 
@@ -1545,11 +1569,10 @@ static int frontier_loop(gphi_graph *gr, uint32_t src, gphi_stats *st)
 	for (;;) {
 		for (uint32_t q = 0; q < kLevelGroup; ++q, ++enq) {
 			const bool ev_step = timing && enq < GPHI_MAX_STEPS - 1;
-			sparse_push_kernel<kBfs><<<kSparseGrid, kBlock, 0, s>>>(gr->queue, gr->qoff, gr->tile_lo, gr->ctrl,
-			                                                     G.row_off, G.cols, G.csr_w, gr->visited,
-			                                                     gr->flag, gr->parent, gr->dist);
-			chunk_select_kernel<kBfs><<<sgrid, kBlock, 0, s>>>(G.meta, G.nchunks, gr->row_stamp, gr->col_unvis,
-			                                                 gr->ctrl, gr->live, list_cap, gr->live_cnt);
+			level_front_kernel<kBfs><<<std::max(kSparseGrid, sgrid), kBlock, 0, s>>>(
+			    kSparseGrid, sgrid, gr->queue, gr->qoff, gr->tile_lo, gr->ctrl, G.row_off, G.cols, G.csr_w,
+			    gr->visited, gr->flag, gr->parent, gr->dist, G.meta, G.nchunks, gr->row_stamp, gr->col_unvis,
+			    gr->live, list_cap, gr->live_cnt);
 			if (ev_step) GPHI_HIP(ctx, hipEventRecord(ev[2 * GPHI_MAX_STEPS + 4 + enq], s));
 			if (kBfs)
 				bfs_dense_kernel<<<grid, kBlock, 0, s>>>(G.heads, G.tails, G.heads16, G.tails16, G.meta, G.pack,
