@@ -591,8 +591,10 @@ int run_bfs_batch(gphi_graph *gr, const uint32_t *srcs, uint32_t k, uint8_t *lev
 	GPHI_HIP(ctx, hipMemcpyAsync(d_srcs, srcs, (size_t)k * 4, hipMemcpyHostToDevice, s));
 	if (st) memset(st, 0, sizeof(*st));
 	u64 sums[2] = {0, 0};
-	// the masks are sized for gr->batch_words >= nw words; use that stride
-	switch (gr->batch_words) {
+	// the buffers hold gr->batch_words >= nw words per vertex; a batch runs at
+	// the width its own k needs (a 64-source batch after a 256-source one
+	// would otherwise gather and update four words per vertex)
+	switch (nw) {
 	case 1: rc = batch_loop<1>(gr, d_srcs, k, st, sums); break;
 	case 2: rc = batch_loop<2>(gr, d_srcs, k, st, sums); break;
 	default: rc = batch_loop<4>(gr, d_srcs, k, st, sums); break;
@@ -604,9 +606,9 @@ int run_bfs_batch(gphi_graph *gr, const uint32_t *srcs, uint32_t k, uint8_t *lev
 	}
 	if (levels) {
 		const unsigned fg = grid_n(G.n);
-		if (gr->batch_words == 1)
+		if (nw == 1)
 			batch_finalize_kernel<1><<<fg, kBlock, 0, s>>>(G.n, G.n_pad, k, gr->b_seen, gr->b_level);
-		else if (gr->batch_words == 2)
+		else if (nw == 2)
 			batch_finalize_kernel<2><<<fg, kBlock, 0, s>>>(G.n, G.n_pad, k, gr->b_seen, gr->b_level);
 		else
 			batch_finalize_kernel<4><<<fg, kBlock, 0, s>>>(G.n, G.n_pad, k, gr->b_seen, gr->b_level);

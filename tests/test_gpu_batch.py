@@ -54,6 +54,22 @@ def test_batch_duplicates_and_isolated_sources(ctx, c1):
     assert np.array_equal(lv2, expected(c1, srcs[:2]))
 
 
+def test_batch_narrower_after_wider(ctx, c1):
+    """Buffers sized for 256 sources, then batches of 64, 100 and 256 again:
+    each runs at its own mask width and stays exact."""
+    print("case: batch c1 k=256 then 64, 100, 256", flush=True)
+    gg = upload(ctx, c1)
+    try:
+        for k in (256, 64, 100, 256):
+            srcs = pick_sources(c1.deg, k, 3000 + k)
+            lv, st = gg.bfs_batch(srcs)
+            want = expected(c1, srcs)
+            assert np.array_equal(lv, want), f"k={k}"
+            assert st.reached == int((want != 255).sum()), f"k={k}"
+    finally:
+        gg.free()
+
+
 def test_batch_other_tilings(ctx):
     n, h, t = O.config_edges("c1")
     g = O.build_from_edgelist(n, h, t, 1000, 3)
