@@ -129,6 +129,9 @@ __device__ __forceinline__ void gather_fronts(const uint32_t (&h)[K], const uint
 // group.  next[v] is zero on entry and written only by v's group.
 constexpr uint32_t kLightGroup = 8;
 constexpr uint32_t kLightSlots = kPullLight / kLightGroup;
+// a wavefront serves one vertex per group per round and takes kLightGroup
+// vertices per round, so there must be kLightGroup groups of kLightGroup lanes
+static_assert(kLightGroup * kLightGroup == 64, "light pull: groups per wavefront == lanes per group");
 
 __device__ __forceinline__ uint32_t shfl_u32(uint32_t x, uint32_t src) { return (uint32_t)__shfl((int)x, (int)src, 64); }
 
@@ -200,7 +203,7 @@ __global__ __launch_bounds__(kBlock) void batch_pull_light_kernel(
 }
 
 // Pull for vertices with more than kPullLight in-edges: a wavefront per piece
-// of up to kPullPiece in-edges, 512 edges between checks of the stop rule.
+// of up to kPullPiece in-edges, 64 · kUnroll edges between checks of the stop rule.
 // Pieces of one vertex combine with one atomicOr per word.
 template <int NW, bool kFilter>
 __global__ __launch_bounds__(kBlock) void batch_pull_heavy_kernel(
@@ -208,7 +211,10 @@ __global__ __launch_bounds__(kBlock) void batch_pull_heavy_kernel(
     const uint32_t *__restrict__ in_src, const uint32_t *__restrict__ cur, const u64 *__restrict__ front,
     const u64 *__restrict__ seen, const u64 *__restrict__ live, u64 *next)
 {
-	constexpr uint32_t kUnroll = 8;
+	// loads in flight per lane between checks of the stop rule: a vertex stops
+	// as soon as every wanted source is found, so short rounds (64 or 128
+	// edges) beat deeper unrolling here
+	constexpr uint32_t kUnroll = NW == 1 ? 1 : 2;
 	const uint32_t lane = threadIdx.x & 63;
 	const uint32_t nwaves = (gridDim.x * kBlock) >> 6;
 	for (uint32_t p = (blockIdx.x * kBlock + threadIdx.x) >> 6; p < npieces; p += nwaves) {
