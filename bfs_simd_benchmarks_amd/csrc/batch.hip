@@ -512,8 +512,8 @@ int batch_loop(gphi_graph *gr, const uint32_t *d_srcs, uint32_t k, gphi_stats *s
 				                          kBlock, 0, s>>>(gr->queue, gr->qoff, gr->tile_lo, hc->qsize, hc->fdeg,
 				                                          G.row_off, G.cols, gr->b_front, gr->b_seen, gr->b_next);
 		} else {
-			// the frontier filter pays while the union frontier is under 1/16 of the vertices
-			if (hc->fsize < G.n / 16)
+			// the frontier filter pays while the union frontier is under 1/8 of the vertices
+			if (hc->fsize < G.n / 8)
 				launch_pull<NW, true>(gr, live);
 			else
 				launch_pull<NW, false>(gr, live);
