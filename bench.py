@@ -333,16 +333,9 @@ def serving(args) -> int:
     import threading
 
     import bfs_simd_benchmarks_amd as G
-    from bfs_simd_benchmarks_amd.workloads import SERVING_MIX
+    from bfs_simd_benchmarks_amd.workloads import SERVING_MIX, serving_stream
 
-    rng = np.random.default_rng(5)
-    gaps = rng.exponential(1.0 / args.qps, args.queries)
-    arrivals = np.cumsum(gaps)
-    weighted_ids = [i for i, w in enumerate(SERVING_MIX) if w.weight_seed]
-    is_sssp = rng.random(args.queries) < 0.2
-    gsel = np.where(is_sssp, rng.choice(weighted_ids, args.queries),
-                    rng.integers(0, len(SERVING_MIX), args.queries))
-    unif = rng.random(args.queries)
+    arrivals, is_sssp, gsel, sources = serving_stream(args.queries, args.qps, 5)
     results = {}
     for mode, max_batch in (("batched", 256), ("one_at_a_time", 1)):
         srv = G.Server(max_batch=max_batch)
@@ -354,7 +347,7 @@ def serving(args) -> int:
             sizes.append((g.n, g.m))
             srv.add_graph(g)
         build_s = time.perf_counter() - t0
-        srcs = (unif * np.array([sizes[i][0] for i in gsel])).astype(np.int64)
+        srcs = sources([n for n, _ in sizes])
         tickets = [0] * args.queries
         start = time.perf_counter()
 

@@ -243,6 +243,8 @@ class Server:
             C.byref(ServerOpts(max_batch, hbm_budget, sssp_lanes)), C.byref(h)))
         self.h = h
         self.graphs = []
+        self._n = {}          # graph id -> vertex count
+        self._outs = {}       # ticket -> output array the worker thread writes into
 
     def add_graph(self, g: "Graph") -> int:
         gid = C.c_uint32()
@@ -250,16 +252,35 @@ class Server:
         g.h = None                      # owned by the server now
         g.ctx.h = None
         self.graphs.append(g)
+        self._n[gid.value] = g.n
         return gid.value
 
     def submit(self, gid: int, kind: int, src: int, out: np.ndarray | None = None) -> int:
+        """Queue a query.  `out` (optional) receives the result from a worker
+        thread: a C-contiguous int32 array (BFS levels) or uint32 array (SSSP
+        distances) of at least the graph's vertex count, kept alive here until
+        wait() returns for the ticket."""
+        if out is not None:
+            want = np.int32 if kind == QUERY_BFS else np.uint32
+            n = self._n.get(gid)
+            if n is None:
+                raise GphiError(-3, f"no graph {gid}")
+            if (not isinstance(out, np.ndarray) or out.dtype != want or not out.flags.c_contiguous
+                    or not out.flags.writeable or out.size < n):
+                raise GphiError(-6, f"out must be a writeable C-contiguous {np.dtype(want).name} array "
+                                    f"of at least {n} elements")
         t = C.c_uint64()
         _host_check(self.lib, self.lib.gphi_server_submit(self.h, gid, kind, src, _ptr(out), C.byref(t)))
+        if out is not None:
+            self._outs[t.value] = out
         return t.value
 
     def wait(self, ticket: int) -> QueryInfo:
         info = QueryInfo()
-        _host_check(self.lib, self.lib.gphi_server_wait(self.h, ticket, C.byref(info)))
+        try:
+            _host_check(self.lib, self.lib.gphi_server_wait(self.h, ticket, C.byref(info)))
+        finally:
+            self._outs.pop(ticket, None)
         return info
 
     def stats(self, gid: int):

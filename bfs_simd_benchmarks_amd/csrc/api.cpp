@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstdarg>
 #include <cstring>
 #include <new>
@@ -28,6 +29,19 @@ int set_error(gphi_ctx *ctx, int code, const char *fmt, ...)
 	return code;
 }
 
+// allocations left before the injected failure (gphi_debug_fail_alloc); < 0: none
+static std::atomic<long> g_fail_alloc_in{-1};
+
+bool take_injected_alloc_failure()
+{
+	long left = g_fail_alloc_in.load(std::memory_order_relaxed);
+	while (left >= 0) {
+		if (g_fail_alloc_in.compare_exchange_weak(left, left - 1, std::memory_order_relaxed))
+			return left == 0;
+	}
+	return false;
+}
+
 int hip_fail(gphi_ctx *ctx, hipError_t e, const char *what, const char *file, int line)
 {
 	return set_error(ctx, GPHI_E_HIP, "%s failed: %s (%s:%d)", what, hipGetErrorString(e), file,
@@ -41,6 +55,8 @@ using gphi::set_error;
 extern "C" {
 
 const char *gphi_version(void) { return "gphi 0.1 (gfx950)"; }
+
+void gphi_debug_fail_alloc(int after) { gphi::g_fail_alloc_in.store(after < 0 ? -1 : after); }
 
 int gphi_open(const gphi_opts *opts, gphi_ctx **out)
 {

@@ -496,8 +496,6 @@ int batch_loop(gphi_graph *gr, const uint32_t *d_srcs, uint32_t k, gphi_stats *s
 	uint32_t steps = 0;
 	while (!hc->done) {
 		const uint32_t L = steps;
-		if (L + 1 >= kUnreached)
-			return set_error(ctx, GPHI_E_RANGE, "a batched BFS deeper than 254 levels does not fit uint8 levels");
 		// the rule applies to the union frontier from the first step on
 		const uint32_t dir = hc->dir;
 		if (st && steps < GPHI_MAX_STEPS) {
@@ -528,6 +526,10 @@ int batch_loop(gphi_graph *gr, const uint32_t *d_srcs, uint32_t k, gphi_stats *s
 		if (ev_step) GPHI_HIP(ctx, hipEventRecord(ev[3 + 2 * steps], s));
 		GPHI_HIP(ctx, hipMemcpyAsync(hc, gr->ctrl, sizeof(StepCtrl), hipMemcpyDeviceToHost, s));
 		GPHI_HIP(ctx, hipStreamSynchronize(s));
+		// a pair reached at level 255 would read as unreached: the expansion
+		// of level 254 may run, but only if it reaches nothing new
+		if (L + 1 >= kUnreached && !hc->done)
+			return set_error(ctx, GPHI_E_RANGE, "a batched BFS deeper than 254 levels does not fit uint8 levels");
 		++steps;
 		if (steps > G.n + 2) return set_error(ctx, GPHI_E_STATE, "batched traversal did not terminate");
 	}

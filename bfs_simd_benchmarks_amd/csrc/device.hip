@@ -108,6 +108,10 @@ __global__ __launch_bounds__(kBlock) void chunk_meta_kernel(const uint32_t *__re
 
 static int dmalloc(gphi_ctx *ctx, void **p, size_t bytes, uint64_t *acc)
 {
+	if (take_injected_alloc_failure()) {
+		*p = nullptr;
+		return set_error(ctx, GPHI_E_OOM, "hipMalloc(%zu bytes) failed: injected failure", bytes);
+	}
 	hipError_t e = hipMalloc(p, bytes ? bytes : 4);
 	if (e != hipSuccess) {
 		*p = nullptr;
@@ -507,10 +511,6 @@ void free_graph_device(gphi_graph *gr)
 		if (p) (void)hipFree(p);
 }
 
-// The per-XCD live-list counters, one 128-byte line apart so that their
-// atomics do not queue on one line.
-constexpr uint32_t kLiveCntStride = 32;
-constexpr size_t kLiveCntBytes = (size_t)kNumXcd * kLiveCntStride * 4;
 
 // Capacity of one XCD's live-chunk list: chunk selection hands out chunks in
 // groups of 64 (a wavefront's), group g to XCD g % 8.
@@ -523,11 +523,23 @@ static uint32_t live_list_cap(uint64_t nchunks)
 // ---------------------------------------------------------------------------
 // traversal state
 // ---------------------------------------------------------------------------
+static int alloc_state(gphi_graph *gr);
+
+// The traversal buffers, all or none: a failed allocation frees the ones
+// already made, so no later call finds gr->level set beside a missing buffer
+// and launches kernels on it.
 static int ensure_state(gphi_graph *gr)
+{
+	if (gr->level) return GPHI_OK;
+	const int rc = alloc_state(gr);
+	if (rc) free_traversal_state(gr);
+	return rc;
+}
+
+static int alloc_state(gphi_graph *gr)
 {
 	gphi_ctx *ctx = gr->ctx;
 	DevGraph &G = gr->g;
-	if (gr->level) return GPHI_OK;
 	int rc;
 	size_t nb = (size_t)G.n_pad * 4;
 	if ((rc = dmalloc(ctx, (void **)&gr->level, nb, nullptr))) return rc;
@@ -1707,6 +1719,10 @@ int run_bfs(gphi_graph *gr, uint32_t src, int32_t *levels, int32_t *parents, gph
 	GPHI_HIP(ctx, hipStreamSynchronize(s));
 	return GPHI_OK;
 }
+
+int bfs_level_loop(gphi_graph *gr, uint32_t src, gphi_stats *st) { return frontier_loop<true>(gr, src, st); }
+
+uint32_t live_list_capacity(uint64_t nchunks) { return live_list_cap(nchunks); }
 
 int run_sssp(gphi_graph *gr, uint32_t src, uint32_t *dist, gphi_stats *st)
 {

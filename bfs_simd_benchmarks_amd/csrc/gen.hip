@@ -628,7 +628,13 @@ int retile_graph(gphi_graph *gr, uint32_t W, uint32_t S, int tile_order)
 	G.ntiles = (uint64_t)side * side;
 	G.tile_order = tile_order;
 	G.has_tiles = true;
-	return finish_tiles(ctx, gr);
+	const int frc = finish_tiles(ctx, gr);
+	if (frc) {
+		// the chunk summaries are incomplete: no sweep may run on these tiles
+		free_chunk_summaries(G);
+		G.has_tiles = false;
+	}
+	return frc;
 }
 
 namespace {

@@ -23,6 +23,10 @@ constexpr uint32_t kPullLight = 64;     // pull steps: a lane per vertex up to t
 constexpr uint32_t kPullPiece = 4096;    // else a wavefront per piece of this many in-edges
 constexpr uint8_t kVisited = 2;          // state byte of a vertex reached in an earlier step
 constexpr int kNumXcd = 8;              // MI355X: 8 XCDs, blocks dispatched round-robin
+// The per-XCD live-list counters, one 128-byte line apart so that their
+// atomics do not queue on one line.
+constexpr uint32_t kLiveCntStride = 32;
+constexpr size_t kLiveCntBytes = (size_t)kNumXcd * kLiveCntStride * 4;
 
 struct Error {
 	int code;
@@ -151,6 +155,8 @@ struct gphi_graph {
 
 namespace gphi {
 int set_error(gphi_ctx *ctx, int code, const char *fmt, ...);
+// true when an allocation failure injected by gphi_debug_fail_alloc is due now
+bool take_injected_alloc_failure();
 int hip_fail(gphi_ctx *ctx, hipError_t e, const char *what, const char *file, int line);
 
 // device-side entry points (device.hip)
@@ -175,6 +181,9 @@ int download_graph(gphi_graph *gr, uint32_t *deg, uint32_t *cols, uint32_t *cw, 
                    uint32_t *tt, uint32_t *tw);
 
 int ensure_traversal_state(gphi_graph *gr);
+// the BFS level loop of run_bfs on gr's state buffers (set up by the caller)
+int bfs_level_loop(gphi_graph *gr, uint32_t src, gphi_stats *st);
+uint32_t live_list_capacity(uint64_t nchunks);  // entries of one XCD's live-chunk list
 void launch_init_blocks(gphi_graph *gr);
 void launch_frontier_update_flags(gphi_graph *gr, uint32_t next_level, uint32_t threshold);
 int run_bfs_batch(gphi_graph *gr, const uint32_t *srcs, uint32_t k, uint8_t *levels, gphi_stats *st);

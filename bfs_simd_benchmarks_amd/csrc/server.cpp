@@ -222,13 +222,20 @@ int gphi_server_add_graph(gphi_server *S, gphi_graph *g, uint32_t *graph_id)
 	const uint64_t need = g->g.device_bytes + gphi::lane_state_bytes(g->g, S->opts.max_batch) +
 	                      nv * gphi::lane_state_bytes(g->g, 1);
 	{
+		// the check and the reservation in one critical section, so two
+		// concurrent admissions cannot both fit into the same free bytes
 		std::lock_guard<std::mutex> lk(S->mu);
 		if (S->hbm_used + need > S->opts.hbm_budget)
 			return set_error(nullptr, GPHI_E_OOM,
 			                 "graph needs %llu MB of HBM; %llu MB of the %llu MB budget are in use",
 			                 (unsigned long long)(need >> 20), (unsigned long long)(S->hbm_used >> 20),
 			                 (unsigned long long)(S->opts.hbm_budget >> 20));
+		S->hbm_used += need;
 	}
+	auto unreserve = [&] {
+		std::lock_guard<std::mutex> lk(S->mu);
+		S->hbm_used -= need;
+	};
 	std::vector<gphi_graph *> views;
 	std::vector<gphi_ctx *> vctxs;
 	for (uint32_t i = 0; i < nv; ++i) {
@@ -241,6 +248,7 @@ int gphi_server_add_graph(gphi_server *S, gphi_graph *g, uint32_t *graph_id)
 			if (vc) gphi_close(vc);
 			for (gphi_graph *x : views) gphi_graph_free(x);
 			for (gphi_ctx *x : vctxs) gphi_close(x);
+			unreserve();
 			return rc;
 		}
 		views.push_back(v);
@@ -252,7 +260,6 @@ int gphi_server_add_graph(gphi_server *S, gphi_graph *g, uint32_t *graph_id)
 	L->ctx = g->ctx;
 	L->n = g->g.n;
 	L->device_bytes = need;
-	S->hbm_used += need;
 	Lane *bl = L.get();
 	std::vector<Lane *> sl;
 	S->lanes.push_back(std::move(L));
@@ -312,6 +319,10 @@ int gphi_server_wait(gphi_server *S, uint64_t ticket, gphi_query_info *info)
 	auto it = S->queries.find(ticket);
 	if (it == S->queries.end()) return set_error(nullptr, GPHI_E_ARG, "unknown ticket %llu", (unsigned long long)ticket);
 	std::shared_ptr<Query> q = it->second;
+	// one waiter per ticket: the ticket leaves the table now, so a second
+	// wait on it (concurrent or later) is an unknown ticket, not a use of an
+	// iterator the first waiter has invalidated
+	S->queries.erase(it);
 	S->cv_done.wait(lk, [&] { return q->done; });
 	if (info) {
 		info->latency_us = q->t_done - q->t_submit;
@@ -319,7 +330,6 @@ int gphi_server_wait(gphi_server *S, uint64_t ticket, gphi_query_info *info)
 		info->run_us = q->t_done - q->t_start;
 		info->batch = q->batch;
 	}
-	S->queries.erase(it);
 	return q->status;
 }
 

@@ -53,6 +53,25 @@ SERVING_MIX = [Workload(f"rmat{s}-ef16-s{seed}", s, 1 << s, seed, 16 << s,
                for i, (s, seed) in enumerate(zip((18, 19, 20, 21, 22, 22, 23, 24), range(10, 18)))]
 
 
+def serving_stream(queries: int, qps: float, seed: int = 5):
+    """The serving mix's query stream: Poisson arrivals at `qps` offered
+    queries/s; 20% SSSP on a uniform random weighted graph, 80% BFS on a
+    uniform random graph; a uniform random source.  Returns (arrival times in
+    s, is_sssp bool[], graph index[], source[] given each graph's vertex count
+    through `sources(nverts)`)."""
+    rng = np.random.default_rng(seed)
+    arrivals = np.cumsum(rng.exponential(1.0 / qps, queries))
+    weighted_ids = [i for i, w in enumerate(SERVING_MIX) if w.weight_seed]
+    is_sssp = rng.random(queries) < 0.2
+    gsel = np.where(is_sssp, rng.choice(weighted_ids, queries),
+                    rng.integers(0, len(SERVING_MIX), queries))
+    unif = rng.random(queries)
+
+    def sources(nverts):
+        return (unif * np.asarray([nverts[i] for i in gsel])).astype(np.int64)
+    return arrivals, is_sssp, gsel, sources
+
+
 def _sm64(state: int) -> tuple[int, int]:
     state = (state + 0x9E3779B97F4A7C15) & MASK64
     z = state

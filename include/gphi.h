@@ -257,6 +257,13 @@ int gphi_graph_load(gphi_ctx *ctx, const char *path, gphi_graph **out);
  * out has SIDE^2 entries. */
 int gphi_graph_tile_offsets(gphi_graph *g, uint32_t *out);
 
+/* Testing hook (no reference counterpart): the device allocation `after`
+ * allocations from now fails as if HBM were exhausted (GPHI_E_OOM); after < 0
+ * clears it.  Lets the tests check that a failed allocation leaves a graph
+ * that later calls can still use or that reports an error, never one that a
+ * kernel would be launched on half-built. */
+void gphi_debug_fail_alloc(int after);
+
 /* Algorithmic HBM bytes of a finished BFS (per step, the byte model of the
  * design doc) from its levels and stats; bytes_per_step has st->steps entries. */
 int gphi_bfs_alg_bytes(const gphi_graph *g, const int32_t *levels, const gphi_stats *st,

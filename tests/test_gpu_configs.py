@@ -1,0 +1,185 @@
+"""The benchmark configurations on the GPU, each against the oracle:
+
+- pokec shape (1,632,803 vertices, 30,622,564 edges, ids permuted then
+  degree-reordered, W=1024 S=16): hybrid BFS from the 64 seeded sources, one
+  at a time, levels bit-exact and parents valid; PageRank FP32, damping 0.85,
+  20 iterations from 1/N, within 1e-6 L1 of the FP64 oracle;
+- weighted rmat16 (weights uniform in [1,255], seed 3): SSSP from 16 seeded
+  sources, distances bit-exact against Dijkstra;
+- rmat25-ef36 (33,554,432 vertices, ~1.18e9 edges, W=4096 S=16): plain BFS
+  from 2 sources against the oracle's OpenMP hybrid BFS (levels, trace,
+  parents); a 64-source batched launch whose rows are checked against the
+  oracle for 16 sources and against the plain BFS for all 64;
+- the serving mix (8 resident graphs, scales 18-24): 240 queries of the
+  Poisson BFS/SSSP stream through gphi_server, every answer against the
+  oracle (BFS levels / Dijkstra distances).
+
+The graphs are generated on the device (gphi_graph_generate); the oracle runs
+on the downloaded arrays (the generator itself equals the oracle's, element
+for element, in test_gpu_generate.py)."""
+import os
+import threading
+import time
+from concurrent.futures import ThreadPoolExecutor
+from types import SimpleNamespace
+
+import numpy as np
+import pytest
+
+import oracle as O
+import bfs_simd_benchmarks_amd as G
+from bfs_simd_benchmarks_amd.workloads import SERVING_MIX, WORKLOADS, pick_sources, serving_stream
+
+pytestmark = pytest.mark.gpu
+
+THREADS = min(16, os.cpu_count() or 1)
+
+
+def host_graph(g):
+    """Downloaded arrays in the shape the oracle functions take."""
+    d = g.download()
+    return SimpleNamespace(n=g.n, m=g.m, W=g.W, S=g.S, deg=d["deg"], cols=d["cols"], cw=d["cw"],
+                           sh=d["heads"], st=d["tails"], soff=g.tile_offsets())
+
+
+# ---------------------------------------------------------------- pokec shape
+@pytest.fixture(scope="module")
+def c2(ctx):
+    g = WORKLOADS["c2"].generate(ctx)
+    yield g, host_graph(g)
+    g.free()
+
+
+def test_c2_bfs_64_sources(c2):
+    g, h = c2
+    assert (g.n, g.m) == (1_632_803, 30_622_564)
+    srcs = pick_sources(h.deg, 64, WORKLOADS["c2"].seed)
+    for i, s in enumerate(srcs):
+        print(f"case: c2 bfs source {i} = {s}", flush=True)
+        lv, pa, st = g.bfs(int(s))
+        ref = O.bfs_levels(h.n, h.deg, h.cols, int(s))
+        assert np.array_equal(lv, ref), (s, np.flatnonzero(lv != ref)[:10])
+        assert O.check_parents(h.n, h.deg, h.cols, int(s), lv, pa) == 0, s
+        assert st.edges_traversed == int(h.deg[ref >= 0].astype(np.uint64).sum())
+
+
+def test_c2_pagerank_20_iterations(c2):
+    g, h = c2
+    print("case: c2 pagerank 20 iterations", flush=True)
+    r, _ = g.pagerank(20, 0.85, 1.0 / g.n)
+    heads = np.repeat(np.arange(h.n, dtype=np.uint32), h.deg)
+    ref = O.pagerank_f64(h.n, heads, h.cols, h.deg, 20, 0.85, 1.0 / h.n)
+    l1 = float(np.abs(r.astype(np.float64) - ref).sum())
+    print(f"c2 pagerank L1 vs FP64 = {l1:.3e} (sum of ranks {ref.sum():.6f})", flush=True)
+    assert l1 < 1e-6
+
+
+# ------------------------------------------------------ weighted rmat16 (SSSP)
+def test_c3_sssp_16_sources(ctx, c3):
+    gg = ctx.graph_from_arrays(c3.n, c3.deg, W=c3.W, S=c3.S, csr_edges=c3.cols, csr_weights=c3.cw,
+                               tile_heads=c3.sh, tile_tails=c3.st, tile_weights=c3.sw,
+                               tile_offsets=c3.soff)
+    try:
+        for s in pick_sources(c3.deg, 16, 3):
+            print(f"case: c3 sssp source {s}", flush=True)
+            d, _ = gg.sssp(int(s))
+            ref = O.sssp_dijkstra(c3.n, c3.deg, c3.cols, c3.cw, int(s))
+            assert np.array_equal(d, ref), (s, np.flatnonzero(d != ref)[:10])
+    finally:
+        gg.free()
+
+
+# ------------------------------------------------------------ rmat25-ef36
+@pytest.fixture(scope="module")
+def c4(ctx):
+    wl = WORKLOADS["c4"]
+    t0 = time.perf_counter()
+    g = wl.generate(ctx)
+    h = host_graph(g)
+    print(f"c4 built and downloaded in {time.perf_counter() - t0:.1f} s", flush=True)
+    yield g, h
+    g.free()
+
+
+def oracle_levels(h, src):
+    """Levels, parents and trace of the oracle's OpenMP hybrid BFS (its levels
+    equal the textbook BFS's, test_oracle_math.py)."""
+    return O.bfs_hybrid(h, int(src), 20, THREADS)
+
+
+def test_c4_plain_bfs_two_sources(c4):
+    g, h = c4
+    for s in pick_sources(h.deg, 2, WORKLOADS["c4"].seed + 500):
+        print(f"case: c4 bfs source {s}", flush=True)
+        lv, pa, st = g.bfs(int(s))
+        ref, _, tr = oracle_levels(h, s)
+        assert np.array_equal(lv, ref), (s, np.flatnonzero(lv != ref)[:10])
+        assert np.array_equal(st.trace(), tr)
+        assert O.check_parents(h.n, h.deg, h.cols, int(s), lv, pa) == 0, s
+
+
+def test_c4_batched_64_sources(c4):
+    g, h = c4
+    srcs = pick_sources(h.deg, 64, WORKLOADS["c4"].seed + 1000 + 64)
+    print("case: c4 batch of 64", flush=True)
+    lv, st = g.bfs_batch(srcs)
+    for i in range(0, 64, 4):
+        print(f"case: c4 batch row {i} (source {srcs[i]}) vs oracle", flush=True)
+        ref, _, _ = oracle_levels(h, srcs[i])
+        want = np.where(ref >= 0, ref, 255).astype(np.uint8)
+        assert np.array_equal(lv[i], want), (i, np.flatnonzero(lv[i] != want)[:10])
+    reached = 0
+    for i, s in enumerate(srcs):
+        one, _, _ = g.bfs(int(s), parents=False)
+        assert np.array_equal(lv[i], np.where(one >= 0, one, 255).astype(np.uint8)), i
+        reached += int((one >= 0).sum())
+    assert st.reached == reached
+
+
+# ------------------------------------------------------------ serving mix
+def test_c5_serving_mix_against_oracle():
+    queries = 240
+    arrivals, is_sssp, gsel, sources = serving_stream(queries, 2000.0, 5)
+    srv = G.Server(max_batch=256)
+    hosts = []
+    try:
+        for w in SERVING_MIX:
+            ctx = G.Context(0)
+            g = w.generate(ctx, W=1024, S=16, weighted=bool(w.weight_seed))
+            d = g.download(tiles=False)
+            hosts.append(SimpleNamespace(n=g.n, deg=d["deg"], cols=d["cols"], cw=d["cw"]))
+            srv.add_graph(g)
+        srcs = sources([x.n for x in hosts])
+        outs = [np.empty(hosts[gi].n, np.uint32 if sp else np.int32) for gi, sp in zip(gsel, is_sssp)]
+        tickets = [0] * queries
+        print(f"case: serving mix, {queries} queries", flush=True)
+        start = time.perf_counter()
+
+        def producer():
+            for q in range(queries):
+                dt = arrivals[q] - (time.perf_counter() - start)
+                if dt > 0:
+                    time.sleep(dt)
+                tickets[q] = srv.submit(int(gsel[q]), G.QUERY_SSSP if is_sssp[q] else G.QUERY_BFS,
+                                        int(srcs[q]), outs[q])
+        th = threading.Thread(target=producer)
+        th.start()
+        th.join()
+        batches = [srv.wait(t).batch for t in tickets]
+        print(f"answered in {time.perf_counter() - start:.2f} s, mean BFS batch "
+              f"{np.mean([b for b, sp in zip(batches, is_sssp) if not sp]):.2f}", flush=True)
+    finally:
+        srv.close()
+
+    def check(q):
+        x = hosts[gsel[q]]
+        if is_sssp[q]:
+            ok = np.array_equal(outs[q], O.sssp_dijkstra(x.n, x.deg, x.cols, x.cw, int(srcs[q])))
+        else:
+            ok = np.array_equal(outs[q], O.bfs_levels(x.n, x.deg, x.cols, int(srcs[q])))
+        outs[q] = None
+        return ok
+    with ThreadPoolExecutor(THREADS) as ex:
+        ok = list(ex.map(check, range(queries)))
+    bad = [q for q in range(queries) if not ok[q]]
+    assert not bad, [(q, int(gsel[q]), bool(is_sssp[q]), int(srcs[q])) for q in bad[:5]]
