@@ -24,6 +24,11 @@ import sys
 import time
 from pathlib import Path
 
+# CPU baseline placement (read by the OpenMP runtime when it loads, so before
+# any import that loads it): threads on consecutive physical cores
+os.environ.setdefault("OMP_PROC_BIND", "close")
+os.environ.setdefault("OMP_PLACES", "cores")
+
 import numpy as np
 
 HBM_PEAK_GBS = 8000.0
@@ -38,6 +43,26 @@ def sweep_bytes(n: int, e_act: float, t_act: float, f_next: float) -> float:
 
 
 PROFILES = Path(__file__).resolve().parent / "profiles"
+
+
+def cpu_model():
+    """(model name, physical cores) of the host, from lscpu."""
+    import subprocess
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+    except (OSError, subprocess.SubprocessError):
+        return None, None
+    info = {}
+    for line in out.splitlines():
+        if ":" in line:
+            k, v = line.split(":", 1)
+            info[k.strip()] = v.strip()
+    phys = None
+    try:
+        phys = int(info["Core(s) per socket"]) * int(info["Socket(s)"])
+    except (KeyError, ValueError):
+        pass
+    return info.get("Model name"), phys
 
 
 def measured_traffic(config: str, W: int, S: int):
@@ -56,12 +81,17 @@ def measured_traffic(config: str, W: int, S: int):
             vals[k.strip()[len("traffic."):]] = v.strip()
     try:
         if (int(vals["W"]), int(vals["S"])) != (W, S):
-            return None, None
+            return None, None, None
         b = (float(vals["fetch_kib"]) * float(vals["fetch_factor"]) +
              float(vals["write_kib"]) * float(vals["write_factor"])) * 1024
     except (KeyError, ValueError):
-        return None, None
-    return round(b), str(f.relative_to(f.parents[1]))
+        return None, None, None
+    mean_ms = None
+    try:
+        mean_ms = float(vals["dense_mean_ms"])  # kernel-trace mean of the dense-step dispatches
+    except (KeyError, ValueError):
+        pass
+    return round(b), str(f.relative_to(f.parents[1])), mean_ms
 
 
 def main() -> int:
@@ -76,6 +106,8 @@ def main() -> int:
                     help="budget of the CPU baseline (0 disables it)")
     ap.add_argument("--batches", default="64,256",
                     help="batched multi-source BFS sizes to measure ('' disables)")
+    ap.add_argument("--batch-launches", type=int, default=30,
+                    help="timed launches per batch size (p50/p99 over them)")
     ap.add_argument("--other-paths", type=int, default=1,
                     help="also time PageRank, hop relaxation and weighted SSSP on the same graph (0 skips)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -168,7 +200,7 @@ def main() -> int:
     per_launch_bytes = dense_bytes / max(dense_launches, 1)
     per_launch_ms = dense_ms / max(dense_launches, 1)
     achieved = per_launch_bytes / (per_launch_ms * 1e-3) / 1e9 if dense_launches else 0.0
-    traffic, traffic_src = measured_traffic(args.config, g.W, g.S)
+    traffic, traffic_src, prof_ms = measured_traffic(args.config, g.W, g.S)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "kernel": "bfs_dense_kernel", "launches": dense_launches,
@@ -177,66 +209,108 @@ def main() -> int:
                 "chunk_select_ms_per_launch": round(select_ms / max(dense_launches, 1), 4)}
     if traffic_src:
         roofline["traffic_source"] = traffic_src
+    if prof_ms:
+        # the same bytes over the committed rocprofv3 kernel-trace mean (that
+        # run's dense-step dispatches); `frac` above, from HIP events in this
+        # run, is the headline
+        roofline["profile_mean_launch_ms"] = prof_ms
+        roofline["frac_profile"] = round(per_launch_bytes / (prof_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
     whole = step_bytes_total / (step_ms_total * 1e-3) / 1e9 if step_ms_total else 0.0
 
-    # batched multi-source BFS vs repeated plain runs (same GPU, same graph)
+    # the oracle (CPU) on the same graph: baseline timing and row checks
+    host = None
+    if args.cpu_seconds > 0:
+        import oracle as O
+        d = g.download()
+        host = {"deg": d["deg"], "cols": d["cols"], "heads": d["heads"], "tails": d["tails"],
+                "toff": g.tile_offsets()}
+        del d
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
+
+    def oracle_bfs(s):
+        lv = np.empty(g.n, np.int32)
+        pa = np.empty(g.n, np.int32)
+        tr = np.zeros(3 * 4096, np.uint32)
+        O.lib().orc_bfs_hybrid(g.n, g.m, g.W, g.S, O._p(host["deg"]), O._p(host["cols"]),
+                               O._p(host["heads"]), O._p(host["tails"]), O._p(host["toff"]), int(s),
+                               20, threads, O._p(lv), O._p(pa), O._p(tr), 4096)
+        return lv
+
+    # batched multi-source BFS vs repeated plain runs (same GPU, same graph);
+    # both rates from host wall clock: plain = timed BFS / their wall time
+    plain_qps = len(timed) / wall
     plain_lat = np.array([st.total_ms for st in stats])
     batch_out = {}
     for k in [int(x) for x in args.batches.split(",") if x.strip()]:
         bsrcs = pick_sources(deg, k, wl.seed + 1000 + k)
         g.bfs_batch(bsrcs, levels=False)                      # warm-up
-        reps = 3
-        t1 = time.perf_counter()
-        bst = None
-        for _ in range(reps):
+        lat, bst = [], None
+        for _ in range(args.batch_launches):
+            t1 = time.perf_counter()
             _, bst = g.bfs_batch(bsrcs, levels=False)
-        bwall = (time.perf_counter() - t1) / reps
-        # exactness: every row equals that source's single-source levels
+            lat.append(time.perf_counter() - t1)
+        lat = np.array(lat)
+        bwall = float(lat.mean())
+        # exactness: rows against the CPU oracle (when it runs) and every
+        # checked row against that source's single-source GPU levels
         blv, _ = g.bfs_batch(bsrcs, levels=True)
-        exact = True
-        for i in range(0, k, max(1, k // 4)):
+        rows = list(range(0, k, max(1, k // 4)))
+        exact_gpu = True
+        for i in rows:
             lv_i, _, _ = g.bfs(int(bsrcs[i]), parents=False, levels=True)
-            exact &= bool(np.array_equal(blv[i], np.where(lv_i >= 0, lv_i, 255).astype(np.uint8)))
+            exact_gpu &= bool(np.array_equal(blv[i], np.where(lv_i >= 0, lv_i, 255).astype(np.uint8)))
+        exact_oracle = None
+        if host is not None:
+            exact_oracle = all(np.array_equal(blv[i], np.where((r := oracle_bfs(bsrcs[i])) >= 0, r, 255)
+                                              .astype(np.uint8)) for i in rows[:2])
         del blv
         batch_out[str(k)] = {
             "queries_per_s": round(k / bwall, 1),
-            "vs_plain": round((k / bwall) / (1e3 / plain_lat.mean()), 2),
-            "latency_ms_p50": round(bwall * 1e3, 3), "latency_ms_p99": round(bwall * 1e3, 3),
+            "vs_plain": round((k / bwall) / plain_qps, 2),
+            "launches": len(lat),
+            "latency_ms_p50": round(float(np.percentile(lat, 50)) * 1e3, 3),
+            "latency_ms_p99": round(float(np.percentile(lat, 99)) * 1e3, 3),
             "device_ms": round(bst.total_ms, 3), "steps": int(bst.steps),
-            "gteps": round(bst.edges_traversed / bwall / 1e9, 1), "levels_exact": exact}
-        print(f"[bench] batch {k}: {k / bwall:.0f} queries/s ({bwall * 1e3:.1f} ms per launch), "
-              f"exact={exact}", file=sys.stderr, flush=True)
+            "gteps": round(bst.edges_traversed / bwall / 1e9, 1),
+            "rows_equal_single_source_gpu": exact_gpu, "rows_checked_gpu": len(rows),
+            "rows_equal_oracle": exact_oracle, "rows_checked_oracle": 2 if host is not None else 0}
+        print(f"[bench] batch {k}: {k / bwall:.0f} queries/s ({bwall * 1e3:.1f} ms per launch, "
+              f"p99 {batch_out[str(k)]['latency_ms_p99']} ms), {batch_out[str(k)]['vs_plain']}x plain, "
+              f"exact gpu={exact_gpu} oracle={exact_oracle}", file=sys.stderr, flush=True)
 
     cpu = None
     levels_match = None
-    if args.cpu_seconds > 0:
-        import oracle as O
-        d = g.download()
-        toff = g.tile_offsets()
-        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
-        runs, cpu_edges, cpu_t = 0, 0, 0.0
-        for s in timed:
-            lv = np.empty(g.n, np.int32)
-            pa = np.empty(g.n, np.int32)
-            tr = np.zeros(3 * 4096, np.uint32)
+    if host is not None:
+        # up to 10 single-source runs within the time budget, the timed sources in order
+        per_run, runs_edges, cpu_t = [], 0, 0.0
+        for s in timed[:10]:
             t1 = time.perf_counter()
-            O.lib().orc_bfs_hybrid(g.n, g.m, g.W, g.S, O._p(d["deg"]), O._p(d["cols"]),
-                                   O._p(d["heads"]), O._p(d["tails"]), O._p(toff), int(s), 20,
-                                   threads, O._p(lv), O._p(pa), O._p(tr), 4096)
-            cpu_t += time.perf_counter() - t1
-            if runs == 0:
+            lv = oracle_bfs(s)
+            dt = time.perf_counter() - t1
+            cpu_t += dt
+            if not per_run:
                 levels_match = bool(np.array_equal(lv, levels0))
-            cpu_edges += int(d["deg"][lv >= 0].astype(np.uint64).sum())
-            runs += 1
+            e = int(host["deg"][lv >= 0].astype(np.uint64).sum())
+            per_run.append(e / dt / 1e9)
+            runs_edges += e
             if cpu_t >= args.cpu_seconds:
                 break
-        cpu = {"value": round(cpu_edges / cpu_t / 1e9, 4), "unit": "GTEPS", "cores": threads,
+        per_run = np.array(per_run)
+        model, phys = cpu_model()
+        cpu = {"value": round(runs_edges / cpu_t / 1e9, 4), "unit": "GTEPS", "cores": threads,
                "kind": "port",
-               "sample": f"{runs} single-source hybrid BFS runs (same graph and sources as the "
-                         f"GPU) with the oracle's OpenMP restatement of apps/bfs_simd, "
-                         f"{cpu_t:.1f} s"}
-        print(f"[bench] cpu baseline {cpu['value']} GTEPS on {threads} threads, levels match: "
-              f"{levels_match}", file=sys.stderr, flush=True)
+               "mean": round(float(per_run.mean()), 4), "median": round(float(np.median(per_run)), 4),
+               "best": round(float(per_run.max()), 4), "runs": len(per_run),
+               "cpu_model": model, "physical_cores": phys,
+               "omp": {"threads": threads, "proc_bind": os.environ.get("OMP_PROC_BIND"),
+                       "places": os.environ.get("OMP_PLACES")},
+               "simd": f"avx512 dense sweep" if O.lib().orc_simd_width() == 512 else "one lane at a time",
+               "sample": f"{len(per_run)} single-source hybrid BFS runs (the GPU's graph and first "
+                         f"timed sources) with the oracle's OpenMP restatement of apps/bfs_simd "
+                         f"(dense sweep bfs.cpp:181-245 in AVX-512 when the host has it), "
+                         f"{cpu_t:.1f} s; value = edges / total time, mean/median/best per run"}
+        print(f"[bench] cpu baseline {cpu['value']} GTEPS on {threads} threads ({model}), "
+              f"levels match: {levels_match}", file=sys.stderr, flush=True)
 
     out = {
         "metric": METRIC,
@@ -259,9 +333,9 @@ def main() -> int:
         "device_ms_per_bfs": round(sum(st.total_ms for st in stats) / max(len(stats), 1), 4),
         "whole_bfs_alg_GBps": round(whole, 1),
         "levels_match_cpu": levels_match,
-        "plain_queries_per_s": round(1e3 / plain_lat.mean(), 1),
-        "plain_latency_ms_p50": round(float(np.percentile(plain_lat, 50)), 3),
-        "plain_latency_ms_p99": round(float(np.percentile(plain_lat, 99)), 3),
+        "plain_queries_per_s": round(plain_qps, 1),
+        "plain_device_ms_p50": round(float(np.percentile(plain_lat, 50)), 3),
+        "plain_device_ms_p99": round(float(np.percentile(plain_lat, 99)), 3),
         "batched": batch_out,
         "graph_build_s": round(gen_s, 2),
     }

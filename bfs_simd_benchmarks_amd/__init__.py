@@ -69,7 +69,7 @@ class ServerOpts(C.Structure):
 
 class QueryInfo(C.Structure):
     _fields_ = [("latency_us", C.c_double), ("queue_us", C.c_double), ("run_us", C.c_double),
-                ("batch", C.c_uint32)]
+                ("batch", C.c_uint32), ("joined_level", C.c_uint32), ("launch", C.c_uint64)]
 
 
 QUERY_BFS = 0
@@ -77,6 +77,7 @@ QUERY_SSSP = 1
 
 REORDER_NONE = 0
 REORDER_DEGREE = 1
+REORDER_BFS = 2
 
 
 class Stats(C.Structure):

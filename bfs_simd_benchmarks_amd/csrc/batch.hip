@@ -1,4 +1,4 @@
-// batch.hip -- multi-source BFS: k <= 256 sources in one traversal.
+// batch.hip -- multi-source BFS: up to 256 sources in one traversal.
 //
 // Each vertex carries k-bit masks (NW 64-bit words): the sources that have
 // reached it (seen), that reached it in the last step (front) and that reach
@@ -17,6 +17,19 @@
 // an in-neighbour had s in its front mask at level L and v had not seen s, so
 // the level of every (source, vertex) pair equals that source's own BFS level.
 // Levels are stored as uint8, source-major; 255 = unreached.
+//
+// The level loop runs on the device, as the single-source one does: every
+// kernel of a level reads the direction and sizes from the control block and
+// returns at once when it has nothing to do, so the host enqueues groups of
+// levels and reads the block back once per group.  A step marks the vertices
+// whose next mask it set in a bitmap, and the per-level update visits only
+// those (and the previous frontier), not every vertex.
+//
+// Sources may also join a running traversal between two levels (the serving
+// loop's continuous batching): a source that joins at level J gets a slot of
+// its own, its vertex enters the union frontier, and the levels stored for
+// it are the traversal's level minus J -- its own BFS levels, since its bit
+// spreads exactly as if it had started alone at level J.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -32,21 +45,23 @@ namespace {
 constexpr uint8_t kUnreached = 255;
 typedef unsigned long long u64;
 
+// b_aux layout (uint32 words): sources to seed, their slots, per-slot start level
+constexpr uint32_t kAuxSrc = 0, kAuxSlot = kBatchMaxSources, kAuxStart = 2 * kBatchMaxSources;
+constexpr uint32_t kAuxWords = 3 * kBatchMaxSources;
+
 __device__ __forceinline__ bool test_bit32(const uint32_t *__restrict__ bm, uint32_t v)
 {
 	return (bm[v >> 5] >> (v & 31)) & 1u;
 }
 
+__device__ __forceinline__ void mark_touched(uint32_t *touched, uint32_t v)
+{
+	atomicOr(touched + (v >> 5), 1u << (v & 31));
+}
+
 template <int NW>
 struct Mask {
 	u64 w[NW];
-	__device__ __forceinline__ bool any() const
-	{
-		u64 a = 0;
-#pragma unroll
-		for (int i = 0; i < NW; ++i) a |= w[i];
-		return a != 0;
-	}
 };
 
 template <int NW>
@@ -58,23 +73,33 @@ __device__ __forceinline__ Mask<NW> load_mask(const u64 *__restrict__ p, uint32_
 	return m;
 }
 
-// next[t] |= f & ~seen[t], skipping words already set in next
+// next[t] |= f & ~seen[t], skipping words already set in next; whoever sets
+// the first bit of a word marks t touched
 template <int NW>
 __device__ __forceinline__ void push_mask(const Mask<NW> &f, uint32_t t, const u64 *__restrict__ seen,
-                                          u64 *next)
+                                          u64 *next, uint32_t *touched)
 {
+	bool first = false;
 #pragma unroll
 	for (int i = 0; i < NW; ++i) {
 		if (!f.w[i]) continue;
 		const u64 m = f.w[i] & ~seen[(uint64_t)t * NW + i];
-		if (m && (m & ~next[(uint64_t)t * NW + i])) atomicOr(next + (uint64_t)t * NW + i, m);
+		if (m && (m & ~next[(uint64_t)t * NW + i])) first |= atomicOr(next + (uint64_t)t * NW + i, m) == 0ull;
 	}
+	if (first) mark_touched(touched, t);
 }
 
 __device__ __forceinline__ u64 wave_or(u64 x)
 {
 #pragma unroll
 	for (int o = 32; o > 0; o >>= 1) x |= __shfl_xor(x, o, 64);
+	return x;
+}
+
+__device__ __forceinline__ u64 wave_add(u64 x)
+{
+#pragma unroll
+	for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
 	return x;
 }
 
@@ -121,12 +146,24 @@ __device__ __forceinline__ void gather_fronts(const uint32_t (&h)[K], const uint
 		if ((use >> j) & 1u) or_front<NW>(front, h[j], acc);
 }
 
+// The pull kernels of a dense step: a variant runs only on a dense step and
+// only if its filter setting is the one the step's union frontier calls for
+// (the frontier filter pays while the union frontier is under 1/8 of the
+// vertices).
+template <bool kFilter>
+__device__ __forceinline__ bool pull_runs(const StepCtrl *ctrl, uint32_t n)
+{
+	return !ctrl->done && ctrl->dir == 1 && (ctrl->fsize < n / 8) == kFilter;
+}
+
 // Pull for vertices with 1..kPullLight in-edges.  Each lane first reads its
 // own vertex (in-edge range, want = sources not yet seen there whose frontier
 // is non-empty); the wavefront then serves its vertices that want something,
 // kLightGroup at a time, with kLightGroup lanes per vertex taking every
 // kLightGroup-th in-edge (at most kLightSlots each) and an OR across the
-// group.  next[v] is zero on entry and written only by v's group.
+// group.  next[v] is zero on entry and written only by v's group; the
+// wavefront's 64 vertices are two whole touched-bitmap words, which it
+// writes with plain stores (the heavy pull, a later launch, ORs into them).
 constexpr uint32_t kLightGroup = 8;
 constexpr uint32_t kLightSlots = kPullLight / kLightGroup;
 // a wavefront serves one vertex per group per round and takes kLightGroup
@@ -137,10 +174,12 @@ __device__ __forceinline__ uint32_t shfl_u32(uint32_t x, uint32_t src) { return 
 
 template <int NW, bool kFilter>
 __global__ __launch_bounds__(kBlock) void batch_pull_light_kernel(
-    uint32_t n, const uint32_t *__restrict__ in_off, const uint32_t *__restrict__ in_src,
-    const uint32_t *__restrict__ cur, const u64 *__restrict__ front, const u64 *__restrict__ seen,
-    const u64 *__restrict__ live, u64 *__restrict__ next)
+    uint32_t n, const StepCtrl *__restrict__ ctrl, const uint32_t *__restrict__ in_off,
+    const uint32_t *__restrict__ in_src, const uint32_t *__restrict__ cur, const u64 *__restrict__ front,
+    const u64 *__restrict__ seen, const u64 *__restrict__ live, u64 *__restrict__ next,
+    uint32_t *__restrict__ touched)
 {
+	if (!pull_runs<kFilter>(ctrl, n)) return;
 	const uint32_t lane = threadIdx.x & 63, g = lane % kLightGroup, grp = lane / kLightGroup;
 	const uint32_t v0 = blockIdx.x * kBlock + (threadIdx.x & ~63u);  // first vertex of the wavefront
 	const uint32_t v = v0 + lane;
@@ -157,6 +196,7 @@ __global__ __launch_bounds__(kBlock) void batch_pull_light_kernel(
 		any |= want[i];
 	}
 	u64 act = uniform64(__ballot(cnt != 0 && any != 0));
+	u64 got = 0;  // group leaders: the wavefront's vertices (bit = lane) that got a source
 	while (act) {
 		// the next kLightGroup wanting lanes of the wavefront, one per group
 		uint32_t pick[kLightGroup];
@@ -193,12 +233,23 @@ __global__ __launch_bounds__(kBlock) void batch_pull_light_kernel(
 			for (int o = kLightGroup / 2; o > 0; o >>= 1) acc[i] |= __shfl_xor(acc[i], o, kLightGroup);
 		if (g == 0 && gcnt) {
 			const uint64_t gv = v0 + src;
+			bool hit = false;
 #pragma unroll
 			for (int i = 0; i < NW; ++i) {
 				const u64 x = acc[i] & gwant[i];
-				if (x) next[gv * NW + i] = x;
+				if (x) {
+					next[gv * NW + i] = x;
+					hit = true;
+				}
 			}
+			if (hit) got |= 1ull << src;
 		}
+	}
+	got = wave_or(got);
+	if (lane == 0 && got) {
+		const uint32_t w0 = v0 >> 5;
+		if ((uint32_t)got) touched[w0] = (uint32_t)got;
+		if ((uint32_t)(got >> 32)) touched[w0 + 1] = (uint32_t)(got >> 32);
 	}
 }
 
@@ -207,10 +258,12 @@ __global__ __launch_bounds__(kBlock) void batch_pull_light_kernel(
 // Pieces of one vertex combine with one atomicOr per word.
 template <int NW, bool kFilter>
 __global__ __launch_bounds__(kBlock) void batch_pull_heavy_kernel(
-    uint32_t npieces, const uint2 *__restrict__ pieces, const uint32_t *__restrict__ in_off,
-    const uint32_t *__restrict__ in_src, const uint32_t *__restrict__ cur, const u64 *__restrict__ front,
-    const u64 *__restrict__ seen, const u64 *__restrict__ live, u64 *next)
+    uint32_t n, const StepCtrl *__restrict__ ctrl, uint32_t npieces, const uint2 *__restrict__ pieces,
+    const uint32_t *__restrict__ in_off, const uint32_t *__restrict__ in_src, const uint32_t *__restrict__ cur,
+    const u64 *__restrict__ front, const u64 *__restrict__ seen, const u64 *__restrict__ live, u64 *next,
+    uint32_t *touched)
 {
+	if (!pull_runs<kFilter>(ctrl, n)) return;
 	// loads in flight per lane between checks of the stop rule: a vertex stops
 	// as soon as every wanted source is found, so short rounds (64 or 128
 	// edges) beat deeper unrolling here
@@ -250,11 +303,16 @@ __global__ __launch_bounds__(kBlock) void batch_pull_heavy_kernel(
 			if (!miss) break;
 		}
 		if (lane == 0) {
+			bool hit = false;
 #pragma unroll
 			for (int i = 0; i < NW; ++i) {
 				const u64 x = red[i] & want[i];
-				if (x) atomicOr(next + (uint64_t)v * NW + i, x);
+				if (x) {
+					atomicOr(next + (uint64_t)v * NW + i, x);
+					hit = true;
+				}
 			}
+			if (hit) mark_touched(touched, v);
 		}
 	}
 }
@@ -263,14 +321,15 @@ __global__ __launch_bounds__(kBlock) void batch_pull_heavy_kernel(
 // single-source push: tiles of kSparseTile edges starting at tile_lo)
 template <int NW>
 __global__ __launch_bounds__(kBlock) void batch_sparse_kernel(
-    const uint32_t *__restrict__ queue, const uint32_t *__restrict__ qoff,
-    const uint32_t *__restrict__ tile_lo, uint32_t qsize, uint32_t qdeg,
-    const uint32_t *__restrict__ row_off, const uint32_t *__restrict__ cols,
-    const u64 *__restrict__ front, const u64 *__restrict__ seen, u64 *next)
+    const StepCtrl *__restrict__ ctrl, const uint32_t *__restrict__ queue, const uint32_t *__restrict__ qoff,
+    const uint32_t *__restrict__ tile_lo, const uint32_t *__restrict__ row_off, const uint32_t *__restrict__ cols,
+    const u64 *__restrict__ front, const u64 *__restrict__ seen, u64 *next, uint32_t *touched)
 {
 	__shared__ uint32_t s_off[kSparseTile + 2];
 	__shared__ uint32_t s_u[kSparseTile + 2];
 	__shared__ uint32_t s_base[kSparseTile + 2];
+	if (ctrl->done || ctrl->dir != 0) return;
+	const uint32_t qsize = ctrl->qsize, qdeg = ctrl->fdeg;
 	const uint32_t ntile = (qdeg + kSparseTile - 1) / kSparseTile;
 	for (uint32_t k = blockIdx.x; k < ntile; k += gridDim.x) {
 		const uint32_t lo = tile_lo[k];
@@ -296,89 +355,136 @@ __global__ __launch_bounds__(kBlock) void batch_sparse_kernel(
 			}
 			const uint32_t u = s_u[a];
 			const uint32_t v = cols[s_base[a] + (e - s_off[a])];
-			push_mask<NW>(load_mask<NW>(front, u), v, seen, next);
+			push_mask<NW>(load_mask<NW>(front, u), v, seen, next, touched);
 		}
 		__syncthreads();
 	}
 }
 
-// Per vertex: new = next & ~seen; seen |= new; front = new; next = 0; the
-// level of every new source is L+1; flag marks the union frontier for the
-// shared mark/scan/emit update.  Level bytes are written one source at a time
-// across the wavefront (64 consecutive vertices, one coalesced store), and each
-// block publishes the sources it saw advance into one of kLiveSlots slots.
+// The level update.  One lane per 32-vertex word of the touched and frontier
+// bitmaps; a wavefront then walks its 2048 vertices 64 at a time (vertex
+// 64i + lane), skipping runs of 64 with neither bit, so a sparse step costs
+// about its own vertices.  Per vertex: new = next & ~seen; seen |= new;
+// front = new; next = 0; a vertex of the old frontier with nothing new drops
+// its front mask.  The level of every new (source, vertex) pair is the level
+// minus the source's start level, written one source at a time across the
+// wavefront (64 consecutive vertices, one coalesced byte store).  Blocks
+// publish the sources they saw advance into one of kLiveSlots slots and add
+// the new pairs (and their out-degrees) to the totals.
 constexpr uint32_t kLiveSlots = 64;
 constexpr uint32_t kLiveStride = 16;  // u64 words per slot (one 128-byte line)
 
 template <int NW>
-__global__ __launch_bounds__(kBlock) void batch_update_kernel(uint32_t n, uint32_t n_pitch, uint32_t next_level,
-                                                              const uint32_t *__restrict__ cur,
-                                                              u64 *__restrict__ front, u64 *__restrict__ seen,
-                                                              u64 *__restrict__ next, uint8_t *__restrict__ level,
-                                                              uint8_t *__restrict__ flag, u64 *slots)
+__global__ __launch_bounds__(kBlock) void batch_update_kernel(
+    uint32_t n, uint32_t n_pitch, uint32_t next_level, StepCtrl *ctrl, const uint32_t *__restrict__ cur,
+    uint32_t *__restrict__ touched, u64 *__restrict__ front, u64 *__restrict__ seen, u64 *__restrict__ next,
+    uint8_t *__restrict__ level, uint8_t *__restrict__ flag, const uint32_t *__restrict__ slot_start,
+    const uint32_t *__restrict__ deg, u64 *slots, u64 *sums)
 {
+	if (ctrl->done) return;
 	__shared__ u64 s_or[kBlock / 64][NW];
-	const uint32_t v = blockIdx.x * kBlock + threadIdx.x;
-	const bool in = v < n;
-	// next first: a vertex nothing reached this step only has to drop its
-	// front mask, and only if it was in the (union) frontier, whose bitmap is
-	// cur -- so most vertices cost one read of next
-	u64 nx[NW], any_nx = 0;
+	__shared__ u64 s_cnt[kBlock / 64][2];
+	const uint32_t lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+	const uint32_t nwords = (n + 31) / 32;
+	const uint32_t w = blockIdx.x * kBlock + threadIdx.x;
+	const uint32_t tw = w < nwords ? touched[w] : 0u;
+	const uint32_t cw = w < nwords ? cur[w] : 0u;
+	if (tw) touched[w] = 0u;  // clear for the next step
+	const u64 act = uniform64(__ballot(tw != 0u) | __ballot(cw != 0u));
+	const uint32_t wv0 = (w - lane) * 32;
+	u64 wor[NW];
 #pragma unroll
-	for (int i = 0; i < NW; ++i) {
-		nx[i] = in ? next[(uint64_t)v * NW + i] : 0ull;
-		any_nx |= nx[i];
-	}
-	const bool was_front = in && test_bit32(cur, v);
-	u64 nwm[NW];
-	bool any = false;
+	for (int j = 0; j < NW; ++j) wor[j] = 0;
+	u64 pairs = 0, pdeg = 0;
+	bool over = false;
+	for (uint32_t i = 0; i < 32; ++i) {
+		if (!((act >> (2 * i)) & 3ull)) continue;  // uniform
+		const uint32_t tb = shfl_u32(tw, 2 * i + (lane >> 5));
+		const uint32_t cb = shfl_u32(cw, 2 * i + (lane >> 5));
+		const uint32_t v = wv0 + 64 * i + lane;
+		const bool t = (tb >> (lane & 31)) & 1u, c = (cb >> (lane & 31)) & 1u;
+		u64 nw[NW];
+		u64 anyn = 0;
 #pragma unroll
-	for (int i = 0; i < NW; ++i) {
-		nwm[i] = 0;
-		const uint64_t idx = (uint64_t)v * NW + i;
-		if (any_nx) {
-			const u64 sn = seen[idx];
-			const u64 nw = nx[i] & ~sn;
-			if (nw || was_front) front[idx] = nw;
-			if (nx[i]) next[idx] = 0;
-			if (nw) {
-				seen[idx] = sn | nw;
-				any = true;
+		for (int j = 0; j < NW; ++j) {
+			nw[j] = 0;
+			const uint64_t idx = (uint64_t)v * NW + j;
+			if (t) {
+				const u64 nx = next[idx];
+				const u64 sn = seen[idx];
+				nw[j] = nx & ~sn;
+				front[idx] = nw[j];
+				if (nx) next[idx] = 0;
+				if (nw[j]) seen[idx] = sn | nw[j];
+			} else if (c) {
+				front[idx] = 0;
 			}
-			nwm[i] = nw;
-		} else if (was_front) {
-			front[idx] = 0;
+			anyn |= nw[j];
 		}
-	}
-	const uint32_t wid = threadIdx.x >> 6;
+		if (anyn) {
+			flag[v] = 1;
+			uint32_t pc = 0;
 #pragma unroll
-	for (int i = 0; i < NW; ++i) {
-		u64 u = uniform64(wave_or(nwm[i]));
-		if ((threadIdx.x & 63) == 0) s_or[wid][i] = u;
-		while (u) {
-			const uint32_t b = (uint32_t)__ffsll((long long)u) - 1;
-			u &= u - 1;
-			if ((nwm[i] >> b) & 1) level[(uint64_t)(i * 64 + b) * n_pitch + v] = (uint8_t)next_level;
+			for (int j = 0; j < NW; ++j) pc += (uint32_t)__popcll(nw[j]);
+			pairs += pc;
+			pdeg += (u64)pc * deg[v];
+		}
+#pragma unroll
+		for (int j = 0; j < NW; ++j) {
+			u64 u = uniform64(wave_or(nw[j]));
+			wor[j] |= u;
+			while (u) {
+				const uint32_t b = (uint32_t)__ffsll((long long)u) - 1;
+				u &= u - 1;
+				const uint32_t s = j * 64 + b;
+				const uint32_t rel = next_level - slot_start[s];
+				over |= rel >= kUnreached;
+				if ((nw[j] >> b) & 1) level[(uint64_t)s * n_pitch + v] = (uint8_t)rel;
+			}
 		}
 	}
-	if (any) flag[v] = 1;
+	pairs = wave_add(pairs);
+	pdeg = wave_add(pdeg);
+	if (lane == 0) {
+#pragma unroll
+		for (int j = 0; j < NW; ++j) s_or[wid][j] = wor[j];
+		s_cnt[wid][0] = pairs;
+		s_cnt[wid][1] = pdeg;
+		if (over) {
+			// a level of 255 or more would read as unreached: stop here
+			ctrl->err = 1;
+			ctrl->done = 1;
+		}
+	}
 	__syncthreads();
 	if (threadIdx.x < NW) {
 		u64 u = 0;
 #pragma unroll
-		for (uint32_t w = 0; w < kBlock / 64; ++w) u |= s_or[w][threadIdx.x];
+		for (uint32_t k = 0; k < kBlock / 64; ++k) u |= s_or[k][threadIdx.x];
 		u64 *slot = slots + (uint64_t)(blockIdx.x % kLiveSlots) * kLiveStride + threadIdx.x;
 		// read the slot at L2 (not a stale L1 copy) and add only new sources
 		const u64 have = __hip_atomic_load(slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 		if (u & ~have) atomicOr(slot, u);
+	} else if (threadIdx.x == 64) {
+		u64 p = 0, d = 0;
+#pragma unroll
+		for (uint32_t k = 0; k < kBlock / 64; ++k) {
+			p += s_cnt[k][0];
+			d += s_cnt[k][1];
+		}
+		if (p) {
+			atomicAdd(sums, p);
+			atomicAdd(sums + 1, d);
+		}
 	}
 }
 
 // live = OR of the slots (the sources whose new frontier is non-empty); the
 // slots are cleared for the next step.  One wavefront.
 template <int NW>
-__global__ void batch_live_kernel(u64 *live, u64 *slots)
+__global__ void batch_live_kernel(const StepCtrl *__restrict__ ctrl, u64 *live, u64 *slots)
 {
+	if (ctrl->done) return;
 	const uint32_t lane = threadIdx.x;
 #pragma unroll
 	for (int i = 0; i < NW; ++i) {
@@ -389,57 +495,79 @@ __global__ void batch_live_kernel(u64 *live, u64 *slots)
 	}
 }
 
-__global__ void batch_seed_kernel(const uint32_t *__restrict__ srcs, uint32_t k, uint32_t nw,
-                                  uint32_t n_pitch, u64 *front, u64 *seen, uint8_t *level,
-                                  uint8_t *flag, u64 *live)
+// Sources joining the traversal (at its start, or between two levels): slot
+// s gets vertex v at level 0 relative to start_level; v enters the union
+// frontier of the next step through its state byte.  One block.
+__global__ void batch_seed_kernel(const uint32_t *__restrict__ aux, uint32_t count, uint32_t start_level,
+                                  uint32_t nw, uint32_t n_pitch, u64 *front, u64 *seen, uint8_t *level,
+                                  uint8_t *flag, u64 *live, uint32_t *slot_start, const uint32_t *__restrict__ deg,
+                                  u64 *sums)
 {
-	for (uint32_t i = threadIdx.x; i < k; i += blockDim.x) {
-		const uint32_t v = srcs[i];
-		atomicOr(front + (uint64_t)v * nw + i / 64, 1ull << (i % 64));
-		atomicOr(seen + (uint64_t)v * nw + i / 64, 1ull << (i % 64));
-		atomicOr(live + i / 64, 1ull << (i % 64));
-		level[(uint64_t)i * n_pitch + v] = 0;
+	for (uint32_t i = threadIdx.x; i < count; i += blockDim.x) {
+		const uint32_t v = aux[kAuxSrc + i], s = aux[kAuxSlot + i];
+		const u64 bit = 1ull << (s % 64);
+		atomicOr(front + (uint64_t)v * nw + s / 64, bit);
+		atomicOr(seen + (uint64_t)v * nw + s / 64, bit);
+		atomicOr(live + s / 64, bit);
+		level[(uint64_t)s * n_pitch + v] = 0;
+		slot_start[s] = start_level;
 		flag[v] = 1;
+		atomicAdd(sums, 1ull);
+		atomicAdd(sums + 1, (u64)deg[v]);
 	}
 }
 
-// level 255 for every (source, vertex) pair the traversal did not reach: the
-// level bytes are written for reached pairs only, so this runs only when the
-// levels are copied out.  One lane per vertex, one coalesced store per source.
+// Per-traversal clears of the masks, state bytes, bitmaps, live words, slot
+// start levels and totals; the control block is set up for the first
+// frontier update (which opens level 0).
+__global__ __launch_bounds__(kBlock) void batch_clear_kernel(uint64_t mask_words, uint32_t n_pad, uint32_t n_words,
+                                                             u64 *front, u64 *seen, u64 *next, uint8_t *flag,
+                                                             uint32_t *cur, uint32_t *touched, u64 *live_all,
+                                                             uint32_t live_words, uint32_t *slot_start,
+                                                             u64 *sums, StepCtrl *ctrl)
+{
+	const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+	const uint64_t i0 = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+	for (uint64_t i = i0; i < mask_words; i += stride) {
+		front[i] = 0;
+		seen[i] = 0;
+		next[i] = 0;
+	}
+	for (uint64_t i = i0; i < n_pad / 16; i += stride) reinterpret_cast<uint4 *>(flag)[i] = make_uint4(0, 0, 0, 0);
+	for (uint64_t i = i0; i < n_words; i += stride) {
+		cur[i] = 0;
+		touched[i] = 0;
+	}
+	for (uint64_t i = i0; i < live_words; i += stride) live_all[i] = 0;
+	for (uint64_t i = i0; i < kBatchMaxSources; i += stride) slot_start[i] = 0;
+	if (i0 < 2) sums[i0] = 0;
+	if (i0 == 0) {
+		ctrl->level = UINT32_MAX;  // the first frontier update opens level 0
+		ctrl->done = 0;
+		ctrl->err = 0;
+		ctrl->dir = 0;
+		ctrl->reached = 0;
+		ctrl->traversed = 0;
+	}
+}
+
+// level 255 for every (slot, vertex) pair of the given slots the traversal
+// did not reach: the level bytes are written for reached pairs only, so this
+// runs only when levels are read out.  One lane per vertex, one coalesced
+// store per slot.
 template <int NW>
-__global__ __launch_bounds__(kBlock) void batch_finalize_kernel(uint32_t n, uint32_t n_pitch, uint32_t k,
+__global__ __launch_bounds__(kBlock) void batch_finalize_kernel(uint32_t n, uint32_t n_pitch, Mask<NW> which,
                                                                 const u64 *__restrict__ seen, uint8_t *level)
 {
 	const uint32_t v = blockIdx.x * kBlock + threadIdx.x;
 	if (v >= n) return;
 #pragma unroll
 	for (int i = 0; i < NW; ++i) {
-		const u64 sn = seen[(uint64_t)v * NW + i];
-		for (uint32_t b = 0; b < 64 && i * 64 + b < k; ++b)
-			if (!((sn >> b) & 1ull)) level[(uint64_t)(i * 64 + b) * n_pitch + v] = kUnreached;
-	}
-}
-
-// totals: sum over v of popcount(seen[v]) and of deg[v] * popcount(seen[v])
-template <int NW>
-__global__ __launch_bounds__(kBlock) void batch_sums_kernel(uint32_t n, const u64 *__restrict__ seen,
-                                                            const uint32_t *__restrict__ deg, u64 *sums)
-{
-	u64 r = 0, e = 0;
-	for (uint32_t v = blockIdx.x * kBlock + threadIdx.x; v < n; v += gridDim.x * kBlock) {
-		uint32_t c = 0;
-#pragma unroll
-		for (int i = 0; i < NW; ++i) c += (uint32_t)__popcll(seen[(uint64_t)v * NW + i]);
-		r += c;
-		e += (u64)c * deg[v];
-	}
-	for (int o = 32; o > 0; o >>= 1) {
-		r += __shfl_xor(r, o, 64);
-		e += __shfl_xor(e, o, 64);
-	}
-	if ((threadIdx.x & 63) == 0) {
-		atomicAdd(sums, r);
-		atomicAdd(sums + 1, e);
+		const u64 miss = which.w[i] & ~seen[(uint64_t)v * NW + i];
+		for (u64 u = which.w[i]; u; u &= u - 1) {
+			const uint32_t b = (uint32_t)__ffsll((long long)u) - 1;
+			if ((miss >> b) & 1ull) level[(uint64_t)(i * 64 + b) * n_pitch + v] = kUnreached;
+		}
 	}
 }
 
@@ -451,176 +579,329 @@ float ev_ms(hipEvent_t a, hipEvent_t b)
 	return hipEventElapsedTime(&ms, a, b) == hipSuccess ? ms : 0.f;
 }
 
-template <int NW, bool kFilter>
-void launch_pull(gphi_graph *gr, const u64 *live)
+constexpr unsigned kBatchSparseGrid = 2048;
+constexpr uint32_t kBatchLevelGroup = 8;  // levels enqueued per read-back of the control block
+
+u64 *live_words(gphi_graph *gr) { return gr->b_live; }
+u64 *live_slots(gphi_graph *gr) { return gr->b_live + kLiveStride; }
+uint32_t *slot_starts(gphi_graph *gr) { return gr->b_aux + kAuxStart; }
+
+// The expansion of one level and its update: the CSR push (sparse step) or
+// the pulls (dense step) -- each kernel returns at once unless the control
+// block says it is its step -- then the update of the vertices the step
+// touched, then the live-source mask.
+template <int NW>
+void enqueue_expand(gphi_graph *gr, uint32_t next_level)
 {
 	const DevGraph &G = gr->g;
 	hipStream_t s = gr->ctx->stream;
-	batch_pull_light_kernel<NW, kFilter><<<grid_n(G.n), kBlock, 0, s>>>(
-	    G.n, gr->in_off, gr->in_src, gr->cur, gr->b_front, gr->b_seen, live, gr->b_next);
-	if (gr->npieces)
-		batch_pull_heavy_kernel<NW, kFilter><<<std::min<uint32_t>((gr->npieces + 3) / 4, 16384), kBlock, 0, s>>>(
-		    gr->npieces, gr->pieces, gr->in_off, gr->in_src, gr->cur, gr->b_front, gr->b_seen, live, gr->b_next);
+	batch_sparse_kernel<NW><<<kBatchSparseGrid, kBlock, 0, s>>>(gr->ctrl, gr->queue, gr->qoff, gr->tile_lo, G.row_off,
+	                                                           G.cols, gr->b_front, gr->b_seen, gr->b_next, gr->b_touch);
+	const unsigned lg = grid_n(G.n);
+	const u64 *live = live_words(gr);
+	batch_pull_light_kernel<NW, true><<<lg, kBlock, 0, s>>>(G.n, gr->ctrl, gr->in_off, gr->in_src, gr->cur, gr->b_front,
+	                                                       gr->b_seen, live, gr->b_next, gr->b_touch);
+	batch_pull_light_kernel<NW, false><<<lg, kBlock, 0, s>>>(G.n, gr->ctrl, gr->in_off, gr->in_src, gr->cur, gr->b_front,
+	                                                        gr->b_seen, live, gr->b_next, gr->b_touch);
+	if (gr->npieces) {
+		const unsigned hg = std::min<uint32_t>((gr->npieces + 3) / 4, 16384);
+		batch_pull_heavy_kernel<NW, true><<<hg, kBlock, 0, s>>>(G.n, gr->ctrl, gr->npieces, gr->pieces, gr->in_off,
+		                                                       gr->in_src, gr->cur, gr->b_front, gr->b_seen, live,
+		                                                       gr->b_next, gr->b_touch);
+		batch_pull_heavy_kernel<NW, false><<<hg, kBlock, 0, s>>>(G.n, gr->ctrl, gr->npieces, gr->pieces, gr->in_off,
+		                                                        gr->in_src, gr->cur, gr->b_front, gr->b_seen, live,
+		                                                        gr->b_next, gr->b_touch);
+	}
 }
 
 template <int NW>
-int batch_loop(gphi_graph *gr, const uint32_t *d_srcs, uint32_t k, gphi_stats *st, u64 *sums_host)
+void enqueue_update(gphi_graph *gr, uint32_t next_level)
 {
-	gphi_ctx *ctx = gr->ctx;
-	DevGraph &G = gr->g;
-	hipStream_t s = ctx->stream;
-	const uint32_t threshold = (uint32_t)G.m / ctx->t_ratio;
-	StepCtrl *hc = ctx->h_ctrl;
-	hipEvent_t *ev = ctx->ev;
-	// total time from two events; per-step events only with level timing on
-	const bool total_timing = st != nullptr && ctx->ev_ok;
-	const bool timing = total_timing && ctx->level_timing;
-	const size_t mask_bytes = (size_t)G.n_pad * NW * 8;
-	if (total_timing) GPHI_HIP(ctx, hipEventRecord(ev[0], s));
-	GPHI_HIP(ctx, hipMemsetAsync(gr->b_front, 0, mask_bytes, s));
-	GPHI_HIP(ctx, hipMemsetAsync(gr->b_seen, 0, mask_bytes, s));
-	GPHI_HIP(ctx, hipMemsetAsync(gr->b_next, 0, mask_bytes, s));
-	GPHI_HIP(ctx, hipMemsetAsync(gr->flag, 0, G.n_pad, s));
-	GPHI_HIP(ctx, hipMemsetAsync(gr->cur, 0, (size_t)G.n_words * 4, s));
-	GPHI_HIP(ctx, hipMemsetAsync(gr->b_live, 0, (size_t)(1 + kLiveSlots) * kLiveStride * 8, s));
-	u64 *live = gr->b_live, *slots = gr->b_live + kLiveStride;
-	launch_init_blocks(gr);
-	batch_seed_kernel<<<1, kBlock, 0, s>>>(d_srcs, k, NW, G.n_pad, gr->b_front, gr->b_seen, gr->b_level,
-	                                       gr->flag, live);
-	// the sources form the first (union) frontier
-	launch_frontier_update_flags(gr, 0, threshold);
-	GPHI_HIP(ctx, hipGetLastError());
-	if (timing) GPHI_HIP(ctx, hipEventRecord(ev[1], s));
-	GPHI_HIP(ctx, hipMemcpyAsync(hc, gr->ctrl, sizeof(StepCtrl), hipMemcpyDeviceToHost, s));
-	GPHI_HIP(ctx, hipStreamSynchronize(s));
-	uint32_t steps = 0;
-	while (!hc->done) {
-		const uint32_t L = steps;
-		// the rule applies to the union frontier from the first step on
-		const uint32_t dir = hc->dir;
-		if (st && steps < GPHI_MAX_STEPS) {
-			st->dir[steps] = dir;
-			st->fsize[steps] = hc->fsize;
-			st->fdeg[steps] = hc->fdeg;
-		}
-		const bool ev_step = timing && steps < GPHI_MAX_STEPS - 1;
-		if (dir == 0) {
-			if (hc->fdeg)
-				batch_sparse_kernel<NW><<<(unsigned)std::min<uint64_t>((hc->fdeg + kSparseTile - 1) / kSparseTile, 4096),
-				                          kBlock, 0, s>>>(gr->queue, gr->qoff, gr->tile_lo, hc->qsize, hc->fdeg,
-				                                          G.row_off, G.cols, gr->b_front, gr->b_seen, gr->b_next);
-		} else {
-			// the frontier filter pays while the union frontier is under 1/8 of the vertices
-			if (hc->fsize < G.n / 8)
-				launch_pull<NW, true>(gr, live);
-			else
-				launch_pull<NW, false>(gr, live);
-		}
-		GPHI_HIP(ctx, hipGetLastError());
-		if (ev_step) GPHI_HIP(ctx, hipEventRecord(ev[2 + 2 * steps], s));
-		batch_update_kernel<NW><<<grid_n(G.n), kBlock, 0, s>>>(G.n, G.n_pad, L + 1, gr->cur, gr->b_front,
-		                                                     gr->b_seen, gr->b_next, gr->b_level, gr->flag, slots);
-		batch_live_kernel<NW><<<1, 64, 0, s>>>(live, slots);
-		launch_frontier_update_flags(gr, L + 1, threshold);
-		GPHI_HIP(ctx, hipGetLastError());
-		if (ev_step) GPHI_HIP(ctx, hipEventRecord(ev[3 + 2 * steps], s));
-		GPHI_HIP(ctx, hipMemcpyAsync(hc, gr->ctrl, sizeof(StepCtrl), hipMemcpyDeviceToHost, s));
-		GPHI_HIP(ctx, hipStreamSynchronize(s));
-		// a pair reached at level 255 would read as unreached: the expansion
-		// of level 254 may run, but only if it reaches nothing new
-		if (L + 1 >= kUnreached && !hc->done)
-			return set_error(ctx, GPHI_E_RANGE, "a batched BFS deeper than 254 levels does not fit uint8 levels");
-		++steps;
-		if (steps > G.n + 2) return set_error(ctx, GPHI_E_STATE, "batched traversal did not terminate");
-	}
-	GPHI_HIP(ctx, hipMemsetAsync(gr->b_sums, 0, 16, s));
-	batch_sums_kernel<NW><<<std::min<unsigned>(grid_n(G.n), 4096), kBlock, 0, s>>>(G.n, gr->b_seen, G.deg,
-	                                                                               gr->b_sums);
-	if (total_timing) GPHI_HIP(ctx, hipEventRecord(ev[2 * GPHI_MAX_STEPS + 2], s));
-	GPHI_HIP(ctx, hipMemcpyAsync(sums_host, gr->b_sums, 16, hipMemcpyDeviceToHost, s));
-	GPHI_HIP(ctx, hipStreamSynchronize(s));
-	if (st) {
-		st->steps = std::min<uint32_t>(steps, GPHI_MAX_STEPS);
-		if (timing) {
-			st->init_ms = ev_ms(ev[0], ev[1]);
-			hipEvent_t prev = ev[1];
-			const uint32_t ns = std::min<uint32_t>(steps, GPHI_MAX_STEPS - 1);
-			for (uint32_t q = 0; q < ns; ++q) {
-				st->sweep_ms[q] = ev_ms(prev, ev[2 + 2 * q]);
-				st->update_ms[q] = ev_ms(ev[2 + 2 * q], ev[3 + 2 * q]);
-				prev = ev[3 + 2 * q];
-			}
-		} else {
-			memset(st->sweep_ms, 0, sizeof(st->sweep_ms));
-			memset(st->update_ms, 0, sizeof(st->update_ms));
-			st->init_ms = 0.f;
-		}
-		st->total_ms = total_timing ? ev_ms(ev[0], ev[2 * GPHI_MAX_STEPS + 2]) : 0.f;
-	}
+	const DevGraph &G = gr->g;
+	hipStream_t s = gr->ctx->stream;
+	const uint32_t nwords = (G.n + 31) / 32;
+	batch_update_kernel<NW><<<grid_n(nwords), kBlock, 0, s>>>(
+	    G.n, G.n_pad, next_level, gr->ctrl, gr->cur, gr->b_touch, gr->b_front, gr->b_seen, gr->b_next, gr->b_level,
+	    gr->flag, slot_starts(gr), G.deg, live_slots(gr), gr->b_sums);
+	batch_live_kernel<NW><<<1, 64, 0, s>>>(gr->ctrl, live_words(gr), live_slots(gr));
+}
+
+template <int NW>
+int launch_finalize(gphi_graph *gr, const u64 *which)
+{
+	const DevGraph &G = gr->g;
+	Mask<NW> m;
+	for (int i = 0; i < NW; ++i) m.w[i] = which[i];
+	batch_finalize_kernel<NW><<<grid_n(G.n), kBlock, 0, gr->ctx->stream>>>(G.n, G.n_pad, m, gr->b_seen, gr->b_level);
+	return hipGetLastError() == hipSuccess ? GPHI_OK : GPHI_E_HIP;
+}
+
+int finalize_rows(gphi_graph *gr, uint32_t nw, const u64 *which)
+{
+	int rc = nw == 1 ? launch_finalize<1>(gr, which) : nw == 2 ? launch_finalize<2>(gr, which) : launch_finalize<4>(gr, which);
+	if (rc) return set_error(gr->ctx, rc, "batched BFS: finalize launch failed");
 	return GPHI_OK;
 }
 
+void enqueue_level(gphi_graph *gr, uint32_t nw, uint32_t next_level, bool expand_half)
+{
+	if (expand_half) {
+		if (nw == 1) enqueue_expand<1>(gr, next_level);
+		else if (nw == 2) enqueue_expand<2>(gr, next_level);
+		else enqueue_expand<4>(gr, next_level);
+	} else {
+		if (nw == 1) enqueue_update<1>(gr, next_level);
+		else if (nw == 2) enqueue_update<2>(gr, next_level);
+		else enqueue_update<4>(gr, next_level);
+	}
+}
+
+// the frontier update of the union frontier (mark / scan / emit of the
+// single-source loop), in its device-driven form: it opens the next level
+void enqueue_frontier(gphi_graph *gr) { launch_frontier_update_driven(gr, (uint32_t)gr->g.m / gr->ctx->t_ratio); }
+
 }  // namespace
 
-int run_bfs_batch(gphi_graph *gr, const uint32_t *srcs, uint32_t k, uint8_t *levels, gphi_stats *st)
+// ---------------------------------------------------------------------------
+// host driver
+// ---------------------------------------------------------------------------
+
+int batch_prepare(gphi_graph *gr, uint32_t nw)
 {
 	gphi_ctx *ctx = gr->ctx;
 	DevGraph &G = gr->g;
-	if (k == 0 || k > 256) return set_error(ctx, GPHI_E_ARG, "batch size %u not in [1, 256]", k);
-	if (!srcs) return set_error(ctx, GPHI_E_ARG, "sources are NULL");
-	for (uint32_t i = 0; i < k; ++i)
-		if (srcs[i] >= G.n) return set_error(ctx, GPHI_E_RANGE, "source %u >= N %u", srcs[i], G.n);
 	if (!G.has_csr || !G.has_tiles) return set_error(ctx, GPHI_E_STATE, "graph needs CSR and tile arrays");
 	int rc = ensure_traversal_state(gr);
 	if (rc) return rc;
 	if ((rc = build_in_edges(gr))) return rc;
-	const uint32_t nw = k <= 64 ? 1 : k <= 128 ? 2 : 4;
-	if (gr->batch_words < nw || !gr->b_level) {
-		void *ps[] = {gr->b_front, gr->b_seen, gr->b_next, gr->b_level, gr->b_sums, gr->b_live};
-		for (void *p : ps)
-			if (p) (void)hipFree(p);
-		gr->b_front = gr->b_seen = gr->b_next = gr->b_sums = gr->b_live = nullptr;
-		gr->b_level = nullptr;
-		gr->batch_words = 0;
-		const size_t mb = (size_t)G.n_pad * nw * 8;
-		hipError_t e = hipMalloc((void **)&gr->b_front, mb);
-		if (e == hipSuccess) e = hipMalloc((void **)&gr->b_seen, mb);
-		if (e == hipSuccess) e = hipMalloc((void **)&gr->b_next, mb);
-		if (e == hipSuccess) e = hipMalloc((void **)&gr->b_level, (size_t)nw * 64 * G.n_pad);
-		if (e == hipSuccess) e = hipMalloc((void **)&gr->b_sums, 16 + 256 * 4);  // totals, then the sources
-		if (e == hipSuccess) e = hipMalloc((void **)&gr->b_live, (size_t)(1 + kLiveSlots) * kLiveStride * 8);
-		if (e != hipSuccess)
-			return set_error(ctx, GPHI_E_OOM, "batched BFS buffers (%zu MB): %s",
-			                 (3 * mb + (size_t)nw * 64 * G.n_pad) >> 20, hipGetErrorString(e));
-		gr->batch_words = nw;
-	}
-	hipStream_t s = ctx->stream;
-	uint32_t *d_srcs = reinterpret_cast<uint32_t *>(gr->b_sums + 2);
-	GPHI_HIP(ctx, hipMemcpyAsync(d_srcs, srcs, (size_t)k * 4, hipMemcpyHostToDevice, s));
-	if (st) memset(st, 0, sizeof(*st));
-	u64 sums[2] = {0, 0};
-	// the buffers hold gr->batch_words >= nw words per vertex; a batch runs at
-	// the width its own k needs (a 64-source batch after a 256-source one
-	// would otherwise gather and update four words per vertex)
-	switch (nw) {
-	case 1: rc = batch_loop<1>(gr, d_srcs, k, st, sums); break;
-	case 2: rc = batch_loop<2>(gr, d_srcs, k, st, sums); break;
-	default: rc = batch_loop<4>(gr, d_srcs, k, st, sums); break;
-	}
+	if (gr->batch_words >= nw && gr->b_level) return GPHI_OK;
+	void *ps[] = {gr->b_front, gr->b_seen, gr->b_next, gr->b_level, gr->b_sums, gr->b_live, gr->b_touch, gr->b_aux};
+	for (void *p : ps)
+		if (p) (void)hipFree(p);
+	gr->b_front = gr->b_seen = gr->b_next = gr->b_sums = gr->b_live = nullptr;
+	gr->b_level = nullptr;
+	gr->b_touch = gr->b_aux = nullptr;
+	gr->batch_words = 0;
+	const size_t mb = (size_t)G.n_pad * nw * 8;
+	hipError_t e = hipMalloc((void **)&gr->b_front, mb);
+	if (e == hipSuccess) e = hipMalloc((void **)&gr->b_seen, mb);
+	if (e == hipSuccess) e = hipMalloc((void **)&gr->b_next, mb);
+	if (e == hipSuccess) e = hipMalloc((void **)&gr->b_level, (size_t)nw * 64 * G.n_pad);
+	if (e == hipSuccess) e = hipMalloc((void **)&gr->b_sums, 16);
+	if (e == hipSuccess) e = hipMalloc((void **)&gr->b_live, (size_t)(1 + kLiveSlots) * kLiveStride * 8);
+	if (e == hipSuccess) e = hipMalloc((void **)&gr->b_touch, (size_t)G.n_words * 4);
+	if (e == hipSuccess) e = hipMalloc((void **)&gr->b_aux, (size_t)kAuxWords * 4);
+	if (e != hipSuccess)
+		return set_error(ctx, GPHI_E_OOM, "batched BFS buffers (%zu MB): %s",
+		                 (3 * mb + (size_t)nw * 64 * G.n_pad) >> 20, hipGetErrorString(e));
+	gr->batch_words = nw;
+	return GPHI_OK;
+}
+
+int BatchRun::begin(gphi_graph *g, uint32_t capacity, const uint32_t *srcs, uint32_t k)
+{
+	gr = g;
+	gphi_ctx *ctx = gr->ctx;
+	DevGraph &G = gr->g;
+	if (capacity == 0 || capacity > kBatchMaxSources)
+		return set_error(ctx, GPHI_E_ARG, "batch capacity %u not in [1, %u]", capacity, kBatchMaxSources);
+	if (k > capacity) return set_error(ctx, GPHI_E_ARG, "%u sources exceed the batch capacity %u", k, capacity);
+	if (k && !srcs) return set_error(ctx, GPHI_E_ARG, "sources are NULL");
+	for (uint32_t i = 0; i < k; ++i)
+		if (srcs[i] >= G.n) return set_error(ctx, GPHI_E_RANGE, "source %u >= N %u", srcs[i], G.n);
+	nw = capacity <= 64 ? 1 : capacity <= 128 ? 2 : 4;
+	cap = nw * 64;
+	int rc = batch_prepare(gr, nw);
 	if (rc) return rc;
+	used = 0;
+	levels = 0;
+	done = false;
+	slot_src.clear();
+	slot_start.clear();
+	memset(live, 0, sizeof live);
+	memset(pairs, 0, sizeof pairs);
+	hipStream_t s = ctx->stream;
+	// the buffers hold gr->batch_words >= nw words per vertex; a traversal runs
+	// at its own width nw (the masks are addressed with that pitch)
+	const uint64_t mask_words = (uint64_t)G.n_pad * nw;
+	const uint64_t items = std::max<uint64_t>({mask_words, G.n_pad / 16, G.n_words, kBatchMaxSources});
+	batch_clear_kernel<<<(unsigned)std::min<uint64_t>(grid_n(items), 16384), kBlock, 0, s>>>(
+	    mask_words, G.n_pad, G.n_words, gr->b_front, gr->b_seen, gr->b_next, gr->flag, gr->cur, gr->b_touch,
+	    gr->b_live, (1 + kLiveSlots) * kLiveStride, slot_starts(gr), gr->b_sums, gr->ctrl);
+	GPHI_HIP(ctx, hipGetLastError());
+	if ((rc = seed(srcs, k, 0))) return rc;
+	// the first frontier update: the sources form the union frontier of level 0
+	enqueue_frontier(gr);
+	GPHI_HIP(ctx, hipGetLastError());
+	return GPHI_OK;
+}
+
+int BatchRun::seed(const uint32_t *srcs, uint32_t k, uint32_t start_level)
+{
+	if (k == 0) return GPHI_OK;
+	gphi_ctx *ctx = gr->ctx;
+	const DevGraph &G = gr->g;
+	hipStream_t s = ctx->stream;
+	for (uint32_t i = 0; i < k; ++i) {
+		aux[kAuxSrc + i] = srcs[i];
+		aux[kAuxSlot + i] = used + i;
+	}
+	// aux is not written again before the stream is synchronized: every
+	// caller of seed synchronizes before its next seed
+	GPHI_HIP(ctx, hipMemcpyAsync(gr->b_aux + kAuxSrc, aux + kAuxSrc, (size_t)k * 4, hipMemcpyHostToDevice, s));
+	GPHI_HIP(ctx, hipMemcpyAsync(gr->b_aux + kAuxSlot, aux + kAuxSlot, (size_t)k * 4, hipMemcpyHostToDevice, s));
+	batch_seed_kernel<<<1, kBlock, 0, s>>>(gr->b_aux, k, start_level, nw, G.n_pad, gr->b_front, gr->b_seen,
+	                                        gr->b_level, gr->flag, live_words(gr), slot_starts(gr), G.deg,
+	                                        gr->b_sums);
+	GPHI_HIP(ctx, hipGetLastError());
+	for (uint32_t i = 0; i < k; ++i) {
+		slot_src.push_back(srcs[i]);
+		slot_start.push_back(start_level);
+	}
+	used += k;
+	return GPHI_OK;
+}
+
+// Runs levels on the device until the traversal ends, kBatchLevelGroup levels
+// per read-back of the control block (no admissions).
+int BatchRun::run_to_end(gphi_stats *st)
+{
+	gphi_ctx *ctx = gr->ctx;
+	hipStream_t s = ctx->stream;
+	StepCtrl *hc = ctx->h_ctrl;
+	hipEvent_t *ev = ctx->ev;
+	const bool total_timing = st != nullptr && ctx->ev_ok;
+	const bool timing = total_timing && ctx->level_timing;
+	constexpr uint32_t kEvEnd = 3 * GPHI_MAX_STEPS + 3;
+	if (timing) GPHI_HIP(ctx, hipEventRecord(ev[1], s));
+	uint32_t enq = 0;
+	for (;;) {
+		for (uint32_t q = 0; q < kBatchLevelGroup; ++q, ++enq) {
+			const bool ev_step = timing && enq < GPHI_MAX_STEPS - 1;
+			enqueue_level(gr, nw, enq + 1, true);
+			if (ev_step) GPHI_HIP(ctx, hipEventRecord(ev[2 + 2 * enq], s));
+			enqueue_level(gr, nw, enq + 1, false);
+			enqueue_frontier(gr);
+			if (ev_step) GPHI_HIP(ctx, hipEventRecord(ev[3 + 2 * enq], s));
+		}
+		GPHI_HIP(ctx, hipGetLastError());
+		if (total_timing && !timing) GPHI_HIP(ctx, hipEventRecord(ev[kEvEnd], s));
+		GPHI_HIP(ctx, hipMemcpyAsync(hc, gr->ctrl, sizeof(StepCtrl), hipMemcpyDeviceToHost, s));
+		GPHI_HIP(ctx, hipStreamSynchronize(s));
+		if (hc->err) return set_error(ctx, GPHI_E_RANGE, "a batched BFS deeper than 254 levels does not fit uint8 levels");
+		if (hc->done) break;
+		// every level ends the traversal, advances it, or (past level 254) stops it with err
+		if (enq > 256 + kBatchLevelGroup)
+			return set_error(ctx, GPHI_E_STATE, "batched traversal did not terminate");
+	}
+	levels = hc->level;
+	done = true;
 	if (st) {
-		st->reached = sums[0];
-		st->edges_traversed = sums[1];
+		const uint32_t ns = std::min<uint32_t>(levels, GPHI_MAX_STEPS);
+		std::vector<uint4> tr(ns);
+		if (ns) {
+			GPHI_HIP(ctx, hipMemcpyAsync(tr.data(), gr->trace, ns * sizeof(uint4), hipMemcpyDeviceToHost, s));
+			GPHI_HIP(ctx, hipStreamSynchronize(s));
+		}
+		st->steps = ns;
+		for (uint32_t k = 0; k < ns; ++k) {
+			st->dir[k] = tr[k].x;
+			st->fsize[k] = tr[k].y;
+			st->fdeg[k] = tr[k].z;
+		}
+		if (timing) {
+			st->init_ms = ev_ms(ev[0], ev[1]);
+			hipEvent_t prev = ev[1];
+			const uint32_t nt = std::min<uint32_t>(levels, GPHI_MAX_STEPS - 1);
+			for (uint32_t q = 0; q < nt; ++q) {
+				st->sweep_ms[q] = ev_ms(prev, ev[2 + 2 * q]);
+				st->update_ms[q] = ev_ms(ev[2 + 2 * q], ev[3 + 2 * q]);
+				prev = ev[3 + 2 * q];
+			}
+			st->total_ms = nt ? ev_ms(ev[0], ev[1 + 2 * nt]) : st->init_ms;
+		} else {
+			st->total_ms = total_timing ? ev_ms(ev[0], ev[kEvEnd]) : 0.f;
+		}
+	}
+	return read_totals();
+}
+
+int BatchRun::read_totals()
+{
+	gphi_ctx *ctx = gr->ctx;
+	GPHI_HIP(ctx, hipMemcpyAsync(pairs, gr->b_sums, 16, hipMemcpyDeviceToHost, ctx->stream));
+	GPHI_HIP(ctx, hipStreamSynchronize(ctx->stream));
+	return GPHI_OK;
+}
+
+// One level, then the admission of k new sources (they join the union
+// frontier of the next level), then the frontier update; reads the control
+// block and the live-source mask back.
+int BatchRun::step(const uint32_t *srcs, uint32_t k)
+{
+	gphi_ctx *ctx = gr->ctx;
+	hipStream_t s = ctx->stream;
+	StepCtrl *hc = ctx->h_ctrl;
+	if (done) return set_error(ctx, GPHI_E_STATE, "the batched traversal has ended");
+	if (used + k > cap) return set_error(ctx, GPHI_E_ARG, "no free slot for %u more sources", k);
+	const uint32_t next_level = levels + 1;
+	enqueue_level(gr, nw, next_level, true);
+	enqueue_level(gr, nw, next_level, false);
+	int rc = seed(srcs, k, next_level);
+	if (rc) return rc;
+	enqueue_frontier(gr);
+	GPHI_HIP(ctx, hipGetLastError());
+	GPHI_HIP(ctx, hipMemcpyAsync(hc, gr->ctrl, sizeof(StepCtrl), hipMemcpyDeviceToHost, s));
+	GPHI_HIP(ctx, hipMemcpyAsync(live, live_words(gr), (size_t)nw * 8, hipMemcpyDeviceToHost, s));
+	GPHI_HIP(ctx, hipStreamSynchronize(s));
+	if (hc->err) return set_error(ctx, GPHI_E_RANGE, "a batched BFS deeper than 254 levels does not fit uint8 levels");
+	levels = next_level;
+	done = hc->done != 0;
+	return GPHI_OK;
+}
+
+// Levels of the given slots (n bytes each, 255 unreached) into host rows.
+int BatchRun::copy_rows(const uint32_t *slots, uint32_t ns, uint8_t *const *rows)
+{
+	gphi_ctx *ctx = gr->ctx;
+	const DevGraph &G = gr->g;
+	hipStream_t s = ctx->stream;
+	u64 which[4] = {0, 0, 0, 0};
+	for (uint32_t i = 0; i < ns; ++i) {
+		if (slots[i] >= used) return set_error(ctx, GPHI_E_ARG, "slot %u is not in use", slots[i]);
+		which[slots[i] / 64] |= 1ull << (slots[i] % 64);
+	}
+	int rc = finalize_rows(gr, nw, which);
+	if (rc) return rc;
+	for (uint32_t i = 0; i < ns; ++i)
+		GPHI_HIP(ctx, hipMemcpyAsync(rows[i], gr->b_level + (size_t)slots[i] * G.n_pad, G.n, hipMemcpyDeviceToHost, s));
+	GPHI_HIP(ctx, hipStreamSynchronize(s));
+	return GPHI_OK;
+}
+
+int run_bfs_batch(gphi_graph *gr, const uint32_t *srcs, uint32_t k, uint8_t *levels, gphi_stats *st)
+{
+	gphi_ctx *ctx = gr->ctx;
+	const DevGraph &G = gr->g;
+	if (k == 0 || k > kBatchMaxSources)
+		return set_error(ctx, GPHI_E_ARG, "batch size %u not in [1, %u]", k, kBatchMaxSources);
+	if (!srcs) return set_error(ctx, GPHI_E_ARG, "sources are NULL");
+	if (st) memset(st, 0, sizeof(*st));
+	const bool total_timing = st != nullptr && ctx->ev_ok;
+	if (total_timing) GPHI_HIP(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
+	BatchRun run;
+	int rc = run.begin(gr, k, srcs, k);
+	if (rc) return rc;
+	if ((rc = run.run_to_end(st))) return rc;
+	if (st) {
+		st->reached = run.pairs[0];
+		st->edges_traversed = run.pairs[1];
 	}
 	if (levels) {
-		const unsigned fg = grid_n(G.n);
-		if (nw == 1)
-			batch_finalize_kernel<1><<<fg, kBlock, 0, s>>>(G.n, G.n_pad, k, gr->b_seen, gr->b_level);
-		else if (nw == 2)
-			batch_finalize_kernel<2><<<fg, kBlock, 0, s>>>(G.n, G.n_pad, k, gr->b_seen, gr->b_level);
-		else
-			batch_finalize_kernel<4><<<fg, kBlock, 0, s>>>(G.n, G.n_pad, k, gr->b_seen, gr->b_level);
-		GPHI_HIP(ctx, hipGetLastError());
+		hipStream_t s = ctx->stream;
+		u64 which[4] = {0, 0, 0, 0};
+		for (uint32_t i = 0; i < k; ++i) which[i / 64] |= 1ull << (i % 64);
+		if ((rc = finalize_rows(gr, run.nw, which))) return rc;
 		GPHI_HIP(ctx, hipMemcpy2DAsync(levels, G.n, gr->b_level, G.n_pad, G.n, k, hipMemcpyDeviceToHost, s));
 		GPHI_HIP(ctx, hipStreamSynchronize(s));
 	}

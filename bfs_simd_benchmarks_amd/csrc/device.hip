@@ -483,7 +483,7 @@ void free_traversal_state(gphi_graph *gr)
 	               (void **)&gr->trace,
 	               (void **)&gr->rank, (void **)&gr->sum, (void **)&gr->contrib, (void **)&gr->b_front,
 	               (void **)&gr->b_seen, (void **)&gr->b_next, (void **)&gr->b_level, (void **)&gr->b_sums,
-	               (void **)&gr->b_live};
+	               (void **)&gr->b_live, (void **)&gr->b_touch, (void **)&gr->b_aux};
 	for (void **p : ps) {
 		if (*p) (void)hipFree(*p);
 		*p = nullptr;
@@ -494,19 +494,14 @@ void free_traversal_state(gphi_graph *gr)
 void free_graph_device(gphi_graph *gr)
 {
 	DevGraph &G = gr->g;
-	if (gr->base) {  // a view: its own state only
-		free_traversal_state(gr);
-		void *in[] = {gr->in_off, gr->in_src, gr->pieces};
-		for (void *p : in)
-			if (p) (void)hipFree(p);
-		return;
-	}
-	void *ps[] = {G.row_off, G.cols, G.csr_w, G.deg, G.heads, G.tails, G.coo_w, G.coo_w8, G.meta, G.pack, G.heads16, G.tails16,
-	              gr->level, gr->parent, gr->dist, gr->flag, gr->visited, gr->cur,
-	              gr->row_stamp, gr->col_unvis, gr->queue, gr->qoff, gr->ctrl, gr->rank,
-	              gr->sum, gr->contrib, gr->tile_lo, gr->blk_cnt, gr->blk_base, gr->wave_cnt, gr->live, gr->live_cnt, gr->trace,
-	              gr->b_front, gr->b_seen, gr->b_next, gr->b_level, gr->b_sums, gr->b_live,
-	              gr->in_off, gr->in_src, gr->pieces};
+	free_traversal_state(gr);
+	void *in[] = {gr->in_off, gr->in_src, gr->pieces};
+	for (void *p : in)
+		if (p) (void)hipFree(p);
+	gr->in_off = gr->in_src = nullptr;
+	gr->pieces = nullptr;
+	if (gr->base) return;  // a view: the graph arrays are its base's
+	void *ps[] = {G.row_off, G.cols, G.csr_w, G.deg, G.heads, G.tails, G.coo_w, G.coo_w8, G.meta, G.pack, G.heads16, G.tails16};
 	for (void *p : ps)
 		if (p) (void)hipFree(p);
 }
@@ -591,17 +586,6 @@ __global__ __launch_bounds__(kBlock) void init_state_kernel(uint32_t n, uint32_t
 			col_unvis[i] = (i + 1 == side) ? n - i * W : W;
 		}
 		if (i < nl) live_cnt[i] = 0;
-	}
-}
-
-// col_unvis[c] = vertices of tail block c; row_stamp[r] = none
-__global__ void init_blocks_kernel(uint32_t n, uint32_t W, uint32_t side, uint32_t *row_stamp,
-                                   uint32_t *col_unvis)
-{
-	uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-	if (c < side) {
-		row_stamp[c] = UINT32_MAX;
-		col_unvis[c] = (c + 1 == side) ? n - c * W : W;
 	}
 }
 
@@ -1666,24 +1650,21 @@ static int frontier_loop(gphi_graph *gr, uint32_t src, gphi_stats *st)
 // ---- helpers shared with other translation units (batch.hip) ----
 int ensure_traversal_state(gphi_graph *gr) { return ensure_state(gr); }
 
-void launch_init_blocks(gphi_graph *gr)
-{
-	DevGraph &G = gr->g;
-	init_blocks_kernel<<<grid_for(G.side, kBlock, 65535), kBlock, 0, gr->ctx->stream>>>(
-	    G.n, G.W, G.side, gr->row_stamp, gr->col_unvis);
-}
-
-// frontier update from the byte flags (mark / scan / emit), no level writes
-void launch_frontier_update_flags(gphi_graph *gr, uint32_t next_level, uint32_t threshold)
+// Frontier update from the state bytes (mark / scan / emit) in the
+// device-driven form: nothing when the control block says the traversal is
+// done; otherwise the next level's union frontier, its direction and queue,
+// and ctrl->level advanced by one (from UINT32_MAX, the first update opens
+// level 0).  No level writes.
+void launch_frontier_update_driven(gphi_graph *gr, uint32_t threshold)
 {
 	DevGraph &G = gr->g;
 	hipStream_t s = gr->ctx->stream;
 	const unsigned nblk = G.n_pad / kPadVerts;
-	frontier_mark_kernel<false><<<nblk, kBlock, 0, s>>>(G.W, next_level, gr->flag, G.deg, gr->level,
+	frontier_mark_kernel<false><<<nblk, kBlock, 0, s>>>(G.W, kLevelFromCtrl, gr->flag, G.deg, gr->level,
 	                                                   gr->visited, gr->cur, gr->row_stamp, gr->col_unvis,
 	                                                   gr->blk_cnt, gr->wave_cnt, gr->ctrl);
 	frontier_scan_kernel<<<1, kScanThreads, 0, s>>>(nblk, threshold, gr->blk_cnt, gr->blk_base, gr->live_cnt,
-	                                              gr->ctrl, nullptr);
+	                                              gr->ctrl, gr->trace);
 	frontier_emit_kernel<<<nblk, kBlock, 0, s>>>(gr->ctrl, G.deg, gr->cur, gr->blk_cnt, gr->blk_base,
 	                                            gr->wave_cnt, gr->queue, gr->qoff, gr->tile_lo);
 }

@@ -174,6 +174,32 @@ __global__ void rank_from_sorted_kernel(uint32_t n, const uint64_t *sorted, uint
 	GRID_STRIDE(r, n) rank[(uint32_t)sorted[r]] = (uint32_t)r;
 }
 
+// One level of the BFS that orders ids for the BFS-order relabelling, over
+// the edge list (h, t): every edge out of a level-L vertex gives an unreached
+// tail level L+1 (concurrent writers store the same value).
+__global__ void bfs_order_level_kernel(uint64_t m, const uint32_t *__restrict__ h, const uint32_t *__restrict__ t,
+                                       int32_t *level, int32_t L, uint32_t *changed)
+{
+	GRID_STRIDE(e, m)
+	{
+		if (level[h[e]] == L && level[t[e]] < 0) {
+			level[t[e]] = L + 1;
+			*changed = 1;
+		}
+	}
+}
+
+// Sort key of the BFS-order relabelling: (level, old id), unreached vertices
+// after every level, in old-id order.
+__global__ void bfs_order_key_kernel(uint32_t n, const int32_t *level, uint64_t *out)
+{
+	GRID_STRIDE(v, n)
+	{
+		const uint32_t l = level[v] < 0 ? 0xFFFFFFFFu : (uint32_t)level[v];
+		out[v] = ((uint64_t)l << 32) | (uint32_t)v;
+	}
+}
+
 __global__ void weights_kernel(uint64_t m, uint64_t seed, uint32_t lo, uint64_t span, uint32_t *w)
 {
 	GRID_STRIDE(e, m)
@@ -468,6 +494,42 @@ int generate_graph(gphi_ctx *ctx, const gphi_rmat_spec *sp, uint32_t W, uint32_t
 		GPHI_HIP(ctx, hipStreamSynchronize(s));
 		sc.release(tmp);
 		sc.release(drnd);
+		sc.release(vk0);
+		sc.release(vk1);
+		sc.release(rank);
+		std::swap(h0, h1);
+		std::swap(t0, t1);
+	} else if (reorder == GPHI_REORDER_BFS) {
+		// tools/vertex_id_remap/bfs.cpp:168-247,465-491: BFS from vertex 0;
+		// new ids level by level in ascending old id, unreached vertices last
+		int32_t *lev = sc.alloc<int32_t>(n);
+		uint32_t *changed = sc.alloc<uint32_t>(1);
+		uint64_t *vk0 = sc.alloc<uint64_t>(n), *vk1 = sc.alloc<uint64_t>(n);
+		uint32_t *rank = sc.alloc<uint32_t>(n);
+		OOM_IF(lev && changed && vk0 && vk1 && rank);
+		GPHI_HIP(ctx, hipMemsetAsync(lev, 0xFF, (size_t)n * 4, s));
+		GPHI_HIP(ctx, hipMemsetAsync(lev, 0, 4, s));  // level[0] = 0
+		for (int32_t L = 0; m > 0; ++L) {
+			uint32_t hc = 0;
+			GPHI_HIP(ctx, hipMemsetAsync(changed, 0, 4, s));
+			bfs_order_level_kernel<<<grid1(m), kB, 0, s>>>(m, h0, t0, lev, L, changed);
+			GPHI_HIP(ctx, hipMemcpyAsync(&hc, changed, 4, hipMemcpyDeviceToHost, s));
+			GPHI_HIP(ctx, hipStreamSynchronize(s));
+			if (!hc) break;
+			if ((uint32_t)L >= n) return set_error(ctx, GPHI_E_STATE, "BFS-order relabelling did not terminate");
+		}
+		bfs_order_key_kernel<<<grid1(n), kB, 0, s>>>(n, lev, vk0);
+		size_t tb = 0;
+		GPHI_HIP(ctx, hipcub::DeviceRadixSort::SortKeys(nullptr, tb, vk0, vk1, n, 0, 64, s));
+		void *tmp = sc.alloc<uint8_t>(tb);
+		OOM_IF(tmp);
+		GPHI_HIP(ctx, hipcub::DeviceRadixSort::SortKeys(tmp, tb, vk0, vk1, n, 0, 64, s));
+		rank_from_sorted_kernel<<<grid1(n), kB, 0, s>>>(n, vk1, rank);
+		relabel_kernel<<<grid1(m), kB, 0, s>>>(m, rank, h0, t0, h1, t1);
+		GPHI_HIP(ctx, hipStreamSynchronize(s));
+		sc.release(tmp);
+		sc.release(lev);
+		sc.release(changed);
 		sc.release(vk0);
 		sc.release(vk1);
 		sc.release(rank);

@@ -63,7 +63,7 @@ struct StepCtrl {
 	uint32_t done;          // frontier empty
 	uint32_t blocks_done;   // last-block detection
 	uint32_t facc;          // accumulator: |F|
-	uint32_t pad0;
+	uint32_t err;           // multi-source BFS: a level would not fit its uint8 levels
 	unsigned long long qpack;  // accumulator: (queue count << 32) | degree sum
 	uint32_t changed;       // hop relaxation: edges that lowered a distance
 	uint32_t chunks;        // chunks on the live lists of the last dense sweep
@@ -145,7 +145,9 @@ struct gphi_graph {
 	unsigned long long *b_front = nullptr, *b_seen = nullptr, *b_next = nullptr;
 	uint8_t *b_level = nullptr;
 	unsigned long long *b_sums = nullptr;
-	unsigned long long *b_live = nullptr;  // 2 x words: sources with a non-empty frontier
+	unsigned long long *b_live = nullptr;  // sources with a non-empty frontier, then the per-block slots
+	uint32_t *b_touch = nullptr;           // bitmap: vertices whose next mask got a bit this step
+	uint32_t *b_aux = nullptr;             // sources to seed, their slots, per-slot start levels
 	// in-edge arrays of the pull steps (built once, independent of W and S)
 	uint32_t *in_off = nullptr, *in_src = nullptr;
 	uint2 *pieces = nullptr;
@@ -184,9 +186,38 @@ int ensure_traversal_state(gphi_graph *gr);
 // the BFS level loop of run_bfs on gr's state buffers (set up by the caller)
 int bfs_level_loop(gphi_graph *gr, uint32_t src, gphi_stats *st);
 uint32_t live_list_capacity(uint64_t nchunks);  // entries of one XCD's live-chunk list
-void launch_init_blocks(gphi_graph *gr);
-void launch_frontier_update_flags(gphi_graph *gr, uint32_t next_level, uint32_t threshold);
+void launch_frontier_update_driven(gphi_graph *gr, uint32_t threshold);
 int run_bfs_batch(gphi_graph *gr, const uint32_t *srcs, uint32_t k, uint8_t *levels, gphi_stats *st);
+
+// A multi-source traversal (batch.hip).  gphi_bfs_batch runs one to its end
+// on the device; the serving loop advances one level by level, admitting new
+// sources into free slots between levels and reading out the rows of the
+// slots whose frontier has emptied.  Slot i is bit i of the masks.
+constexpr uint32_t kBatchMaxSources = 256;
+struct BatchRun {
+	gphi_graph *gr = nullptr;
+	uint32_t nw = 0;       // 64-bit mask words per vertex
+	uint32_t cap = 0;      // slots (64 * nw)
+	uint32_t used = 0;     // slots handed out (slots are not reused within a run)
+	uint32_t levels = 0;   // levels expanded
+	bool done = false;     // the union frontier is empty
+	std::vector<uint32_t> slot_src, slot_start;  // per slot: source vertex, level it joined at
+	unsigned long long live[4] = {0, 0, 0, 0};   // sources with a non-empty frontier (after step)
+	unsigned long long pairs[2] = {0, 0};        // reached (source, vertex) pairs, their out-degrees
+	uint32_t aux[3 * kBatchMaxSources] = {};     // staging of the sources to seed
+
+	// clears the state and seeds k sources (slots 0..k-1); capacity picks the
+	// mask width (1..256 sources)
+	int begin(gphi_graph *g, uint32_t capacity, const uint32_t *srcs, uint32_t k);
+	// one level, then k new sources join (slots used..used+k-1); reads back
+	// the control block and the live mask
+	int step(const uint32_t *srcs, uint32_t k);
+	int run_to_end(gphi_stats *st);  // the remaining levels, on the device
+	int read_totals();               // pairs[] from the device
+	// levels of the given slots into host rows of n bytes (255 unreached)
+	int copy_rows(const uint32_t *slots, uint32_t ns, uint8_t *const *rows);
+	int seed(const uint32_t *srcs, uint32_t k, uint32_t start_level);
+};
 
 // generator + tool pipeline on the device (gen.hip)
 int generate_graph(gphi_ctx *ctx, const gphi_rmat_spec *sp, uint32_t W, uint32_t S, int reorder,

@@ -5,12 +5,15 @@
 // own gphi_ctx (HIP stream) and worker thread, so the lanes' kernels overlap
 // on the device.  A lane's worker takes every BFS query pending for its graph
 // when it becomes free (up to max_batch) and answers them with one
-// multi-source traversal (gphi_bfs_batch) when that is expected to beat
-// running them one by one; queries that arrive meanwhile join the next
-// launch.  A weighted graph also gets an SSSP lane -- a view of the graph
-// (gphi_graph_view) on a stream of its own -- that runs SSSP queries one at a
-// time beside the BFS lane.  Graphs are admitted while their HBM bytes plus
-// traversal state fit the budget.
+// multi-source traversal when that is expected to beat running them one by
+// one.  The traversal advances level by level; between two levels the worker
+// admits the BFS queries that arrived meanwhile into free source slots (they
+// join the running traversal, each from its own level 0), and every query
+// completes as soon as its own frontier is empty: its levels are copied out
+// while the traversal goes on for the others.  A weighted graph also gets SSSP
+// lanes -- views of the graph (gphi_graph_view) on streams of their own --
+// that run SSSP queries one at a time beside the BFS lane.  Graphs are
+// admitted while their HBM bytes plus traversal state fit the budget.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -41,7 +44,8 @@ struct Query {
 	uint32_t src;
 	void *out;      // caller buffer (int32 levels / uint32 distances), may be null
 	double t_submit = 0, t_start = 0, t_done = 0;
-	uint32_t batch = 0;
+	uint32_t batch = 0, joined_level = 0;
+	uint64_t launch = 0;
 	int status = GPHI_OK;
 	bool done = false;
 };
@@ -54,21 +58,25 @@ struct Lane {
 	std::thread worker;
 	uint64_t device_bytes = 0;
 	uint64_t batches = 0, served = 0;
-	// running estimates (microseconds) of one plain BFS and of one batched
-	// launch per mask width (1, 2, 4 words); 0 = not measured yet
-	double t_plain = 0, t_batch[3] = {0, 0, 0};
+	// running estimates (microseconds) of one plain BFS query and of a BFS
+	// query's share of a multi-source traversal, per mask width (1, 2, 4
+	// words); 0 = not measured yet
+	double t_plain = 0, t_share[3] = {0, 0, 0};
 	bool busy = false;  // a launch is running (guarded by the server mutex)
 };
 
-// the multi-source traversal has a fixed cost (mask and level set-up, a
-// vertex pass per level) that small batches do not repay: run k pending BFS
-// queries as one launch only when that is expected to beat k plain runs
+int width_index(size_t slots) { return slots <= 64 ? 0 : slots <= 128 ? 1 : 2; }
+
+// The multi-source traversal has a fixed cost per level (the update and
+// frontier passes, a host round trip) that small groups do not repay: start
+// one for k pending BFS queries only when a query's expected share of it
+// beats a plain run.
 bool use_batch(const Lane &L, size_t k, uint32_t max_batch)
 {
 	if (max_batch <= 1 || k <= 1) return false;
-	const int w = k <= 64 ? 0 : k <= 128 ? 1 : 2;
-	if (L.t_plain > 0 && L.t_batch[w] > 0) return L.t_batch[w] < (double)k * L.t_plain;
-	return k >= 8;  // no estimate yet: batch only clearly large groups
+	const int w = width_index(k);
+	if (L.t_plain > 0 && L.t_share[w] > 0) return L.t_share[w] < L.t_plain;
+	return k >= 4;  // no estimate yet: batch only clearly large groups
 }
 
 void ema(double &x, double v) { x = x > 0 ? 0.8 * x + 0.2 * v : v; }
@@ -83,6 +91,7 @@ struct gphi_server {
 	std::vector<std::pair<Lane *, std::vector<Lane *>>> graphs;  // per graph id: BFS lane, SSSP lanes
 	std::unordered_map<uint64_t, std::shared_ptr<Query>> queries;
 	uint64_t next_ticket = 1;
+	uint64_t next_launch = 1;
 	uint64_t hbm_used = 0;
 	bool stopping = false;
 };
@@ -91,19 +100,122 @@ namespace gphi {
 namespace {
 
 // state a lane needs besides the graph: single-source arrays (~42 B per
-// vertex), batched masks and levels for max_batch sources, and the in-edge
-// arrays of the batched pull steps (4 B per edge and per vertex, pieces)
+// vertex), batched masks, levels and bitmaps for max_batch sources, and the
+// in-edge arrays of the batched pull steps (4 B per edge and per vertex, pieces)
 uint64_t lane_state_bytes(const DevGraph &G, uint32_t max_batch)
 {
 	const uint64_t nw = max_batch <= 64 ? 1 : max_batch <= 128 ? 2 : 4;
 	const uint64_t in_edges = max_batch > 1 ? 4 * G.m + 12 * (uint64_t)G.n_pad + G.m / 256 : 0;
-	return (uint64_t)G.n_pad * (42 + 3 * 8 * nw + 64 * nw) + G.m / 256 + in_edges;
+	return (uint64_t)G.n_pad * (42 + 3 * 8 * nw + 64 * nw + 1) + G.m / 256 + in_edges;
+}
+
+// BFS levels of a finished slot into the query's int32 buffer (-1 unreached)
+void store_levels(const std::vector<uint8_t> &row, uint32_t n, Query &q)
+{
+	if (!q.out) return;
+	int32_t *o = static_cast<int32_t *>(q.out);
+	for (uint32_t v = 0; v < n; ++v) o[v] = row[v] == 255 ? -1 : (int32_t)row[v];
+}
+
+void complete(gphi_server *S, const std::vector<std::shared_ptr<Query>> &qs, int rc, double t)
+{
+	{
+		std::lock_guard<std::mutex> lk(S->mu);
+		for (auto &q : qs) {
+			q->t_done = t;
+			q->status = rc;
+			q->done = true;
+		}
+	}
+	S->cv_done.notify_all();
+}
+
+// A multi-source traversal with admission between levels.  `take` holds the
+// queries it starts with; arrivals join while slots are free.
+void run_continuous(gphi_server *S, Lane *L, std::vector<std::shared_ptr<Query>> take)
+{
+	const uint32_t cap = std::min<uint32_t>(S->opts.max_batch, std::max<uint32_t>((uint32_t)take.size(), 64u));
+	BatchRun run;
+	std::vector<std::shared_ptr<Query>> slot_q;  // per slot
+	std::vector<uint8_t> finished;               // per slot: answered
+	std::vector<uint32_t> srcs;
+	std::vector<uint8_t> row(L->n);
+	uint64_t launch;
+	{
+		std::lock_guard<std::mutex> lk(S->mu);
+		launch = S->next_launch++;
+	}
+	const double t0 = now_us();
+	for (auto &q : take) srcs.push_back(q->src);
+	int rc = run.begin(L->g, cap, srcs.data(), (uint32_t)srcs.size());
+	for (auto &q : take) {
+		q->t_start = t0;
+		q->launch = launch;
+		slot_q.push_back(q);
+		finished.push_back(0);
+	}
+	while (rc == GPHI_OK && !run.done) {
+		// arrivals join at this level boundary, up to the free slots
+		std::vector<std::shared_ptr<Query>> join;
+		{
+			std::lock_guard<std::mutex> lk(S->mu);
+			const uint32_t free_slots = run.cap - run.used;
+			for (auto it = L->pending.begin(); it != L->pending.end() && join.size() < free_slots;) {
+				if ((*it)->kind == GPHI_QUERY_BFS) {
+					join.push_back(*it);
+					it = L->pending.erase(it);
+				} else {
+					++it;
+				}
+			}
+		}
+		srcs.clear();
+		const double tj = now_us();
+		for (auto &q : join) {
+			srcs.push_back(q->src);
+			q->t_start = tj;
+			q->launch = launch;
+			q->joined_level = run.levels + 1;
+			slot_q.push_back(q);
+			finished.push_back(0);
+		}
+		rc = run.step(srcs.data(), (uint32_t)srcs.size());
+		if (rc) break;
+		// slots whose frontier emptied: copy their levels out and answer them
+		for (uint32_t s = 0; s < run.used && rc == GPHI_OK; ++s) {
+			if (finished[s] || ((run.live[s / 64] >> (s % 64)) & 1ull)) continue;
+			if (slot_q[s]->out) {
+				uint8_t *r = row.data();
+				rc = run.copy_rows(&s, 1, &r);
+				if (rc == GPHI_OK) store_levels(row, L->n, *slot_q[s]);
+			}
+			finished[s] = 1;
+			slot_q[s]->batch = run.used;
+			if (rc == GPHI_OK) complete(S, {slot_q[s]}, GPHI_OK, now_us());
+		}
+	}
+	// the traversal ended (or failed): answer whatever is left
+	std::vector<std::shared_ptr<Query>> rest;
+	for (uint32_t s = 0; s < slot_q.size(); ++s) {
+		if (finished[s]) continue;
+		if (rc == GPHI_OK && slot_q[s]->out) {
+			uint8_t *r = row.data();
+			rc = run.copy_rows(&s, 1, &r);
+			if (rc == GPHI_OK) store_levels(row, L->n, *slot_q[s]);
+		}
+		slot_q[s]->batch = (uint32_t)slot_q.size();
+		rest.push_back(slot_q[s]);
+	}
+	const double t1 = now_us();
+	if (!rest.empty()) complete(S, rest, rc, t1);
+	if (rc == GPHI_OK && !slot_q.empty()) ema(L->t_share[width_index(run.cap)], (t1 - t0) / (double)slot_q.size());
+	std::lock_guard<std::mutex> lk(S->mu);
+	L->batches += 1;
+	L->served += slot_q.size();
 }
 
 void lane_loop(gphi_server *S, Lane *L)
 {
-	std::vector<uint32_t> srcs;
-	std::vector<uint8_t> lv8;
 	std::vector<std::shared_ptr<Query>> take;
 	for (;;) {
 		take.clear();
@@ -127,57 +239,51 @@ void lane_loop(gphi_server *S, Lane *L)
 			}
 			L->busy = true;
 		}
-		const double t0 = now_us();
-		int rc = GPHI_OK;
-		std::vector<double> done_at;  // per-query completion times of a run of plain BFS
 		if (take[0]->kind == GPHI_QUERY_SSSP) {
-			rc = gphi_sssp(L->g, take[0]->src, (uint32_t *)take[0]->out, nullptr);
-		} else if (!use_batch(*L, take.size(), S->opts.max_batch)) {
-			for (auto &q : take) {
-				const double a = now_us();
-				const int r = gphi_bfs(L->g, q->src, (int32_t *)q->out, nullptr, nullptr);
-				const double b = now_us();
-				if (!q->out) ema(L->t_plain, b - a);
-				done_at.push_back(b);
-				if (r != GPHI_OK && rc == GPHI_OK) rc = r;
+			Query &q = *take[0];
+			uint64_t launch;
+			{
+				std::lock_guard<std::mutex> lk(S->mu);
+				launch = S->next_launch++;
+			}
+			q.t_start = now_us();
+			q.launch = launch;
+			q.batch = 1;
+			const int rc = gphi_sssp(L->g, q.src, (uint32_t *)q.out, nullptr);
+			complete(S, take, rc, now_us());
+			std::lock_guard<std::mutex> lk(S->mu);
+			L->batches += 1;
+			L->served += 1;
+		} else if (use_batch(*L, take.size(), S->opts.max_batch)) {
+			std::lock_guard<std::mutex> ctx_lk(L->ctx->mu);  // the traversal's kernels on this context
+			if (hipSetDevice(L->ctx->device) != hipSuccess) {
+				complete(S, take, GPHI_E_HIP, now_us());
+			} else {
+				run_continuous(S, L, take);
 			}
 		} else {
-			srcs.resize(take.size());
-			bool want = false;
-			for (size_t i = 0; i < take.size(); ++i) {
-				srcs[i] = take[i]->src;
-				want |= take[i]->out != nullptr;
-			}
-			if (want) lv8.resize((size_t)take.size() * L->n);
-			const double a = now_us();
-			rc = gphi_bfs_batch(L->g, srcs.data(), (uint32_t)take.size(), want ? lv8.data() : nullptr, nullptr);
-			if (!want) ema(L->t_batch[take.size() <= 64 ? 0 : take.size() <= 128 ? 1 : 2], now_us() - a);
-			if (rc == GPHI_OK && want) {
-				for (size_t i = 0; i < take.size(); ++i) {
-					if (!take[i]->out) continue;
-					int32_t *o = (int32_t *)take[i]->out;
-					const uint8_t *r = lv8.data() + i * L->n;
-					for (uint32_t v = 0; v < L->n; ++v) o[v] = r[v] == 255 ? -1 : (int32_t)r[v];
+			for (auto &q : take) {
+				uint64_t launch;
+				{
+					std::lock_guard<std::mutex> lk(S->mu);
+					launch = S->next_launch++;
 				}
+				q->t_start = now_us();
+				q->launch = launch;
+				q->batch = 1;
+				const int r = gphi_bfs(L->g, q->src, (int32_t *)q->out, nullptr, nullptr);
+				const double t = now_us();
+				ema(L->t_plain, t - q->t_start);
+				complete(S, {q}, r, t);
+				std::lock_guard<std::mutex> lk(S->mu);
+				L->batches += 1;
+				L->served += 1;
 			}
 		}
-		const double t1 = now_us();
 		{
 			std::lock_guard<std::mutex> lk(S->mu);
-			const bool batched = done_at.empty();
-			for (size_t i = 0; i < take.size(); ++i) {
-				Query &q = *take[i];
-				q.t_start = batched || i == 0 ? t0 : done_at[i - 1];
-				q.t_done = batched ? t1 : done_at[i];
-				q.batch = batched ? (uint32_t)take.size() : 1;
-				q.status = rc;
-				q.done = true;
-			}
-			L->batches += done_at.empty() ? 1 : done_at.size();
-			L->served += take.size();
 			L->busy = false;
 		}
-		S->cv_done.notify_all();
 	}
 }
 
@@ -329,6 +435,8 @@ int gphi_server_wait(gphi_server *S, uint64_t ticket, gphi_query_info *info)
 		info->queue_us = q->t_start - q->t_submit;
 		info->run_us = q->t_done - q->t_start;
 		info->batch = q->batch;
+		info->joined_level = q->joined_level;
+		info->launch = q->launch;
 	}
 	return q->status;
 }

@@ -29,11 +29,13 @@ class Workload:
     weight_seed: int = 0
 
     def generate(self, ctx, W: int | None = None, S: int | None = None, weighted: bool = False,
-                 tile_order: int = 0):
+                 tile_order: int = 0, reorder: int = 1):
+        """reorder: 1 degree order (the workloads' own), 2 BFS order, 0 none."""
         return ctx.generate(scale=self.scale, nverts=self.nverts, seed=self.seed,
                             samples=self.samples, target_unique=self.target_unique,
                             perm_seed=self.perm_seed, W=W or self.W, S=S or self.S,
-                            weight_seed=self.weight_seed if weighted else 0, tile_order=tile_order)
+                            reorder=reorder, weight_seed=self.weight_seed if weighted else 0,
+                            tile_order=tile_order)
 
 
 WORKLOADS = {

@@ -146,9 +146,12 @@ int gphi_bfs_batch(gphi_graph *g, const uint32_t *srcs, uint32_t k, uint8_t *lev
  * Each added graph gets a BFS lane with the graph's own context (HIP stream)
  * and a worker thread; lanes run concurrently on the device.  A BFS lane
  * takes all BFS queries pending for its graph when it becomes free (up to
- * max_batch) and answers them with one gphi_bfs_batch launch when that is
- * expected to beat running them one by one (running estimates of both);
- * queries arriving meanwhile join the next launch.  A weighted graph also
+ * max_batch) and answers them with one multi-source traversal when that is
+ * expected to beat running them one by one (running estimates of both).
+ * That traversal runs level by level: BFS queries that arrive while it runs
+ * join it at the next level boundary, each in a free source slot of its own
+ * (up to the launch's mask width: 64, 128 or 256 sources), and every query
+ * completes as soon as its own frontier is empty.  A weighted graph also
  * gets sssp_lanes SSSP lanes, each a gphi_graph_view on a context of its own,
  * running SSSP queries one at a time beside the BFS lane.  A graph is
  * admitted only while its HBM bytes plus every lane's traversal state fit
@@ -163,10 +166,12 @@ typedef struct gphi_server_opts {
 #define GPHI_QUERY_BFS 0
 #define GPHI_QUERY_SSSP 1
 typedef struct gphi_query_info {
-	double latency_us;  /* submit -> result */
-	double queue_us;    /* submit -> start of its launch */
-	double run_us;      /* its launch */
-	uint32_t batch;     /* queries answered by that launch */
+	double latency_us;      /* submit -> result */
+	double queue_us;        /* submit -> start of its work (launch start, or the level it joined at) */
+	double run_us;          /* start of its work -> result */
+	uint32_t batch;         /* queries answered by that launch (so far, when it completed) */
+	uint32_t joined_level;  /* level of the running traversal it joined at; 0 = started with it */
+	uint64_t launch;        /* launch that answered it (per server, counting from 1) */
 } gphi_query_info;
 int gphi_server_create(const gphi_server_opts *opts, gphi_server **out);
 int gphi_server_add_graph(gphi_server *s, gphi_graph *g, uint32_t *graph_id);
@@ -204,6 +209,8 @@ typedef struct gphi_weight_spec {
 
 #define GPHI_REORDER_NONE 0
 #define GPHI_REORDER_DEGREE 1 /* tools/vertex_id_reorder_to_degrees */
+#define GPHI_REORDER_BFS 2    /* tools/vertex_id_remap/bfs.cpp:168-247,465-561: BFS from vertex 0,
+                                 ids level by level in ascending old id, unreached last */
 
 int gphi_graph_generate(gphi_ctx *ctx, const gphi_rmat_spec *spec, uint32_t W, uint32_t S,
                         int reorder, const gphi_weight_spec *weights, int tile_order,

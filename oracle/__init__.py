@@ -47,6 +47,8 @@ def lib():
             "orc_bfs_levels": (None, [u32, P, P, u32, P]),
             "orc_bfs_hybrid": (i32, [u32, u64, u32, u32, P, P, P, P, P, u32, u32, i32, P, P, P, u32]),
             "orc_check_parents": (C.c_int64, [u32, P, P, u32, P, P]),
+            "orc_simd_width": (i32, []),
+            "orc_set_simd": (None, [i32]),
             "orc_sssp_dijkstra": (None, [u32, P, P, P, u32, P]),
             "orc_pagerank_f64": (None, [u32, u64, P, P, P, u32, f64, f64, P]),
             "orc_bellman_ford_hops": (None, [u32, u64, P, P, u32, u32, P]),

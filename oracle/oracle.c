@@ -8,6 +8,7 @@
 #define _GNU_SOURCE
 #include "oracle.h"
 
+#include <immintrin.h>
 #include <limits.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -469,10 +470,52 @@ static void hyb_dense_edges(hyb_t *g, uint64_t b, uint64_t e)
 	}
 }
 
+/* bfs_kernel_dense with AVX-512, bfs.cpp:181-245: 16 edges per vector; the
+ * frontier flags of the heads, then the parents of the active edges' tails,
+ * are gathered; the unvisited lanes scatter cost, the updating flag and the
+ * parent.  Within a vector, lanes with the same tail write the same cost and
+ * one of their heads as parent (the scatter order picks it, as in the
+ * reference).  The updating flag of the tail's block, which the reference
+ * computes with SVML's _mm512_div_epi32, is set lane by lane from the mask. */
+__attribute__((target("avx512f"))) static void hyb_dense_edges_avx512(hyb_t *g, uint64_t b, uint64_t e)
+{
+	const __m512i minus1 = _mm512_set1_epi32(-1), one = _mm512_set1_epi32(1);
+	for (uint64_t k = b; k < e; k += 16) {
+		const uint64_t left = e - k;
+		const __mmask16 in = left >= 16 ? (__mmask16)0xFFFF : (__mmask16)((1u << left) - 1u);
+		const __m512i head = _mm512_maskz_loadu_epi32(in, g->heads + k);
+		const __m512i act = _mm512_mask_i32gather_epi32(_mm512_setzero_si512(), in, head, g->mask, 4);
+		const __mmask16 am = _mm512_mask_test_epi32_mask(in, act, minus1);
+		if (!am) continue;
+		const __m512i tail = _mm512_maskz_loadu_epi32(am, g->tails + k);
+		const __m512i par = _mm512_mask_i32gather_epi32(_mm512_setzero_si512(), am, tail, g->parents, 4);
+		const __mmask16 nv = _mm512_mask_cmpeq_epi32_mask(am, par, minus1);
+		if (!nv) continue;
+		const __m512i ch = _mm512_mask_i32gather_epi32(_mm512_setzero_si512(), nv, head, g->cost, 4);
+		_mm512_mask_i32scatter_epi32(g->cost, nv, tail, _mm512_add_epi32(ch, one), 4);
+		_mm512_mask_i32scatter_epi32(g->updating, nv, tail, one, 4);
+		_mm512_mask_i32scatter_epi32(g->parents, nv, tail, head, 4);
+		uint32_t t[16];
+		_mm512_storeu_si512(t, tail);
+		for (unsigned m = nv; m; m &= m - 1) g->upd_side[t[__builtin_ctz(m)] / g->W] = 1;
+	}
+}
+
+/* 512: the dense sweep runs hyb_dense_edges_avx512 (the host has AVX-512F
+ * and it is not switched off); 0: the one-lane-at-a-time hyb_dense_edges */
+static int g_simd = -1;
+int orc_simd_width(void)
+{
+	if (g_simd < 0) g_simd = __builtin_cpu_supports("avx512f") ? 512 : 0;
+	return g_simd;
+}
+void orc_set_simd(int on) { g_simd = on && __builtin_cpu_supports("avx512f") ? 512 : 0; }
+
 /* scheduler_dense / BFS_dense, bfs.cpp:246-388: stripes in sequence, columns of
  * a stripe in parallel (dynamic,1), each column owning its tail block. */
 static void hyb_dense(hyb_t *g)
 {
+	const int simd = orc_simd_width() == 512;
 	uint64_t side = g->side;
 	uint64_t rem = side % g->S, bound = side - rem;
 	for (uint64_t sr = 0; sr < side;) {
@@ -485,7 +528,10 @@ static void hyb_dense(hyb_t *g)
 				uint64_t b = g->toff[tile];
 				uint64_t e = tile != g->ntiles - 1 ? g->toff[tile + 1] : g->m;
 				if (b == e || !g->active_side[row]) continue;
-				hyb_dense_edges(g, b, e);
+				if (simd)
+					hyb_dense_edges_avx512(g, b, e);
+				else
+					hyb_dense_edges(g, b, e);
 			}
 		}
 		sr += step;
@@ -661,30 +707,35 @@ int orc_bfs_hybrid(uint32_t n, uint64_t m, uint32_t W, uint32_t S, const uint32_
 
 /* Parent-tree invariant: parent[src]=src; reached v != src has an edge
  * (parent[v], v) with level[parent[v]] == level[v]-1; unreached has -1.
- * Returns 0 when valid, else 1 + the first offending vertex. */
+ * Returns 0 when valid, else 1 + the first offending vertex.  The edge test
+ * is one pass over the CSR (every edge (u, w) with parent[w] == u marks w),
+ * so hubs with many children cost their degree once, not once per child. */
 int64_t orc_check_parents(uint32_t n, const uint64_t *row_off, const uint32_t *cols,
                           uint32_t src, const int32_t *level, const int32_t *parent)
 {
-	for (uint32_t v = 0; v < n; ++v) {
+	uint8_t *has_edge = (uint8_t *)calloc(n ? n : 1, 1);
+	if (!has_edge) return -1;
+#pragma omp parallel for schedule(dynamic, 1024)
+	for (uint32_t u = 0; u < n; ++u)
+		for (uint64_t k = row_off[u]; k < row_off[u + 1]; ++k) {
+			const uint32_t w = cols[k];
+			if (parent[w] == (int32_t)u) has_edge[w] = 1;
+		}
+	int64_t bad = 0;
+	for (uint32_t v = 0; v < n && !bad; ++v) {
 		if (v == src) {
-			if (parent[v] != (int32_t)src || level[v] != 0) return (int64_t)v + 1;
+			if (parent[v] != (int32_t)src || level[v] != 0) bad = (int64_t)v + 1;
 			continue;
 		}
 		if (level[v] < 0) {
-			if (parent[v] != -1) return (int64_t)v + 1;
+			if (parent[v] != -1) bad = (int64_t)v + 1;
 			continue;
 		}
 		int32_t p = parent[v];
-		if (p < 0 || (uint32_t)p >= n || level[p] != level[v] - 1) return (int64_t)v + 1;
-		int found = 0;
-		for (uint64_t k = row_off[p]; k < row_off[p + 1]; ++k)
-			if (cols[k] == v) {
-				found = 1;
-				break;
-			}
-		if (!found) return (int64_t)v + 1;
+		if (p < 0 || (uint32_t)p >= n || level[p] != level[v] - 1 || !has_edge[v]) bad = (int64_t)v + 1;
 	}
-	return 0;
+	free(has_edge);
+	return bad;
 }
 
 /* Dijkstra with a binary heap; unreached = INT_MAX as the reference

@@ -83,6 +83,10 @@ int orc_bfs_hybrid(uint32_t n, uint64_t m, uint32_t W, uint32_t S, const uint32_
                    const uint32_t *cols, const uint32_t *heads, const uint32_t *tails,
                    const uint32_t *tile_offsets, uint32_t src, uint32_t t_ratio, int nthreads,
                    int32_t *level, int32_t *parent, uint32_t *trace, uint32_t trace_cap);
+/* vector width of orc_bfs_hybrid's dense sweep: 512 (AVX-512 restatement of
+ * bfs.cpp:181-245) or 0 (one lane at a time); orc_set_simd(0) forces the latter */
+int orc_simd_width(void);
+void orc_set_simd(int on);
 int64_t orc_check_parents(uint32_t n, const uint64_t *row_off, const uint32_t *cols,
                           uint32_t src, const int32_t *level, const int32_t *parent);
 void orc_sssp_dijkstra(uint32_t n, const uint64_t *row_off, const uint32_t *cols,

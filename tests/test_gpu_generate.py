@@ -58,6 +58,34 @@ def test_generate_c1_other_tilings(ctx, W, S, order):
     _cmp(d["tails"], ref.rt if order else ref.st, "tile tails")
 
 
+@pytest.mark.parametrize("name,weighted", [("c1", False), ("c3", True)])
+def test_generate_bfs_order_matches_tool_pipeline(ctx, name, weighted):
+    """GPHI_REORDER_BFS: the relabelling of tools/vertex_id_remap, whose oracle
+    restatement reproduces that tool's output file byte for byte."""
+    print(f"case: generate {name} BFS order", flush=True)
+    import bfs_simd_benchmarks_amd as G
+    n, h, t = O.config_edges("c1")
+    ref = O.build_from_edgelist(n, h, t, 1024, 16, reorder="bfs")
+    if weighted:
+        ref = O.tile_edgelist(n, ref.eh, ref.et, 1024, 16, O.weights(ref.m, 3))
+    g = WORKLOADS[name].generate(ctx, weighted=weighted, reorder=G.REORDER_BFS)
+    try:
+        assert (g.n, g.m) == (ref.n, ref.m)
+        d = g.download()
+        lv, _, _ = g.bfs(0)
+    finally:
+        g.free()
+    _cmp(d["deg"], ref.deg, "deg")
+    _cmp(d["cols"], ref.cols, "csr")
+    _cmp(d["heads"], ref.sh, "tile heads")
+    _cmp(d["tails"], ref.st, "tile tails")
+    if weighted:
+        _cmp(d["cw"], ref.cw, "csr weights")
+    # BFS order: new ids rise with the BFS level from vertex 0
+    reached = lv[lv >= 0]
+    assert np.all(np.diff(reached) >= 0) and np.all(lv[: reached.size] >= 0)
+
+
 def test_generate_c2_pokec_shape(ctx):
     print("case: generate c2 (first 30,622,564 unique samples, permuted ids)", flush=True)
     ref = O.config_graph("c2")

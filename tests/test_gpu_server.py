@@ -2,6 +2,7 @@
 together are answered with batched launches that overlap across graphs, and
 each result equals the query's result alone (oracle / single-source run)."""
 import threading
+import time
 
 import numpy as np
 import pytest
@@ -118,3 +119,48 @@ def test_view_runs_beside_its_graph(c3):
         g.free()
         other.close()
         base_ctx.close()
+
+
+def test_query_joins_running_traversal(c1):
+    """Continuous batching: BFS queries submitted while a multi-source
+    traversal runs join it at a level boundary (same launch, joined_level > 0)
+    and still get exactly their own BFS levels.
+
+    Graph: the rmat16 graph plus a chain of 200 vertices hanging off vertex 0
+    (so a traversal that reaches vertex 0 lasts 200 more levels) and a
+    separate chain of 20,000 vertices whose first vertex, as a lone plain BFS,
+    keeps the lane busy while the first wave of queries queues up."""
+    base, chain_b, chain_a = c1.n, 200, 20_000
+    n = base + chain_b + chain_a
+    h = [c1.csr_heads, np.array([0], np.uint32), np.arange(base, base + chain_b - 1, dtype=np.uint32),
+         np.arange(base + chain_b, n - 1, dtype=np.uint32)]
+    t = [c1.cols, np.array([base], np.uint32), np.arange(base + 1, base + chain_b, dtype=np.uint32),
+         np.arange(base + chain_b + 1, n, dtype=np.uint32)]
+    g = O.tile_edgelist(n, np.concatenate(h), np.concatenate(t), 1024, 16)
+    wave = [int(s) for s in pick_sources(c1.deg, 16, 77)]
+    srv = G.Server(max_batch=64)
+    try:
+        gid = srv.add_graph(lane(g))
+        outs = {}
+
+        def submit(src):
+            outs[src] = np.empty(n, np.int32)
+            return srv.submit(gid, G.QUERY_BFS, src, outs[src])
+        print("case: server join, blocker then two waves", flush=True)
+        blocker = submit(base + chain_b)
+        time.sleep(0.05)
+        first = [submit(s) for s in wave[:8]]
+        srv.wait(blocker)
+        second = [submit(s) for s in wave[8:]]
+        info1 = [srv.wait(x) for x in first]
+        info2 = [srv.wait(x) for x in second]
+    finally:
+        srv.close()
+    for s in [base + chain_b] + wave:
+        assert np.array_equal(outs[s], O.bfs_levels(n, g.deg, g.cols, s)), s
+    launch = info1[0].launch
+    assert all(i.launch == launch for i in info1), [i.launch for i in info1]
+    joined = [i for i in info2 if i.launch == launch and i.joined_level > 0]
+    print(f"joined the running launch: {len(joined)} of {len(info2)}, at levels "
+          f"{sorted(i.joined_level for i in joined)}", flush=True)
+    assert joined, [(i.launch, i.joined_level) for i in info2]

@@ -186,3 +186,56 @@ def test_bfs_level_timing_off(ctx, c1):
         assert st_off.total_ms > 0 and any(st_on.sweep_times())
     finally:
         gg.free()
+
+
+def brute_force_step_bytes(g, lv, trace):
+    """The byte model of each expanded frontier, from the levels alone:
+    sparse step  4|F| + 8|F| + 4 D_F + 4 D_F + 12 |F'|
+    dense step   8 E_act + 4 T_act + 3 ceil(N/8) + 8 |F'|
+    plus ceil(N/8) on a switch to dense, ceil(N/8) + 4|F| on a switch to sparse.
+    E_act / T_act: edges / count of the non-empty tiles whose head row holds a
+    frontier vertex and whose tail column still has an unvisited vertex when
+    the step begins -- counted here tile by tile over every edge."""
+    W, n = g.W, g.n
+    side = -(-n // W)
+    heads = g.csr_heads.astype(np.int64)
+    tails = g.cols.astype(np.int64)
+    tile = (heads // W) * side + tails // W
+    nbm = (n + 7) // 8
+    out = []
+    for k, (dense, f, dF) in enumerate(trace):
+        fn = trace[k + 1][1] if k + 1 < len(trace) else 0
+        if dense:
+            row_hot = np.zeros(side, bool)
+            row_hot[np.flatnonzero(lv == k) // W] = True
+            col_open = np.zeros(side, bool)
+            col_open[np.flatnonzero((lv > k) | (lv < 0)) // W] = True
+            act = row_hot[heads // W] & col_open[tails // W]
+            e_act = int(act.sum())
+            t_act = len(np.unique(tile[act]))
+            b = 8 * e_act + 4 * t_act + 3 * nbm + 8 * fn
+        else:
+            b = 4 * f + 8 * f + 4 * dF + 4 * dF + 12 * fn
+        if k > 0 and dense != trace[k - 1][0]:
+            b += nbm if dense else nbm + 4 * f
+        out.append(b)
+    return np.array(out, np.float64)
+
+
+@pytest.mark.parametrize("W,S,src", [(1024, 16, 0), (1024, 16, 17), (256, 3, 0)])
+def test_alg_bytes_equal_brute_force(ctx, W, S, src):
+    """gphi_bfs_alg_bytes (E_act/T_act counted on the device) equals a
+    tile-by-tile count over every edge; this is the numerator of the roofline."""
+    print(f"case: alg bytes W={W} S={S} src={src}", flush=True)
+    n, h, t = O.config_edges("c1")
+    g = O.build_from_edgelist(n, h, t, W, S)
+    gg = upload(ctx, g)
+    try:
+        lv, _, st = gg.bfs(src, parents=False)
+        got = gg.alg_bytes(lv, st)
+    finally:
+        gg.free()
+    tr = st.trace()
+    assert any(r[0] == 1 for r in tr)
+    want = brute_force_step_bytes(g, lv, tr.tolist())
+    assert np.array_equal(got, want), (got, want)

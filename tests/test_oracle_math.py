@@ -159,10 +159,37 @@ def test_rmat_invariants():
     k = (h.astype(np.uint64) << 32) | t
     assert np.all(np.diff(k) > 0)                  # sorted, no duplicates
     assert np.all(h != t) and h.max() < 4096 and t.max() < 4096
-    # top-bit quadrant frequencies of raw samples follow (a, b, c, d)
-    q = np.zeros(4)
+    # at every bit level, the quadrant frequencies of raw samples follow
+    # (a, b, c, d), and levels are independent: (a+b)^2 of the samples take
+    # head bit 0 at both the top level and the next one
+    hs, ts = np.empty(20000, np.int64), np.empty(20000, np.int64)
     for i in range(20000):
-        hh, tt = O.rmat_sample(5, i, 12)
-        q[(hh >> 11) * 2 + (tt >> 11)] += 1
-    q /= q.sum()
-    assert np.allclose(q, [0.57, 0.19, 0.19, 0.05], atol=0.015)
+        hs[i], ts[i] = O.rmat_sample(5, i, 12)
+    for level in range(12):
+        hb, tb = (hs >> level) & 1, (ts >> level) & 1
+        q = np.bincount(hb * 2 + tb, minlength=4) / len(hs)
+        assert np.allclose(q, [0.57, 0.19, 0.19, 0.05], atol=0.015), (level, q)
+    both = np.mean(((hs >> 11) & 1 == 0) & ((hs >> 10) & 1 == 0))
+    assert abs(both - 0.76 ** 2) < 0.015
+
+
+def test_hybrid_avx512_and_one_lane_sweeps_agree():
+    """The oracle's dense sweep restated with AVX-512 (apps/bfs_simd/bfs.cpp:
+    181-245) and one lane at a time give the same levels and direction trace
+    (both equal the textbook BFS) and valid parents."""
+    g = O.config_graph("c1")
+    width = O.lib().orc_simd_width()
+    try:
+        for src in (0, 9, 4097):
+            ref = O.bfs_levels(g.n, g.deg, g.cols, src)
+            got = []
+            for simd in (1, 0):
+                O.lib().orc_set_simd(simd)
+                lv, pa, tr = O.bfs_hybrid(g, src, nthreads=4)
+                assert np.array_equal(lv, ref)
+                assert O.check_parents(g.n, g.deg, g.cols, src, lv, pa) == 0
+                got.append(tr)
+            assert np.array_equal(got[0], got[1])
+            assert any(r[0] == 1 for r in got[0])      # the sweep ran
+    finally:
+        O.lib().orc_set_simd(1 if width == 512 else 0)
