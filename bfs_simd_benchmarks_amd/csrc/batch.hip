@@ -321,15 +321,19 @@ __global__ __launch_bounds__(kBlock) void batch_pull_heavy_kernel(
 // single-source push: tiles of kSparseTile edges starting at tile_lo)
 template <int NW>
 __global__ __launch_bounds__(kBlock) void batch_sparse_kernel(
-    const StepCtrl *__restrict__ ctrl, const uint32_t *__restrict__ queue, const uint32_t *__restrict__ qoff,
-    const uint32_t *__restrict__ tile_lo, const uint32_t *__restrict__ row_off, const uint32_t *__restrict__ cols,
-    const u64 *__restrict__ front, const u64 *__restrict__ seen, u64 *next, uint32_t *touched)
+    const StepCtrl *__restrict__ ctrl, QueueBufs qb, const uint32_t *__restrict__ row_off,
+    const uint32_t *__restrict__ cols, const u64 *__restrict__ front, const u64 *__restrict__ seen, u64 *next,
+    uint32_t *touched)
 {
 	__shared__ uint32_t s_off[kSparseTile + 2];
 	__shared__ uint32_t s_u[kSparseTile + 2];
 	__shared__ uint32_t s_base[kSparseTile + 2];
 	if (ctrl->done || ctrl->dir != 0) return;
 	const uint32_t qsize = ctrl->qsize, qdeg = ctrl->fdeg;
+	const uint32_t par = ctrl->level & 1u;  // the half the frontier update wrote
+	const uint32_t *__restrict__ queue = qb.queue + par * qb.qstride;
+	const uint32_t *__restrict__ qoff = qb.qoff + par * qb.qstride;
+	const uint32_t *__restrict__ tile_lo = qb.tile_lo + par * qb.tstride;
 	const uint32_t ntile = (qdeg + kSparseTile - 1) / kSparseTile;
 	for (uint32_t k = blockIdx.x; k < ntile; k += gridDim.x) {
 		const uint32_t lo = tile_lo[k];
@@ -361,18 +365,20 @@ __global__ __launch_bounds__(kBlock) void batch_sparse_kernel(
 	}
 }
 
-// The level update.  One lane per 32-vertex word of the touched and frontier
-// bitmaps; a wavefront then walks its 2048 vertices 64 at a time (vertex
-// 64i + lane), skipping runs of 64 with neither bit, so a sparse step costs
-// about its own vertices.  Per vertex: new = next & ~seen; seen |= new;
-// front = new; next = 0; a vertex of the old frontier with nothing new drops
-// its front mask.  The level of every new (source, vertex) pair is the level
-// minus the source's start level, written one source at a time across the
-// wavefront (64 consecutive vertices, one coalesced byte store).  Blocks
-// publish the sources they saw advance into one of kLiveSlots slots and add
-// the new pairs (and their out-degrees) to the totals.
+// The level update, one lane per vertex.  A lane first reads the touched and
+// frontier bitmap words of its vertex (one word serves 32 lanes); a vertex
+// with neither bit has nothing to do, and a wavefront with none skips the
+// rest, so a sparse step costs little beyond the two bitmaps.  Per vertex:
+// new = next & ~seen; seen |= new; front = new; next = 0; a vertex of the
+// old frontier with nothing new drops its front mask.  The level of every
+// new (source, vertex) pair is the level minus the source's start level,
+// written one source at a time across the wavefront (64 consecutive
+// vertices, one coalesced byte store).  Blocks publish the sources they saw
+// advance into one of kLiveSlots slots and add the new pairs (and their
+// out-degrees) to the totals.
 constexpr uint32_t kLiveSlots = 64;
 constexpr uint32_t kLiveStride = 16;  // u64 words per slot (one 128-byte line)
+constexpr uint32_t kSlotPairs = 8;    // slot words 8, 9: new pairs and their out-degrees
 
 template <int NW>
 __global__ __launch_bounds__(kBlock) void batch_update_kernel(
@@ -385,29 +391,21 @@ __global__ __launch_bounds__(kBlock) void batch_update_kernel(
 	__shared__ u64 s_or[kBlock / 64][NW];
 	__shared__ u64 s_cnt[kBlock / 64][2];
 	const uint32_t lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-	const uint32_t nwords = (n + 31) / 32;
-	const uint32_t w = blockIdx.x * kBlock + threadIdx.x;
-	const uint32_t tw = w < nwords ? touched[w] : 0u;
-	const uint32_t cw = w < nwords ? cur[w] : 0u;
-	if (tw) touched[w] = 0u;  // clear for the next step
-	const u64 act = uniform64(__ballot(tw != 0u) | __ballot(cw != 0u));
-	const uint32_t wv0 = (w - lane) * 32;
-	u64 wor[NW];
-#pragma unroll
-	for (int j = 0; j < NW; ++j) wor[j] = 0;
+	const uint32_t v = blockIdx.x * kBlock + threadIdx.x;
+	const bool in = v < n;
+	const uint32_t tw = in ? touched[v >> 5] : 0u;
+	const uint32_t cw = in ? cur[v >> 5] : 0u;
+	const bool t = (tw >> (v & 31)) & 1u, c = (cw >> (v & 31)) & 1u;
+	u64 nw[NW], wor[NW];
 	u64 pairs = 0, pdeg = 0;
 	bool over = false;
-	for (uint32_t i = 0; i < 32; ++i) {
-		if (!((act >> (2 * i)) & 3ull)) continue;  // uniform
-		const uint32_t tb = shfl_u32(tw, 2 * i + (lane >> 5));
-		const uint32_t cb = shfl_u32(cw, 2 * i + (lane >> 5));
-		const uint32_t v = wv0 + 64 * i + lane;
-		const bool t = (tb >> (lane & 31)) & 1u, c = (cb >> (lane & 31)) & 1u;
-		u64 nw[NW];
+#pragma unroll
+	for (int j = 0; j < NW; ++j) nw[j] = wor[j] = 0;
+	if (__ballot(t || c)) {  // uniform
+		if ((v & 31) == 0 && tw) touched[v >> 5] = 0u;  // clear for the next step
 		u64 anyn = 0;
 #pragma unroll
 		for (int j = 0; j < NW; ++j) {
-			nw[j] = 0;
 			const uint64_t idx = (uint64_t)v * NW + j;
 			if (t) {
 				const u64 nx = next[idx];
@@ -426,13 +424,13 @@ __global__ __launch_bounds__(kBlock) void batch_update_kernel(
 			uint32_t pc = 0;
 #pragma unroll
 			for (int j = 0; j < NW; ++j) pc += (uint32_t)__popcll(nw[j]);
-			pairs += pc;
-			pdeg += (u64)pc * deg[v];
+			pairs = pc;
+			pdeg = (u64)pc * deg[v];
 		}
 #pragma unroll
 		for (int j = 0; j < NW; ++j) {
 			u64 u = uniform64(wave_or(nw[j]));
-			wor[j] |= u;
+			wor[j] = u;
 			while (u) {
 				const uint32_t b = (uint32_t)__ffsll((long long)u) - 1;
 				u &= u - 1;
@@ -442,9 +440,9 @@ __global__ __launch_bounds__(kBlock) void batch_update_kernel(
 				if ((nw[j] >> b) & 1) level[(uint64_t)s * n_pitch + v] = (uint8_t)rel;
 			}
 		}
+		pairs = wave_add(pairs);
+		pdeg = wave_add(pdeg);
 	}
-	pairs = wave_add(pairs);
-	pdeg = wave_add(pdeg);
 	if (lane == 0) {
 #pragma unroll
 		for (int j = 0; j < NW; ++j) s_or[wid][j] = wor[j];
@@ -463,9 +461,13 @@ __global__ __launch_bounds__(kBlock) void batch_update_kernel(
 		for (uint32_t k = 0; k < kBlock / 64; ++k) u |= s_or[k][threadIdx.x];
 		u64 *slot = slots + (uint64_t)(blockIdx.x % kLiveSlots) * kLiveStride + threadIdx.x;
 		// read the slot at L2 (not a stale L1 copy) and add only new sources
-		const u64 have = __hip_atomic_load(slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-		if (u & ~have) atomicOr(slot, u);
+		if (u) {
+			const u64 have = __hip_atomic_load(slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+			if (u & ~have) atomicOr(slot, u);
+		}
 	} else if (threadIdx.x == 64) {
+		// the totals go to the same spread-out slot lines (words kSlotPairs,
+		// kSlotPairs + 1); the live kernel adds them up
 		u64 p = 0, d = 0;
 #pragma unroll
 		for (uint32_t k = 0; k < kBlock / 64; ++k) {
@@ -473,8 +475,9 @@ __global__ __launch_bounds__(kBlock) void batch_update_kernel(
 			d += s_cnt[k][1];
 		}
 		if (p) {
-			atomicAdd(sums, p);
-			atomicAdd(sums + 1, d);
+			u64 *slot = slots + (uint64_t)(blockIdx.x % kLiveSlots) * kLiveStride + kSlotPairs;
+			atomicAdd(slot, p);
+			atomicAdd(slot + 1, d);
 		}
 	}
 }
@@ -482,7 +485,7 @@ __global__ __launch_bounds__(kBlock) void batch_update_kernel(
 // live = OR of the slots (the sources whose new frontier is non-empty); the
 // slots are cleared for the next step.  One wavefront.
 template <int NW>
-__global__ void batch_live_kernel(const StepCtrl *__restrict__ ctrl, u64 *live, u64 *slots)
+__global__ void batch_live_kernel(const StepCtrl *__restrict__ ctrl, u64 *live, u64 *slots, u64 *sums)
 {
 	if (ctrl->done) return;
 	const uint32_t lane = threadIdx.x;
@@ -492,6 +495,15 @@ __global__ void batch_live_kernel(const StepCtrl *__restrict__ ctrl, u64 *live, 
 		const u64 x = wave_or(*p);
 		*p = 0;
 		if (lane == 0) live[i] = x;
+	}
+	// the step's new pairs and their out-degrees, into the totals
+	u64 *t = slots + (uint64_t)lane * kLiveStride + kSlotPairs;
+	const u64 p = wave_add(t[0]), d = wave_add(t[1]);
+	t[0] = 0;
+	t[1] = 0;
+	if (lane == 0) {
+		sums[0] += p;
+		sums[1] += d;
 	}
 }
 
@@ -595,8 +607,8 @@ void enqueue_expand(gphi_graph *gr, uint32_t next_level)
 {
 	const DevGraph &G = gr->g;
 	hipStream_t s = gr->ctx->stream;
-	batch_sparse_kernel<NW><<<kBatchSparseGrid, kBlock, 0, s>>>(gr->ctrl, gr->queue, gr->qoff, gr->tile_lo, G.row_off,
-	                                                           G.cols, gr->b_front, gr->b_seen, gr->b_next, gr->b_touch);
+	batch_sparse_kernel<NW><<<kBatchSparseGrid, kBlock, 0, s>>>(gr->ctrl, gr->queue_bufs(), G.row_off, G.cols,
+	                                                           gr->b_front, gr->b_seen, gr->b_next, gr->b_touch);
 	const unsigned lg = grid_n(G.n);
 	const u64 *live = live_words(gr);
 	batch_pull_light_kernel<NW, true><<<lg, kBlock, 0, s>>>(G.n, gr->ctrl, gr->in_off, gr->in_src, gr->cur, gr->b_front,
@@ -619,11 +631,10 @@ void enqueue_update(gphi_graph *gr, uint32_t next_level)
 {
 	const DevGraph &G = gr->g;
 	hipStream_t s = gr->ctx->stream;
-	const uint32_t nwords = (G.n + 31) / 32;
-	batch_update_kernel<NW><<<grid_n(nwords), kBlock, 0, s>>>(
+	batch_update_kernel<NW><<<grid_n(G.n), kBlock, 0, s>>>(
 	    G.n, G.n_pad, next_level, gr->ctrl, gr->cur, gr->b_touch, gr->b_front, gr->b_seen, gr->b_next, gr->b_level,
 	    gr->flag, slot_starts(gr), G.deg, live_slots(gr), gr->b_sums);
-	batch_live_kernel<NW><<<1, 64, 0, s>>>(gr->ctrl, live_words(gr), live_slots(gr));
+	batch_live_kernel<NW><<<1, 64, 0, s>>>(gr->ctrl, live_words(gr), live_slots(gr), gr->b_sums);
 }
 
 template <int NW>

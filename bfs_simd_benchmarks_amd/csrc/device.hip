@@ -481,13 +481,14 @@ void free_traversal_state(gphi_graph *gr)
 	               (void **)&gr->queue, (void **)&gr->qoff, (void **)&gr->tile_lo, (void **)&gr->ctrl,
 	               (void **)&gr->blk_cnt, (void **)&gr->blk_base, (void **)&gr->wave_cnt, (void **)&gr->live, (void **)&gr->live_cnt,
 	               (void **)&gr->trace,
-	               (void **)&gr->rank, (void **)&gr->sum, (void **)&gr->contrib, (void **)&gr->b_front,
+	               (void **)&gr->rank, (void **)&gr->b_front,
 	               (void **)&gr->b_seen, (void **)&gr->b_next, (void **)&gr->b_level, (void **)&gr->b_sums,
 	               (void **)&gr->b_live, (void **)&gr->b_touch, (void **)&gr->b_aux};
 	for (void **p : ps) {
 		if (*p) (void)hipFree(*p);
 		*p = nullptr;
 	}
+	gr->sum = gr->contrib = nullptr;  // inside rank's allocation
 	gr->batch_words = 0;
 }
 
@@ -544,9 +545,9 @@ static int alloc_state(gphi_graph *gr)
 	if ((rc = dmalloc(ctx, (void **)&gr->cur, (size_t)G.n_words * 4, nullptr))) return rc;
 	if ((rc = dmalloc(ctx, (void **)&gr->row_stamp, ((size_t)G.side + 1) * 4, nullptr))) return rc;
 	if ((rc = dmalloc(ctx, (void **)&gr->col_unvis, ((size_t)G.side + 1) * 4, nullptr))) return rc;
-	if ((rc = dmalloc(ctx, (void **)&gr->queue, nb, nullptr))) return rc;
-	if ((rc = dmalloc(ctx, (void **)&gr->qoff, nb, nullptr))) return rc;
-	if ((rc = dmalloc(ctx, (void **)&gr->tile_lo, (G.m / kSparseTile + 2) * 4, nullptr))) return rc;
+	if ((rc = dmalloc(ctx, (void **)&gr->queue, 2 * (size_t)queue_stride(G.n_pad) * 4, nullptr))) return rc;
+	if ((rc = dmalloc(ctx, (void **)&gr->qoff, 2 * (size_t)queue_stride(G.n_pad) * 4, nullptr))) return rc;
+	if ((rc = dmalloc(ctx, (void **)&gr->tile_lo, 2 * (size_t)tile_lo_stride(G.m) * 4, nullptr))) return rc;
 	if ((rc = dmalloc(ctx, (void **)&gr->ctrl, sizeof(StepCtrl), nullptr))) return rc;
 	const size_t list_cap = live_list_cap(G.nchunks);
 	if ((rc = dmalloc(ctx, (void **)&gr->live, list_cap * kNumXcd * 4 + 4, nullptr))) return rc;
@@ -626,6 +627,8 @@ __global__ void init_source_kernel(uint32_t src, uint32_t W, const uint32_t *__r
 		ctrl->changed = 0;
 		ctrl->chunks = 0;
 		ctrl->level = 0;
+		ctrl->queued = 0;
+		ctrl->col_stale = 0;
 		ctrl->reached = 1;
 		ctrl->traversed = d;
 		trace[0] = make_uint4(0u, 1u, d, 0u);
@@ -1129,19 +1132,37 @@ __global__ __launch_bounds__(kBlock) void sssp_dense_kernel(
 // sparse step: CSR push, edge-balanced over kSparseTile-edge tiles whose first
 // queue entry (tile_lo) was written by the previous frontier update.
 // ---------------------------------------------------------------------------
+// A small BFS sparse step (queue_step) also builds the next frontier itself:
+// a tail is claimed by setting its visited bit (the lane whose atomicOr finds
+// the bit clear wins), and the winner writes its parent, level, state and
+// tile-row stamp, then appends the tail (when it has out-edges) to the next
+// queue -- position and degree prefix reserved with one atomic per block and
+// tile -- and the first-entry index of every next-step tile its edges start.
+// So the level needs no pass over all vertices.  Column counts are left to a
+// recount before the next dense step (col_recount_kernel).  A large step
+// claims through the state byte and leaves the next frontier to that pass.
 template <bool kBfs>
 __device__ __forceinline__ void sparse_push_block(
-    uint32_t bid, uint32_t nbid, const uint32_t *__restrict__ queue, const uint32_t *__restrict__ qoff,
-    const uint32_t *__restrict__ tile_lo, const StepCtrl *__restrict__ ctrl,
-    const uint32_t *__restrict__ row_off, const uint32_t *__restrict__ cols,
-    const uint32_t *__restrict__ csr_w, const uint32_t *__restrict__ visited,
-    uint8_t *__restrict__ flag, int32_t *__restrict__ parent, uint32_t *dist)
+    uint32_t bid, uint32_t nbid, QueueBufs qb, StepCtrl *ctrl, const uint32_t *__restrict__ row_off,
+    const uint32_t *__restrict__ cols, const uint32_t *__restrict__ csr_w, const uint32_t *__restrict__ deg,
+    uint32_t *visited, uint8_t *__restrict__ flag, int32_t *__restrict__ parent, int32_t *__restrict__ level,
+    uint32_t *dist, uint32_t W, uint32_t *__restrict__ row_stamp, uint32_t *col_unvis)
 {
 	__shared__ uint32_t s_off[kSparseTile + 2];
 	__shared__ uint32_t s_u[kSparseTile + 2];
 	__shared__ uint32_t s_base[kSparseTile + 2];
+	__shared__ uint32_t s_res[kBlock / 64][3];  // BFS: per wavefront |F'|, queue entries, degree sum
+	__shared__ uint32_t s_rbase[2];             // BFS: the block's queue position and degree offset
 	if (ctrl->done || ctrl->dir != 0) return;  // a dense step, or a finished traversal
 	const uint32_t qsize = ctrl->qsize, qdeg = ctrl->fdeg;
+	const uint32_t L = ctrl->level, par = L & 1u;
+	const bool qmode = kBfs && queue_step(0, qdeg);  // this step builds the next queue itself
+	const uint32_t *__restrict__ queue = qb.queue + par * qb.qstride;
+	const uint32_t *__restrict__ qoff = qb.qoff + par * qb.qstride;
+	const uint32_t *__restrict__ tile_lo = qb.tile_lo + par * qb.tstride;
+	uint32_t *nqueue = qb.queue + (par ^ 1u) * qb.qstride;
+	uint32_t *nqoff = qb.qoff + (par ^ 1u) * qb.qstride;
+	uint32_t *ntile_lo = qb.tile_lo + (par ^ 1u) * qb.tstride;
 	uint32_t ntile = (qdeg + kSparseTile - 1) / kSparseTile;
 	for (uint32_t k = bid; k < ntile; k += nbid) {
 		uint32_t lo = tile_lo[k];
@@ -1184,17 +1205,95 @@ __device__ __forceinline__ void sparse_push_block(
 			}
 #pragma unroll
 			for (int r = 0; r < kPer; ++r) vw[r] = on[r] ? visited[vv[r] >> 5] : 0xFFFFFFFFu;
+			if (!qmode) {
+				// a large step: a tail is claimed through its state byte, and the
+				// frontier update's pass builds the next frontier from those
+#pragma unroll
+				for (int r = 0; r < kPer; ++r) {
+					on[r] = on[r] && !((vw[r] >> (vv[r] & 31)) & 1u);
+					sv[r] = on[r] ? flag[vv[r]] : 1u;
+				}
+#pragma unroll
+				for (int r = 0; r < kPer; ++r)
+					if (on[r] && sv[r] == 0) {
+						parent[vv[r]] = (int32_t)uu[r];
+						flag[vv[r]] = 1;
+					}
+				__syncthreads();
+				continue;
+			}
+			// claims: the lane that sets a still-clear visited bit wins the tail
 #pragma unroll
 			for (int r = 0; r < kPer; ++r) {
-				on[r] = on[r] && !((vw[r] >> (vv[r] & 31)) & 1u);
-				sv[r] = on[r] ? flag[vv[r]] : 1u;
+				const uint32_t bit = 1u << (vv[r] & 31);
+				on[r] = on[r] && !(vw[r] & bit);
+				sv[r] = on[r] ? atomicOr(visited + (vv[r] >> 5), bit) & bit : bit;
 			}
+			const uint32_t next_level = L + 1;
+			uint32_t nf = 0, nq = 0, nd = 0, dv[kPer];
 #pragma unroll
-			for (int r = 0; r < kPer; ++r)
+			for (int r = 0; r < kPer; ++r) {
+				dv[r] = 0;
 				if (on[r] && sv[r] == 0) {
-					parent[vv[r]] = (int32_t)uu[r];
-					flag[vv[r]] = 1;
+					const uint32_t v = vv[r];
+					dv[r] = deg[v];
+					parent[v] = (int32_t)uu[r];
+					level[v] = (int32_t)next_level;
+					flag[v] = kVisited;
+					// many winners share a hub row: store the stamp only once
+					if (dv[r] && row_stamp[v / W] != next_level) row_stamp[v / W] = next_level;
+					nf += 1;
+					nq += dv[r] > 0;
+					nd += dv[r];
 				}
+			}
+			// the winners' next-queue entries: one reservation per block and
+			// tile (|F'| and the (entries, degree sum) pair, two atomics on
+			// the control block), then each winner's position and degree
+			// offset from the block's and the wavefront's prefixes
+			uint32_t ftot, qtot, dtot;
+			(void)wave_excl_scan(nf, &ftot);
+			uint32_t qx = wave_excl_scan(nq, &qtot), dx = wave_excl_scan(nd, &dtot);
+			const uint32_t wid = threadIdx.x >> 6;
+			if (lane_id() == 0) {
+				s_res[wid][0] = ftot;
+				s_res[wid][1] = qtot;
+				s_res[wid][2] = dtot;
+			}
+			__syncthreads();
+			if (threadIdx.x == 0) {
+				uint32_t f = 0, q = 0, d = 0;
+				for (uint32_t w2 = 0; w2 < kBlock / 64; ++w2) {
+					f += s_res[w2][0];
+					q += s_res[w2][1];
+					d += s_res[w2][2];
+				}
+				unsigned long long base = 0;
+				if (f) atomicAdd(&ctrl->facc, f);
+				if (q) base = atomicAdd(&ctrl->qpack, ((unsigned long long)q << 32) | d);
+				s_rbase[0] = (uint32_t)(base >> 32);
+				s_rbase[1] = (uint32_t)base;
+			}
+			__syncthreads();
+			if (qtot) {
+				uint32_t pos = s_rbase[0] + qx, off = s_rbase[1] + dx;
+				for (uint32_t w2 = 0; w2 < wid; ++w2) {
+					pos += s_res[w2][1];
+					off += s_res[w2][2];
+				}
+#pragma unroll
+				for (int r = 0; r < kPer; ++r)
+					if (dv[r]) {
+						nqueue[pos] = vv[r];
+						nqoff[pos] = off;
+						// next-step tiles whose first edge falls in [off, off + d)
+						for (uint32_t k = (off + kSparseTile - 1) / kSparseTile;
+						     (uint64_t)k * kSparseTile < (uint64_t)off + dv[r]; ++k)
+							ntile_lo[k] = pos;
+						pos += 1;
+						off += dv[r];
+					}
+			}
 		}
 		for (uint32_t e = e0 + threadIdx.x; !kBfs && e < e1; e += kBlock) {
 			// largest j with s_off[j] <= e
@@ -1229,23 +1328,37 @@ __device__ __forceinline__ void sparse_push_block(
 // [0, npush), or the chunk selection of a dense step on blocks [0, nsel)
 // (block-uniform branches), so a level does not launch the one it does not
 // need.
+//
+// A BFS dense step that follows a queue-building sparse step first sets the
+// frontier bits of its queue (that step built the queue but not the bitmap,
+// which the frontier update cleared); a frontier vertex without out-edges is
+// in no queue and needs no bit, as it heads no edge of the sweep.
 template <bool kBfs>
 __global__ __launch_bounds__(kBlock) void level_front_kernel(
-    uint32_t npush, uint32_t nsel, const uint32_t *__restrict__ queue, const uint32_t *__restrict__ qoff,
-    const uint32_t *__restrict__ tile_lo, const StepCtrl *__restrict__ ctrl, const uint32_t *__restrict__ row_off,
-    const uint32_t *__restrict__ cols, const uint32_t *__restrict__ csr_w, const uint32_t *__restrict__ visited,
-    uint8_t *__restrict__ flag, int32_t *__restrict__ parent, uint32_t *dist, const ChunkMeta *__restrict__ meta,
-    uint64_t nchunks, const uint32_t *__restrict__ row_stamp, const uint32_t *__restrict__ col_unvis,
-    uint32_t *__restrict__ lists, uint32_t list_cap, uint32_t *__restrict__ counts)
+    uint32_t npush, uint32_t nsel, QueueBufs qb, StepCtrl *ctrl, const uint32_t *__restrict__ row_off,
+    const uint32_t *__restrict__ cols, const uint32_t *__restrict__ csr_w, const uint32_t *__restrict__ deg,
+    uint32_t *visited, uint8_t *__restrict__ flag, int32_t *__restrict__ parent, int32_t *__restrict__ level,
+    uint32_t *dist, uint32_t W, const ChunkMeta *__restrict__ meta, uint64_t nchunks, uint32_t *row_stamp,
+    uint32_t *col_unvis, uint32_t *__restrict__ lists, uint32_t list_cap, uint32_t *__restrict__ counts,
+    uint32_t *cur)
 {
 	if (ctrl->done) return;
 	if (ctrl->dir == 0) {
 		if (blockIdx.x < npush)
-			sparse_push_block<kBfs>(blockIdx.x, npush, queue, qoff, tile_lo, ctrl, row_off, cols, csr_w, visited,
-			                        flag, parent, dist);
-	} else if (blockIdx.x < nsel) {
-		chunk_select_block<kBfs>(blockIdx.x, meta, nchunks, row_stamp, col_unvis, ctrl, lists, list_cap, counts);
+			sparse_push_block<kBfs>(blockIdx.x, npush, qb, ctrl, row_off, cols, csr_w, deg, visited, flag, parent,
+			                        level, dist, W, row_stamp, col_unvis);
+		return;
 	}
+	if (kBfs && ctrl->queued) {
+		const uint32_t *queue = qb.queue + (ctrl->level & 1u) * qb.qstride;
+		const uint32_t qsize = ctrl->qsize;
+		for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < qsize; i += gridDim.x * kBlock) {
+			const uint32_t v = queue[i];
+			atomicOr(cur + (v >> 5), 1u << (v & 31));
+		}
+	}
+	if (blockIdx.x < nsel)
+		chunk_select_block<kBfs>(blockIdx.x, meta, nchunks, row_stamp, col_unvis, ctrl, lists, list_cap, counts);
 }
 
 // ---------------------------------------------------------------------------
@@ -1272,6 +1385,8 @@ __global__ __launch_bounds__(kBlock) void frontier_mark_kernel(
 	__shared__ uint32_t s_red[3][kBlock / 64];
 	if (next_level == kLevelFromCtrl) {  // device-driven loop
 		if (ctrl->done) return;
+		// a small BFS sparse step built its next frontier itself (sparse_push_block)
+		if (kBfs && queue_step(ctrl->dir, ctrl->fdeg)) return;
 		next_level = ctrl->level + 1;
 	}
 	const uint32_t w = blockIdx.x * kBlock + threadIdx.x;
@@ -1368,13 +1483,58 @@ __global__ __launch_bounds__(kBlock) void frontier_mark_kernel(
 // One block: exclusive scan of per-block (queue, degree) counts, totals, and
 // the direction rule |F| + sum(outdeg) > M / T_RATIO in uint32 arithmetic.
 constexpr uint32_t kScanThreads = 1024;
+// The end of a level, one thread: the next frontier's totals, the direction
+// rule |F| + sum(outdeg) > M / T_RATIO in uint32 arithmetic, and (device-
+// driven loop) the step's trace entry, the next level, and cleared live-list
+// counts for its chunk selection.
+__device__ void finish_level(StepCtrl *ctrl, uint4 *trace, uint32_t *live_cnt, uint32_t threshold, uint32_t fall,
+                             uint32_t qall, uint32_t dall, bool queued)
+{
+	ctrl->fsize = fall;
+	ctrl->fdeg = dall;
+	ctrl->qsize = qall;
+	ctrl->queued = queued;
+	// the column counts were recounted before a dense step and kept by its
+	// frontier pass; a queue-building step leaves them behind again
+	if (ctrl->dir == 1) ctrl->col_stale = 0;
+	if (queued) ctrl->col_stale = 1;
+	ctrl->dir = (uint32_t)(fall + dall) > threshold ? 1u : 0u;
+	ctrl->done = fall == 0;
+	uint32_t chunks = 0;  // chunks the step's dense sweep took (its live lists)
+	for (int x = 0; x < kNumXcd; ++x) chunks += live_cnt[x * kLiveCntStride];
+	ctrl->chunks = chunks;
+	if (trace) {
+		const uint32_t L = ctrl->level;
+		if (L <= GPHI_MAX_STEPS) trace[L].w = chunks;
+		if (fall && L + 1 <= GPHI_MAX_STEPS) trace[L + 1] = make_uint4(ctrl->dir, fall, dall, 0u);
+		ctrl->level = L + 1;
+		ctrl->reached += fall;
+		ctrl->traversed += dall;
+		for (int x = 0; x < kNumXcd; ++x) live_cnt[x * kLiveCntStride] = 0;
+	}
+}
+
+// queue_path: a small BFS sparse step (queue_step) counted its own next
+// frontier (|F'|, queue entries and degree sum in the control block's
+// accumulators), so there is nothing to scan.
 __global__ __launch_bounds__(kScanThreads) void frontier_scan_kernel(uint32_t nblk, uint32_t threshold,
                                                                      const uint4 *__restrict__ blk_cnt,
                                                                      uint2 *__restrict__ blk_base,
                                                                      uint32_t *__restrict__ live_cnt,
-                                                                     StepCtrl *ctrl, uint4 *__restrict__ trace)
+                                                                     StepCtrl *ctrl, uint4 *__restrict__ trace,
+                                                                     bool queue_path)
 {
 	if (trace && ctrl->done) return;  // device-driven loop: nothing left to do
+	if (queue_path && queue_step(ctrl->dir, ctrl->fdeg)) {
+		if (threadIdx.x == 0) {
+			const uint32_t fall = ctrl->facc;
+			const unsigned long long qp = ctrl->qpack;
+			ctrl->facc = 0;
+			ctrl->qpack = 0;
+			finish_level(ctrl, trace, live_cnt, threshold, fall, (uint32_t)(qp >> 32), (uint32_t)qp, true);
+		}
+		return;
+	}
 	__shared__ uint32_t sq[kScanThreads / 64], sd[kScanThreads / 64], sf[kScanThreads / 64];
 	const uint32_t per = (nblk + kScanThreads - 1) / kScanThreads;
 	const uint32_t b0 = min(nblk, threadIdx.x * per), b1 = min(nblk, b0 + per);
@@ -1413,37 +1573,55 @@ __global__ __launch_bounds__(kScanThreads) void frontier_scan_kernel(uint32_t nb
 		qx += c.y;
 		dx += c.z;
 	}
-	if (threadIdx.x == 0) {
-		ctrl->fsize = fall;
-		ctrl->fdeg = dall;
-		ctrl->qsize = qall;
-		ctrl->dir = (uint32_t)(fall + dall) > threshold ? 1u : 0u;
-		ctrl->done = fall == 0;
-		uint32_t chunks = 0;  // chunks the step's dense sweep took (its live lists)
-		for (int x = 0; x < kNumXcd; ++x) chunks += live_cnt[x * kLiveCntStride];
-		ctrl->chunks = chunks;
-		if (trace) {
-			// device-driven loop: record the step, open the next one, and clear
-			// the live-list counts for its chunk selection
-			const uint32_t L = ctrl->level;
-			if (L <= GPHI_MAX_STEPS) trace[L].w = chunks;
-			if (fall && L + 1 <= GPHI_MAX_STEPS) trace[L + 1] = make_uint4(ctrl->dir, fall, dall, 0u);
-			ctrl->level = L + 1;
-			ctrl->reached += fall;
-			ctrl->traversed += dall;
-			for (int x = 0; x < kNumXcd; ++x) live_cnt[x * kLiveCntStride] = 0;
-		}
-	}
+	if (threadIdx.x == 0) finish_level(ctrl, trace, live_cnt, threshold, fall, qall, dall, false);
 }
 
-__global__ __launch_bounds__(kBlock) void frontier_emit_kernel(
-    const StepCtrl *__restrict__ ctrl, const uint32_t *__restrict__ deg, const uint32_t *__restrict__ cur,
-    const uint4 *__restrict__ blk_cnt, const uint2 *__restrict__ blk_base, const uint2 *__restrict__ wave_cnt,
-    uint32_t *__restrict__ queue, uint32_t *__restrict__ qoff, uint32_t *__restrict__ tile_lo)
+// Unvisited vertices per tail column, recounted from the visited bitmap
+// before a BFS dense step when queue-building sparse steps came before it
+// (their pushes leave the counts alone: many winners share a column, and one
+// atomic each on it would queue); the dense step's chunk selection skips
+// columns with none left.  A thread per column.
+__global__ __launch_bounds__(kBlock) void col_recount_kernel(const StepCtrl *__restrict__ ctrl, uint32_t n,
+                                                             uint32_t W, uint32_t side,
+                                                             const uint32_t *__restrict__ visited,
+                                                             uint32_t *__restrict__ col_unvis)
 {
-	// the queue only feeds a sparse step; empty blocks have nothing to emit
-	if (ctrl->done || ctrl->dir != 0 || blk_cnt[blockIdx.x].y == 0) return;
+	if (ctrl->done || !ctrl->col_stale || ctrl->dir != 1) return;
+	const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
+	if (c >= side) return;
+	const uint32_t lo = c * W, hi = min(lo + W, n);  // vertices [lo, hi)
+	uint32_t seen = 0;
+	for (uint32_t w = lo >> 5; w <= (hi - 1) >> 5; ++w) {
+		uint32_t bits = visited[w];
+		if (w == (lo >> 5) && (lo & 31)) bits &= ~0u << (lo & 31);
+		if (w == ((hi - 1) >> 5) && (hi & 31)) bits &= ~0u >> (32 - (hi & 31));
+		seen += (uint32_t)__popc(bits);
+	}
+	col_unvis[c] = (hi - lo) - seen;
+}
+
+// The queue goes to the half of the next level's parity.  queue_path: after a
+// queue-building BFS sparse step the push built the queue already; on a
+// switch to a dense step the frontier bitmap, which such a step leaves as it
+// was, is cleared here (one lane per word) and refilled from the queue by the
+// next level's first kernel.
+__global__ __launch_bounds__(kBlock) void frontier_emit_kernel(
+    const StepCtrl *__restrict__ ctrl, const uint32_t *__restrict__ deg, uint32_t *__restrict__ cur,
+    const uint4 *__restrict__ blk_cnt, const uint2 *__restrict__ blk_base, const uint2 *__restrict__ wave_cnt,
+    QueueBufs qb, bool queue_path)
+{
+	if (ctrl->done) return;
 	const uint32_t w = blockIdx.x * kBlock + threadIdx.x;
+	if (queue_path && ctrl->queued) {
+		if (ctrl->dir == 1) cur[w] = 0u;
+		return;
+	}
+	// the queue only feeds a sparse step; empty blocks have nothing to emit
+	if (ctrl->dir != 0 || blk_cnt[blockIdx.x].y == 0) return;
+	const uint32_t par = ctrl->level & 1u;
+	uint32_t *__restrict__ queue = qb.queue + par * qb.qstride;
+	uint32_t *__restrict__ qoff = qb.qoff + par * qb.qstride;
+	uint32_t *__restrict__ tile_lo = qb.tile_lo + par * qb.tstride;
 	const uint32_t lane = lane_id(), wid = threadIdx.x >> 6;
 	const uint32_t bits = cur[w];
 	// this wavefront's first queue entry and degree prefix: the block's, plus
@@ -1561,14 +1739,15 @@ static int frontier_loop(gphi_graph *gr, uint32_t src, gphi_stats *st)
 	unsigned grid = (unsigned)std::min<uint64_t>(dense_cap, (G.nchunks + kNumXcd - 1) / kNumXcd * kNumXcd);
 	grid = std::max<unsigned>(grid / kNumXcd * kNumXcd, kNumXcd);
 	const uint32_t win_rows = dense_window_rows(G.W);
+	const QueueBufs qb = gr->queue_bufs();
 	uint32_t enq = 0;
 	for (;;) {
 		for (uint32_t q = 0; q < kLevelGroup; ++q, ++enq) {
 			const bool ev_step = timing && enq < GPHI_MAX_STEPS - 1;
 			level_front_kernel<kBfs><<<std::max(kSparseGrid, sgrid), kBlock, 0, s>>>(
-			    kSparseGrid, sgrid, gr->queue, gr->qoff, gr->tile_lo, gr->ctrl, G.row_off, G.cols, G.csr_w,
-			    gr->visited, gr->flag, gr->parent, gr->dist, G.meta, G.nchunks, gr->row_stamp, gr->col_unvis,
-			    gr->live, list_cap, gr->live_cnt);
+			    kSparseGrid, sgrid, qb, gr->ctrl, G.row_off, G.cols, G.csr_w, G.deg, gr->visited, gr->flag,
+			    gr->parent, gr->level, gr->dist, G.W, G.meta, G.nchunks, gr->row_stamp, gr->col_unvis, gr->live,
+			    list_cap, gr->live_cnt, gr->cur);
 			if (ev_step) GPHI_HIP(ctx, hipEventRecord(ev[2 * GPHI_MAX_STEPS + 4 + enq], s));
 			if (kBfs)
 				bfs_dense_kernel<<<grid, kBlock, 0, s>>>(G.heads, G.tails, G.heads16, G.tails16, G.meta, G.pack,
@@ -1591,9 +1770,12 @@ static int frontier_loop(gphi_graph *gr, uint32_t src, gphi_stats *st)
 			                                                  gr->visited, gr->cur, gr->row_stamp, gr->col_unvis,
 			                                                  gr->blk_cnt, gr->wave_cnt, gr->ctrl);
 			frontier_scan_kernel<<<1, kScanThreads, 0, s>>>(nblk, threshold, gr->blk_cnt, gr->blk_base,
-			                                              gr->live_cnt, gr->ctrl, gr->trace);
+			                                              gr->live_cnt, gr->ctrl, gr->trace, kBfs);
 			frontier_emit_kernel<<<nblk, kBlock, 0, s>>>(gr->ctrl, G.deg, gr->cur, gr->blk_cnt, gr->blk_base,
-			                                            gr->wave_cnt, gr->queue, gr->qoff, gr->tile_lo);
+			                                            gr->wave_cnt, qb, kBfs);
+			if (kBfs)
+				col_recount_kernel<<<grid_for(G.side, kBlock, 1u << 30), kBlock, 0, s>>>(gr->ctrl, G.n, G.W, G.side,
+				                                                                        gr->visited, gr->col_unvis);
 			if (ev_step) GPHI_HIP(ctx, hipEventRecord(ev[3 + 2 * enq], s));
 		}
 		GPHI_HIP(ctx, hipGetLastError());
@@ -1664,9 +1846,9 @@ void launch_frontier_update_driven(gphi_graph *gr, uint32_t threshold)
 	                                                   gr->visited, gr->cur, gr->row_stamp, gr->col_unvis,
 	                                                   gr->blk_cnt, gr->wave_cnt, gr->ctrl);
 	frontier_scan_kernel<<<1, kScanThreads, 0, s>>>(nblk, threshold, gr->blk_cnt, gr->blk_base, gr->live_cnt,
-	                                              gr->ctrl, gr->trace);
+	                                              gr->ctrl, gr->trace, false);
 	frontier_emit_kernel<<<nblk, kBlock, 0, s>>>(gr->ctrl, G.deg, gr->cur, gr->blk_cnt, gr->blk_base,
-	                                            gr->wave_cnt, gr->queue, gr->qoff, gr->tile_lo);
+	                                            gr->wave_cnt, gr->queue_bufs(), false);
 }
 
 // level and parent -1 for the vertices a BFS did not reach (state 0; every
@@ -1733,26 +1915,29 @@ int run_sssp(gphi_graph *gr, uint32_t src, uint32_t *dist, gphi_stats *st)
 // rank = (1-d)/N + d*sum   (apps/pageRank_simd/page_rank.cpp:128-199,428-446)
 // ---------------------------------------------------------------------------
 __global__ void pr_prepare_kernel(uint32_t n, const float *__restrict__ rank,
-                                  const uint32_t *__restrict__ deg, float *__restrict__ contrib,
-                                  float *__restrict__ sum)
+                                  const uint32_t *__restrict__ deg, float *__restrict__ contrib)
 {
 	uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
 	if (v < n) {
 		uint32_t d = deg[v];
 		contrib[v] = d ? rank[v] / (float)d : 0.f;
-		sum[v] = 0.f;
 	}
 }
 
 // PageRank sum (apps/pageRank_simd/page_rank.cpp:128-199 gathers
 // rank[h]/deg[h] into sum[t] over the tiles) as a pull over the in-edges:
-// sum[t] = sum of contrib[h] over t's in-neighbours, with no atomics except
-// one per piece of a high in-degree vertex.  Vertices with up to kPullLight
-// in-edges get kPullGroup lanes each (every kPullGroup-th in-edge, then a
-// sum across the group); the others a wavefront per piece of up to
-// kPullPiece in-edges, whose partial sums are added with atomicAdd after the
-// light kernel has zeroed them.
+// sum[t] = sum of contrib[h] over t's in-neighbours, in a fixed order, so an
+// iteration gives the same FP32 bits on every run.  Vertices with up to
+// kPullLight in-edges get kPullGroup lanes each: lane g loads in-edges g,
+// g + kPullGroup, ... (all of them before any is used), adds their
+// contributions in that order, and the group adds its lanes in a fixed
+// butterfly.  The others get a wavefront per piece of up to kPullPiece
+// in-edges, rounds of kPrUnroll * 64 edges with the same scheme; each piece
+// writes its partial sum, and pr_piece_sum_kernel adds a vertex's pieces in
+// piece order.
 constexpr uint32_t kPullGroup = 8;
+constexpr uint32_t kPrSlots = kPullLight / kPullGroup;  // in-edges per lane of a light vertex
+constexpr uint32_t kPrUnroll = 8;                       // loads in flight per lane of a heavy piece
 
 __global__ __launch_bounds__(kBlock) void pr_pull_light_kernel(uint32_t n, const uint32_t *__restrict__ in_off,
                                                                const uint32_t *__restrict__ in_src,
@@ -1767,36 +1952,81 @@ __global__ __launch_bounds__(kBlock) void pr_pull_light_kernel(uint32_t n, const
 		cnt = in_off[v + 1] - e0;
 	}
 	const bool heavy = cnt > kPullLight;
+	uint32_t h[kPrSlots];
+#pragma unroll
+	for (uint32_t j = 0; j < kPrSlots; ++j) {
+		const uint32_t k = j * kPullGroup + g;
+		h[j] = !heavy && k < cnt ? in_src[e0 + k] : 0xFFFFFFFFu;
+	}
+	float c[kPrSlots];
+#pragma unroll
+	for (uint32_t j = 0; j < kPrSlots; ++j) c[j] = h[j] != 0xFFFFFFFFu ? contrib[h[j]] : 0.f;
 	float acc = 0.f;
-	if (!heavy)
-		for (uint32_t j = g; j < cnt; j += kPullGroup) acc += contrib[in_src[e0 + j]];
+#pragma unroll
+	for (uint32_t j = 0; j < kPrSlots; ++j) acc += c[j];
 #pragma unroll
 	for (int o = kPullGroup / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, kPullGroup);
-	if (g == 0 && v < n) sum[v] = heavy ? 0.f : acc;
+	if (g == 0 && v < n && !heavy) sum[v] = acc;  // heavy vertices: pr_piece_sum_kernel
 }
 
 __global__ __launch_bounds__(kBlock) void pr_pull_heavy_kernel(uint32_t npieces, const uint2 *__restrict__ pieces,
                                                                const uint32_t *__restrict__ in_off,
                                                                const uint32_t *__restrict__ in_src,
-                                                               const float *__restrict__ contrib, float *sum)
+                                                               const float *__restrict__ contrib,
+                                                               float *__restrict__ psum)
 {
 	const uint32_t lane = threadIdx.x & 63, nwaves = (gridDim.x * kBlock) >> 6;
 	for (uint32_t p = (blockIdx.x * kBlock + threadIdx.x) >> 6; p < npieces; p += nwaves) {
 		const uint2 pc = pieces[p];
 		const uint32_t cnt = min(pc.y + kPullPiece, in_off[pc.x + 1]) - pc.y;
 		float acc = 0.f;
-		for (uint32_t j = lane; j < cnt; j += 64) acc += contrib[in_src[pc.y + j]];
+		for (uint32_t off = 0; off < cnt; off += 64 * kPrUnroll) {
+			uint32_t h[kPrUnroll];
+#pragma unroll
+			for (uint32_t u = 0; u < kPrUnroll; ++u) {
+				const uint32_t j = off + u * 64 + lane;
+				h[u] = j < cnt ? in_src[pc.y + j] : 0xFFFFFFFFu;
+			}
+			float c[kPrUnroll];
+#pragma unroll
+			for (uint32_t u = 0; u < kPrUnroll; ++u) c[u] = h[u] != 0xFFFFFFFFu ? contrib[h[u]] : 0.f;
+#pragma unroll
+			for (uint32_t u = 0; u < kPrUnroll; ++u) acc += c[u];
+		}
 #pragma unroll
 		for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
-		if (lane == 0) atomicAdd(sum + pc.x, acc);
+		if (lane == 0) psum[p] = acc;
 	}
 }
 
+// sum[v] of a heavy vertex: its pieces' partial sums in piece order (pieces
+// are sorted by vertex, then by first in-edge)
+__global__ __launch_bounds__(kBlock) void pr_piece_sum_kernel(uint32_t npieces, const uint2 *__restrict__ pieces,
+                                                              const float *__restrict__ psum,
+                                                              float *__restrict__ sum)
+{
+	const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
+	if (p >= npieces) return;
+	const uint32_t v = pieces[p].x;
+	if (p > 0 && pieces[p - 1].x == v) return;  // not the vertex's first piece
+	float s = 0.f;
+	for (uint32_t q = p; q < npieces && pieces[q].x == v; ++q) s += psum[q];
+	sum[v] = s;
+}
+
+// rank = (1-d)/N + d*sum (page_rank.cpp:428-446), and the next iteration's
+// contributions rank/deg in the same pass
 __global__ void pr_apply_kernel(uint32_t n, float first, float damping, const float *__restrict__ sum,
-                                float *__restrict__ rank)
+                                const uint32_t *__restrict__ deg, float *__restrict__ rank,
+                                float *__restrict__ contrib)
 {
 	uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
-	if (v < n) rank[v] = first + damping * sum[v];
+	if (v < n) {
+		const float r = first + damping * sum[v];
+		rank[v] = r;
+		const uint32_t d = deg[v];
+		contrib[v] = d ? r / (float)d : 0.f;
+	}
 }
 
 int run_pagerank(gphi_graph *gr, uint32_t iters, float damping, float init, float *rank, float *ms)
@@ -1807,10 +2037,13 @@ int run_pagerank(gphi_graph *gr, uint32_t iters, float damping, float init, floa
 	int rc;
 	if ((rc = build_in_edges(gr))) return rc;
 	if (!gr->rank) {
-		if ((rc = dmalloc(ctx, (void **)&gr->rank, (size_t)G.n_pad * 4, nullptr))) return rc;
-		if ((rc = dmalloc(ctx, (void **)&gr->sum, (size_t)G.n_pad * 4, nullptr))) return rc;
-		if ((rc = dmalloc(ctx, (void **)&gr->contrib, (size_t)G.n_pad * 4, nullptr))) return rc;
+		// rank, sum, contrib (n_pad each), then the partial sums of the pieces
+		const size_t bytes = 3 * (size_t)G.n_pad * 4 + ((size_t)gr->npieces + 1) * 4;
+		if ((rc = dmalloc(ctx, (void **)&gr->rank, bytes, nullptr))) return rc;
+		gr->sum = gr->rank + G.n_pad;
+		gr->contrib = gr->sum + G.n_pad;
 	}
+	float *psum = gr->contrib + G.n_pad;
 	uint32_t bits;
 	memcpy(&bits, &init, 4);
 	fill_u32_kernel<<<grid_for(G.n, kBlock, 4096), kBlock, 0, s>>>((uint32_t *)gr->rank, G.n, bits);
@@ -1818,14 +2051,17 @@ int run_pagerank(gphi_graph *gr, uint32_t iters, float damping, float init, floa
 	unsigned vgrid = grid_for(G.n, kBlock, 1u << 30);
 	const unsigned lgrid = (unsigned)(((uint64_t)G.n * kPullGroup + kBlock - 1) / kBlock);
 	const unsigned hgrid = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>((gr->npieces + 3) / 4, 16384));
+	const unsigned pgrid = grid_for(gr->npieces, kBlock, 1u << 30);
 	if (ctx->ev_ok) GPHI_HIP(ctx, hipEventRecord(ctx->ev[0], s));
+	pr_prepare_kernel<<<vgrid, kBlock, 0, s>>>(G.n, gr->rank, G.deg, gr->contrib);
 	for (uint32_t it = 0; it < iters; ++it) {
-		pr_prepare_kernel<<<vgrid, kBlock, 0, s>>>(G.n, gr->rank, G.deg, gr->contrib, gr->sum);
 		pr_pull_light_kernel<<<lgrid, kBlock, 0, s>>>(G.n, gr->in_off, gr->in_src, gr->contrib, gr->sum);
-		if (gr->npieces)
+		if (gr->npieces) {
 			pr_pull_heavy_kernel<<<hgrid, kBlock, 0, s>>>(gr->npieces, gr->pieces, gr->in_off, gr->in_src,
-			                                             gr->contrib, gr->sum);
-		pr_apply_kernel<<<vgrid, kBlock, 0, s>>>(G.n, first, damping, gr->sum, gr->rank);
+			                                             gr->contrib, psum);
+			pr_piece_sum_kernel<<<pgrid, kBlock, 0, s>>>(gr->npieces, gr->pieces, psum, gr->sum);
+		}
+		pr_apply_kernel<<<vgrid, kBlock, 0, s>>>(G.n, first, damping, gr->sum, G.deg, gr->rank, gr->contrib);
 	}
 	GPHI_HIP(ctx, hipGetLastError());
 	if (ctx->ev_ok) GPHI_HIP(ctx, hipEventRecord(ctx->ev[1], s));

@@ -68,11 +68,32 @@ struct StepCtrl {
 	uint32_t changed;       // hop relaxation: edges that lowered a distance
 	uint32_t chunks;        // chunks on the live lists of the last dense sweep
 	uint32_t level;         // the step being expanded (device-driven loop)
-	uint32_t pad1;
+	uint32_t queued;        // BFS: the step expanded last built the next queue itself
 	unsigned long long reached, traversed;  // sums of |F| and of sum deg(F) over the steps
-	uint32_t pad[14];
+	uint32_t col_stale;     // BFS: column counts miss the vertices of queue-building steps
+	uint32_t pad[13];
 };
+
+// A BFS sparse step whose frontier's edges fill at most this many CSR-push
+// tiles builds the next queue itself (its claims append to it); a larger one
+// leaves the next frontier to the frontier update's pass over all vertices,
+// which then costs less than the appends' reservations on one counter.
+constexpr uint32_t kQueueStepTiles = 512;
+__host__ __device__ inline bool queue_step(uint32_t dir, uint32_t fdeg)
+{
+	return dir == 0 && ((uint64_t)fdeg + kSparseTile - 1) / kSparseTile <= kQueueStepTiles;
+}
 static_assert(sizeof(StepCtrl) == 128, "StepCtrl is 128 bytes");
+
+// The queue of a sparse step (vertices, degree prefix, first entry of each
+// kSparseTile-edge tile) in two halves, one per level parity: the step that
+// expands level L reads half L & 1, and the next frontier goes to the other.
+struct QueueBufs {
+	uint32_t *queue, *qoff, *tile_lo;  // half 0
+	uint32_t qstride, tstride;         // half 1 starts this many entries later
+};
+inline uint32_t queue_stride(uint32_t n_pad) { return n_pad; }
+inline uint32_t tile_lo_stride(uint64_t m) { return (uint32_t)(m / kSparseTile + 2); }
 
 struct DevGraph {
 	uint32_t n = 0;
@@ -131,7 +152,13 @@ struct gphi_graph {
 	uint32_t *visited = nullptr;  // BFS: bitmap of the vertices reached in earlier steps
 	uint32_t *cur = nullptr;      // frontier bitmap of the current step
 	uint32_t *row_stamp = nullptr, *col_unvis = nullptr;
+	// sparse-step queue, two halves each (QueueBufs): queue / qoff of
+	// 2 * n_pad entries, tile_lo of 2 * (m / kSparseTile + 2)
 	uint32_t *queue = nullptr, *qoff = nullptr, *tile_lo = nullptr;
+	gphi::QueueBufs queue_bufs() const
+	{
+		return gphi::QueueBufs{queue, qoff, tile_lo, gphi::queue_stride(g.n_pad), gphi::tile_lo_stride(g.m)};
+	}
 	gphi::StepCtrl *ctrl = nullptr;
 	uint4 *blk_cnt = nullptr;  // per update block: |F|, queue entries, degree sum
 	uint2 *blk_base = nullptr; // per update block: queue and degree prefix
@@ -139,6 +166,8 @@ struct gphi_graph {
 	uint32_t *live = nullptr;  // per-XCD lists of live chunks of a dense step
 	uint32_t *live_cnt = nullptr;
 	uint4 *trace = nullptr;    // per step: direction, |F|, sum deg(F), chunks swept
+	// PageRank: one allocation holding rank, sum and contrib (n_pad each) and
+	// the partial sums of the in-edge pieces
 	float *rank = nullptr, *sum = nullptr, *contrib = nullptr;
 	// multi-source BFS (batch.hip): per-vertex source masks and uint8 levels
 	uint32_t batch_words = 0;  // 64-bit words per vertex the buffers were sized for

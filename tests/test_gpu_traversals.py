@@ -124,6 +124,32 @@ def test_pagerank_c1_vs_f64(ctx, c1):
     assert l1 < 1e-6
 
 
+def test_pagerank_fixed_summation_order(ctx, c1):
+    """Every in-edge sum is added in a fixed order (no atomics), so repeated
+    runs, on the same handle and on a view with its own stream, give the same
+    FP32 bits; c1's largest in-degree (6310) spans two 4096-edge pieces."""
+    print("case: pagerank c1 repeatable", flush=True)
+    indeg = np.bincount(c1.cols, minlength=c1.n)
+    assert indeg.max() > 4096
+    gg = ctx.graph_from_arrays(c1.n, c1.deg, W=c1.W, S=c1.S, csr_edges=c1.cols, tile_heads=c1.sh,
+                               tile_tails=c1.st, tile_offsets=c1.soff)
+    import bfs_simd_benchmarks_amd as G
+    other = G.Context(0)
+    v = other.view(gg)
+    try:
+        a, _ = gg.pagerank(20, 0.85, 1.0 / c1.n)
+        b, _ = gg.pagerank(20, 0.85, 1.0 / c1.n)
+        c, _ = v.pagerank(20, 0.85, 1.0 / c1.n)
+    finally:
+        v.free()
+        gg.free()
+        other.close()
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    assert np.array_equal(a.view(np.uint32), c.view(np.uint32))
+    ref = O.pagerank_f64(c1.n, c1.csr_heads, c1.cols, c1.deg, 20, 0.85, 1.0 / c1.n)
+    assert float(np.abs(a.astype(np.float64) - ref).sum()) < 1e-6
+
+
 def test_bellman_ford_c1(ctx, c1):
     print("case: bellman-ford c1", flush=True)
     gg = ctx.graph_from_arrays(c1.n, c1.deg, W=c1.W, S=c1.S, tile_heads=c1.rh, tile_tails=c1.rt,
