@@ -153,6 +153,8 @@ def _declare(lib):
         "gphi_server_wait": (i32, [_P, C.c_uint64, C.POINTER(QueryInfo)]),
         "gphi_server_stats": (i32, [_P, u32, _P, _P, _P]),
         "gphi_server_destroy": (None, [_P]),
+        "gphi_debug_fail_alloc": (None, [i32]),
+        "gphi_bfs_noalloc": (i32, [_P, u32, _P, _P, C.POINTER(Stats)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

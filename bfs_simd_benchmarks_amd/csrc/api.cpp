@@ -84,6 +84,7 @@ int gphi_open(const gphi_opts *opts, gphi_ctx **out)
 		return rc;
 	}
 	memset(ctx->h_ctrl, 0, sizeof(gphi::StepCtrl));
+	gphi::noalloc_setup(ctx->device);  // device address of gphi_bfs_noalloc's scratch array
 	ctx->ev_ok = true;
 	for (auto &ev : ctx->ev)
 		if (hipEventCreate(&ev) != hipSuccess) ctx->ev_ok = false;
@@ -261,6 +262,12 @@ int gphi_bfs(gphi_graph *g, uint32_t src, int32_t *levels, int32_t *parents, gph
 {
 	GPHI_ENTER(g);
 	return gphi::run_bfs(g, src, levels, parents, st);
+}
+
+int gphi_bfs_noalloc(gphi_graph *g, uint32_t src, int32_t *levels_dev, int32_t *parents_dev, gphi_stats *st)
+{
+	GPHI_ENTER(g);
+	return gphi::run_bfs_noalloc(g, src, levels_dev, parents_dev, st);
 }
 
 int gphi_bfs_batch(gphi_graph *g, const uint32_t *srcs, uint32_t k, uint8_t *levels, gphi_stats *st)

@@ -197,6 +197,8 @@ void free_traversal_state(gphi_graph *gr);
 int retile_graph(gphi_graph *gr, uint32_t W, uint32_t S, int tile_order);
 int build_in_edges(gphi_graph *gr);
 int run_bfs(gphi_graph *gr, uint32_t src, int32_t *levels, int32_t *parents, gphi_stats *st);
+int run_bfs_noalloc(gphi_graph *gr, uint32_t src, int32_t *levels_dev, int32_t *parents_dev, gphi_stats *st);
+void noalloc_setup(int device);  // with `device` current, once per device (gphi_open)
 int run_sssp(gphi_graph *gr, uint32_t src, uint32_t *dist, gphi_stats *st);
 int run_pagerank(gphi_graph *gr, uint32_t iters, float damping, float init, float *rank,
                  float *ms);

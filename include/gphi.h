@@ -118,6 +118,20 @@ void gphi_graph_free(gphi_graph *g);
  * levels[n]: hop level (-1 unreached, the h_cost array); parents[n]: BFS-tree
  * parent (-1 unreached, parent[src]=src).  Either output may be NULL. */
 int gphi_bfs(gphi_graph *g, uint32_t src, int32_t *levels, int32_t *parents, gphi_stats *st);
+/* Hybrid BFS that allocates nothing.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * traversal and results as gphi_bfs, but the traversal state lives in one
+ * fixed 128 MiB device array per GPU (held from gphi_open until the process
+ * ends) instead of buffers the library allocates, and the results stay on the
+ * device: levels_dev / parents_dev are device pointers of n int32 each (-1
+ * unreached, parent[src]=src), either may be NULL.  A graph whose state does
+ * not fit (about 25 bytes per vertex plus 8 per 1024 edges: N up to roughly
+ * 5 million) gets GPHI_E_RANGE.  The calls on one GPU share its array, so
+ * they are serialised by a process-wide lock (calls that overlapped would
+ * race on it); each returns after its kernels are done. */
+int gphi_bfs_noalloc(gphi_graph *g, uint32_t src, int32_t *levels_dev, int32_t *parents_dev,
+                     gphi_stats *st);
 /* Weighted hybrid SSSP: sssp_weighted, apps/sssp_simd/sssp.cpp:969-1112.
  * dist[n]: shortest distance, 2147483647 when unreached. */
 int gphi_sssp(gphi_graph *g, uint32_t src, uint32_t *dist, gphi_stats *st);
