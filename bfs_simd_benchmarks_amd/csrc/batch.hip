@@ -158,10 +158,11 @@ __device__ __forceinline__ bool pull_runs(const StepCtrl *ctrl, uint32_t n)
 
 // Pull for vertices with 1..kPullLight in-edges.  Each lane first reads its
 // own vertex (in-edge range, want = sources not yet seen there whose frontier
-// is non-empty); the wavefront then serves its vertices that want something,
+// is non-empty).  A vertex with at most kLightSlots in-edges is served by its
+// own lane; the wavefront then serves its other vertices that want something,
 // kLightGroup at a time, with kLightGroup lanes per vertex taking every
 // kLightGroup-th in-edge (at most kLightSlots each) and an OR across the
-// group.  next[v] is zero on entry and written only by v's group; the
+// group.  next[v] is zero on entry and written only by v's lane or group; the
 // wavefront's 64 vertices are two whole touched-bitmap words, which it
 // writes with plain stores (the heavy pull, a later launch, ORs into them).
 constexpr uint32_t kLightGroup = 8;
@@ -181,7 +182,10 @@ __global__ __launch_bounds__(kBlock) void batch_pull_light_kernel(
 {
 	if (!pull_runs<kFilter>(ctrl, n)) return;
 	const uint32_t lane = threadIdx.x & 63, g = lane % kLightGroup, grp = lane / kLightGroup;
-	const uint32_t v0 = blockIdx.x * kBlock + (threadIdx.x & ~63u);  // first vertex of the wavefront
+	// blocks walk the vertices kBlock at a time: a resident-size grid, so a
+	// launch on a step it has nothing to do in returns at once
+	for (uint32_t vb = blockIdx.x * kBlock; vb < n; vb += gridDim.x * kBlock) {
+	const uint32_t v0 = vb + (threadIdx.x & ~63u);  // first vertex of the wavefront
 	const uint32_t v = v0 + lane;
 	uint32_t e0 = 0, cnt = 0;
 	u64 want[NW], any = 0;
@@ -195,8 +199,31 @@ __global__ __launch_bounds__(kBlock) void batch_pull_light_kernel(
 		want[i] = cnt ? ~seen[(uint64_t)v * NW + i] & live[i] : 0;
 		any |= want[i];
 	}
-	u64 act = uniform64(__ballot(cnt != 0 && any != 0));
-	u64 got = 0;  // group leaders: the wavefront's vertices (bit = lane) that got a source
+	u64 got = 0;  // the wavefront's vertices (bit = lane) that got a source
+	// a vertex with at most kLightSlots in-edges (most of them) is served by
+	// its own lane in one pass: all its heads loaded, then all gathers
+	const bool small = cnt != 0 && cnt <= kLightSlots && any != 0;
+	if (__ballot(small)) {  // uniform
+		uint32_t h[kLightSlots];
+#pragma unroll
+		for (uint32_t j = 0; j < kLightSlots; ++j) h[j] = small && j < cnt ? in_src[e0 + j] : 0xFFFFFFFFu;
+		u64 acc[NW];
+#pragma unroll
+		for (int i = 0; i < NW; ++i) acc[i] = 0;
+		gather_fronts<NW, kFilter, kLightSlots>(h, cur, front, acc);
+		bool hit = false;
+#pragma unroll
+		for (int i = 0; i < NW; ++i) {
+			const u64 x = acc[i] & want[i];
+			if (small && x) {
+				next[(uint64_t)v * NW + i] = x;
+				hit = true;
+			}
+		}
+		if (hit) got = 1ull << lane;
+	}
+	// the others, kLightGroup lanes per vertex
+	u64 act = uniform64(__ballot(cnt > kLightSlots && any != 0));
 	while (act) {
 		// the next kLightGroup wanting lanes of the wavefront, one per group
 		uint32_t pick[kLightGroup];
@@ -251,6 +278,7 @@ __global__ __launch_bounds__(kBlock) void batch_pull_light_kernel(
 		if ((uint32_t)got) touched[w0] = (uint32_t)got;
 		if ((uint32_t)(got >> 32)) touched[w0 + 1] = (uint32_t)(got >> 32);
 	}
+	}
 }
 
 // Pull for vertices with more than kPullLight in-edges: a wavefront per piece
@@ -286,12 +314,21 @@ __global__ __launch_bounds__(kBlock) void batch_pull_heavy_kernel(
 		}
 		if (!any) continue;
 		const uint32_t cnt = e1 - pc.y;  // 1..kPullPiece
+		// the heads of round r+1 are loaded before the fronts of round r are
+		// gathered, so a round waits on one load, not two in sequence
+		uint32_t hn[kUnroll];
+#pragma unroll
+		for (uint32_t u = 0; u < kUnroll; ++u) {
+			const uint32_t j = u * 64 + lane;
+			hn[u] = j < cnt ? in_src[pc.y + j] : 0xFFFFFFFFu;
+		}
 		for (uint32_t off = 0; off < cnt; off += 64 * kUnroll) {
 			uint32_t h[kUnroll];
 #pragma unroll
 			for (uint32_t u = 0; u < kUnroll; ++u) {
-				const uint32_t j = off + u * 64 + lane;
-				h[u] = j < cnt ? in_src[pc.y + j] : 0xFFFFFFFFu;
+				h[u] = hn[u];
+				const uint32_t j = off + 64 * kUnroll + u * 64 + lane;
+				hn[u] = j < cnt ? in_src[pc.y + j] : 0xFFFFFFFFu;
 			}
 			gather_fronts<NW, kFilter, kUnroll>(h, cur, front, acc);
 			u64 miss = 0;
@@ -391,21 +428,25 @@ __global__ __launch_bounds__(kBlock) void batch_update_kernel(
 	__shared__ u64 s_or[kBlock / 64][NW];
 	__shared__ u64 s_cnt[kBlock / 64][2];
 	const uint32_t lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-	const uint32_t v = blockIdx.x * kBlock + threadIdx.x;
-	const bool in = v < n;
-	const uint32_t tw = in ? touched[v >> 5] : 0u;
-	const uint32_t cw = in ? cur[v >> 5] : 0u;
-	const bool t = (tw >> (v & 31)) & 1u, c = (cw >> (v & 31)) & 1u;
-	u64 nw[NW], wor[NW];
+	u64 wor[NW];
 	u64 pairs = 0, pdeg = 0;
 	bool over = false;
 #pragma unroll
-	for (int j = 0; j < NW; ++j) nw[j] = wor[j] = 0;
-	if (__ballot(t || c)) {  // uniform
+	for (int j = 0; j < NW; ++j) wor[j] = 0;
+	// blocks walk the vertices kBlock at a time (any grid size works)
+	for (uint32_t vb = blockIdx.x * kBlock; vb < n; vb += gridDim.x * kBlock) {
+		const uint32_t v = vb + threadIdx.x;
+		const bool in = v < n;
+		const uint32_t tw = in ? touched[v >> 5] : 0u;
+		const uint32_t cw = in ? cur[v >> 5] : 0u;
+		const bool t = (tw >> (v & 31)) & 1u, c = (cw >> (v & 31)) & 1u;
+		if (!__ballot(t || c)) continue;  // uniform
 		if ((v & 31) == 0 && tw) touched[v >> 5] = 0u;  // clear for the next step
+		u64 nw[NW];
 		u64 anyn = 0;
 #pragma unroll
 		for (int j = 0; j < NW; ++j) {
+			nw[j] = 0;
 			const uint64_t idx = (uint64_t)v * NW + j;
 			if (t) {
 				const u64 nx = next[idx];
@@ -424,13 +465,13 @@ __global__ __launch_bounds__(kBlock) void batch_update_kernel(
 			uint32_t pc = 0;
 #pragma unroll
 			for (int j = 0; j < NW; ++j) pc += (uint32_t)__popcll(nw[j]);
-			pairs = pc;
-			pdeg = (u64)pc * deg[v];
+			pairs += pc;
+			pdeg += (u64)pc * deg[v];
 		}
 #pragma unroll
 		for (int j = 0; j < NW; ++j) {
 			u64 u = uniform64(wave_or(nw[j]));
-			wor[j] = u;
+			wor[j] |= u;
 			while (u) {
 				const uint32_t b = (uint32_t)__ffsll((long long)u) - 1;
 				u &= u - 1;
@@ -440,9 +481,9 @@ __global__ __launch_bounds__(kBlock) void batch_update_kernel(
 				if ((nw[j] >> b) & 1) level[(uint64_t)s * n_pitch + v] = (uint8_t)rel;
 			}
 		}
-		pairs = wave_add(pairs);
-		pdeg = wave_add(pdeg);
 	}
+	pairs = wave_add(pairs);
+	pdeg = wave_add(pdeg);
 	if (lane == 0) {
 #pragma unroll
 		for (int j = 0; j < NW; ++j) s_or[wid][j] = wor[j];
@@ -536,19 +577,23 @@ __global__ __launch_bounds__(kBlock) void batch_clear_kernel(uint64_t mask_words
                                                              u64 *front, u64 *seen, u64 *next, uint8_t *flag,
                                                              uint32_t *cur, uint32_t *touched, u64 *live_all,
                                                              uint32_t live_words, uint32_t *slot_start,
-                                                             u64 *sums, StepCtrl *ctrl)
+                                                             u64 *sums, StepCtrl *ctrl, bool clean)
 {
 	const uint64_t stride = (uint64_t)gridDim.x * kBlock;
 	const uint64_t i0 = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+	// clean: the last traversal on these buffers ran to its end, which leaves
+	// front, next and the touched bitmap all zero (only seen keeps its bits)
 	for (uint64_t i = i0; i < mask_words; i += stride) {
-		front[i] = 0;
 		seen[i] = 0;
-		next[i] = 0;
+		if (!clean) {
+			front[i] = 0;
+			next[i] = 0;
+		}
 	}
 	for (uint64_t i = i0; i < n_pad / 16; i += stride) reinterpret_cast<uint4 *>(flag)[i] = make_uint4(0, 0, 0, 0);
 	for (uint64_t i = i0; i < n_words; i += stride) {
 		cur[i] = 0;
-		touched[i] = 0;
+		if (!clean) touched[i] = 0;
 	}
 	for (uint64_t i = i0; i < live_words; i += stride) live_all[i] = 0;
 	for (uint64_t i = i0; i < kBatchMaxSources; i += stride) slot_start[i] = 0;
@@ -592,6 +637,7 @@ float ev_ms(hipEvent_t a, hipEvent_t b)
 }
 
 constexpr unsigned kBatchSparseGrid = 2048;
+constexpr unsigned kBatchWalkGrid = 8192;  // blocks of the kernels that walk every vertex
 constexpr uint32_t kBatchLevelGroup = 8;  // levels enqueued per read-back of the control block
 
 u64 *live_words(gphi_graph *gr) { return gr->b_live; }
@@ -609,7 +655,7 @@ void enqueue_expand(gphi_graph *gr, uint32_t next_level)
 	hipStream_t s = gr->ctx->stream;
 	batch_sparse_kernel<NW><<<kBatchSparseGrid, kBlock, 0, s>>>(gr->ctrl, gr->queue_bufs(), G.row_off, G.cols,
 	                                                           gr->b_front, gr->b_seen, gr->b_next, gr->b_touch);
-	const unsigned lg = grid_n(G.n);
+	const unsigned lg = std::min<unsigned>(grid_n(G.n), kBatchWalkGrid);
 	const u64 *live = live_words(gr);
 	batch_pull_light_kernel<NW, true><<<lg, kBlock, 0, s>>>(G.n, gr->ctrl, gr->in_off, gr->in_src, gr->cur, gr->b_front,
 	                                                       gr->b_seen, live, gr->b_next, gr->b_touch);
@@ -631,6 +677,9 @@ void enqueue_update(gphi_graph *gr, uint32_t next_level)
 {
 	const DevGraph &G = gr->g;
 	hipStream_t s = gr->ctx->stream;
+	// a block per kBlock vertices: the dense steps' updates touch most
+	// vertices, and independent blocks hide their latency better than a
+	// resident grid walking the vertices in turn
 	batch_update_kernel<NW><<<grid_n(G.n), kBlock, 0, s>>>(
 	    G.n, G.n_pad, next_level, gr->ctrl, gr->cur, gr->b_touch, gr->b_front, gr->b_seen, gr->b_next, gr->b_level,
 	    gr->flag, slot_starts(gr), G.deg, live_slots(gr), gr->b_sums);
@@ -706,6 +755,7 @@ int batch_prepare(gphi_graph *gr, uint32_t nw)
 		return set_error(ctx, GPHI_E_OOM, "batched BFS buffers (%zu MB): %s",
 		                 (3 * mb + (size_t)nw * 64 * G.n_pad) >> 20, hipGetErrorString(e));
 	gr->batch_words = nw;
+	gr->batch_clean_words = 0;
 	return GPHI_OK;
 }
 
@@ -734,11 +784,17 @@ int BatchRun::begin(gphi_graph *g, uint32_t capacity, const uint32_t *srcs, uint
 	hipStream_t s = ctx->stream;
 	// the buffers hold gr->batch_words >= nw words per vertex; a traversal runs
 	// at its own width nw (the masks are addressed with that pitch)
-	const uint64_t mask_words = (uint64_t)G.n_pad * nw;
+	mask_words = (uint64_t)G.n_pad * nw;
 	const uint64_t items = std::max<uint64_t>({mask_words, G.n_pad / 16, G.n_words, kBatchMaxSources});
+	// the front and next masks and the touched bitmap are left all zero by a
+	// traversal that ran to its end; they need clearing only over words no
+	// such traversal has covered since the buffers were made
+	prev_clean = gr->batch_clean_words;
+	const bool clean = prev_clean >= mask_words;
+	gr->batch_clean_words = 0;  // until this traversal ends
 	batch_clear_kernel<<<(unsigned)std::min<uint64_t>(grid_n(items), 16384), kBlock, 0, s>>>(
 	    mask_words, G.n_pad, G.n_words, gr->b_front, gr->b_seen, gr->b_next, gr->flag, gr->cur, gr->b_touch,
-	    gr->b_live, (1 + kLiveSlots) * kLiveStride, slot_starts(gr), gr->b_sums, gr->ctrl);
+	    gr->b_live, (1 + kLiveSlots) * kLiveStride, slot_starts(gr), gr->b_sums, gr->ctrl, clean);
 	GPHI_HIP(ctx, hipGetLastError());
 	if ((rc = seed(srcs, k, 0))) return rc;
 	// the first frontier update: the sources form the union frontier of level 0
@@ -807,6 +863,7 @@ int BatchRun::run_to_end(gphi_stats *st)
 	}
 	levels = hc->level;
 	done = true;
+	gr->batch_clean_words = std::max(prev_clean, mask_words);
 	if (st) {
 		const uint32_t ns = std::min<uint32_t>(levels, GPHI_MAX_STEPS);
 		std::vector<uint4> tr(ns);
@@ -868,6 +925,7 @@ int BatchRun::step(const uint32_t *srcs, uint32_t k)
 	if (hc->err) return set_error(ctx, GPHI_E_RANGE, "a batched BFS deeper than 254 levels does not fit uint8 levels");
 	levels = next_level;
 	done = hc->done != 0;
+	if (done) gr->batch_clean_words = std::max(prev_clean, mask_words);
 	return GPHI_OK;
 }
 

@@ -490,6 +490,7 @@ void free_traversal_state(gphi_graph *gr)
 	}
 	gr->sum = gr->contrib = nullptr;  // inside rank's allocation
 	gr->batch_words = 0;
+	gr->batch_clean_words = 0;
 }
 
 void free_graph_device(gphi_graph *gr)

@@ -171,6 +171,9 @@ struct gphi_graph {
 	float *rank = nullptr, *sum = nullptr, *contrib = nullptr;
 	// multi-source BFS (batch.hip): per-vertex source masks and uint8 levels
 	uint32_t batch_words = 0;  // 64-bit words per vertex the buffers were sized for
+	// leading mask words that front / next (and the touched bitmap) are known
+	// to hold as zero: a traversal that runs to its end leaves them so
+	uint64_t batch_clean_words = 0;
 	unsigned long long *b_front = nullptr, *b_seen = nullptr, *b_next = nullptr;
 	uint8_t *b_level = nullptr;
 	unsigned long long *b_sums = nullptr;
@@ -233,6 +236,8 @@ struct BatchRun {
 	uint32_t levels = 0;   // levels expanded
 	bool done = false;     // the union frontier is empty
 	std::vector<uint32_t> slot_src, slot_start;  // per slot: source vertex, level it joined at
+	uint64_t mask_words = 0;                     // n_pad * nw
+	uint64_t prev_clean = 0;                     // gr->batch_clean_words when the run began
 	unsigned long long live[4] = {0, 0, 0, 0};   // sources with a non-empty frontier (after step)
 	unsigned long long pairs[2] = {0, 0};        // reached (source, vertex) pairs, their out-degrees
 	uint32_t aux[3 * kBatchMaxSources] = {};     // staging of the sources to seed
