@@ -116,8 +116,9 @@ def main() -> int:
     ap.add_argument("--serving", action="store_true",
                     help="run the serving mix instead: 8 resident RMAT graphs (scales 18-24), a "
                          "Poisson stream of BFS/SSSP queries, batched vs one-at-a-time")
-    ap.add_argument("--qps", type=float, default=3000.0, help="offered queries/s (serving)")
-    ap.add_argument("--queries", type=int, default=100_000, help="queries in the stream (serving)")
+    ap.add_argument("--qps", default="1000,4000,16000,64000",
+                    help="offered loads in queries/s, comma-separated (serving)")
+    ap.add_argument("--queries", type=int, default=20_000, help="queries in each load's stream (serving)")
     args = ap.parse_args()
     if args.serving:
         return serving(args)
@@ -399,18 +400,20 @@ def other_paths(G, ctx, g, wl, deg) -> dict:
 def serving(args) -> int:
     """Serving mix: 8 resident graphs (RMAT scales 18,19,20,21,22,22,23,24, edge
     factor 16, seeds 10-17, every other graph with uint32 weights in [1,255]);
-    a Poisson stream of `queries` queries at `qps` offered rate (seed 5): 80% BFS
-    on a uniform random graph, 20% SSSP on a uniform random weighted graph,
-    uniform random source.  The same stream runs against a server that batches
-    the BFS queries pending on a graph (continuous batching, up to 256 per
-    launch) and against one that runs every query alone (max_batch 1)."""
+    at each offered load, a Poisson stream of `queries` queries (seed 5): 80%
+    BFS on a uniform random graph, 20% SSSP on a uniform random weighted graph,
+    uniform random source.  The same streams run against a server that batches
+    the BFS queries pending on a graph (continuous batching: up to 256 sources
+    per traversal, later arrivals join it between levels) and against one that
+    runs every query alone (max_batch 1).  Past saturation the answered rate
+    stays below the offered one and latency grows with the backlog."""
     import threading
 
     import bfs_simd_benchmarks_amd as G
     from bfs_simd_benchmarks_amd.workloads import SERVING_MIX, serving_stream
 
-    arrivals, is_sssp, gsel, sources = serving_stream(args.queries, args.qps, 5)
-    results = {}
+    loads = [float(x) for x in str(args.qps).split(",") if x]
+    results = {f"{q:g}": {} for q in loads}
     for mode, max_batch in (("batched", 256), ("one_at_a_time", 1)):
         srv = G.Server(max_batch=max_batch)
         sizes = []
@@ -421,46 +424,53 @@ def serving(args) -> int:
             sizes.append((g.n, g.m))
             srv.add_graph(g)
         build_s = time.perf_counter() - t0
-        srcs = sources([n for n, _ in sizes])
-        tickets = [0] * args.queries
-        start = time.perf_counter()
+        for qps in loads:
+            arrivals, is_sssp, gsel, sources = serving_stream(args.queries, qps, 5)
+            srcs = sources([n for n, _ in sizes])
+            launches0 = sum(srv.stats(i)["launches"] for i in range(len(SERVING_MIX)))
+            tickets = [0] * args.queries
+            start = time.perf_counter()
 
-        def producer():
+            def producer():
+                for q in range(args.queries):
+                    dt = arrivals[q] - (time.perf_counter() - start)
+                    if dt > 0:
+                        time.sleep(dt)
+                    tickets[q] = srv.submit(int(gsel[q]), G.QUERY_SSSP if is_sssp[q] else G.QUERY_BFS,
+                                            int(srcs[q]))
+            th = threading.Thread(target=producer)
+            th.start()
+            lat, batch, joined = np.empty(args.queries), np.empty(args.queries), np.empty(args.queries)
             for q in range(args.queries):
-                dt = arrivals[q] - (time.perf_counter() - start)
-                if dt > 0:
-                    time.sleep(dt)
-                tickets[q] = srv.submit(int(gsel[q]), G.QUERY_SSSP if is_sssp[q] else G.QUERY_BFS,
-                                        int(srcs[q]))
-        th = threading.Thread(target=producer)
-        th.start()
-        lat, batch = np.empty(args.queries), np.empty(args.queries)
-        for q in range(args.queries):
-            while tickets[q] == 0:          # not submitted yet
-                time.sleep(0.0005)
-            info = srv.wait(tickets[q])
-            lat[q], batch[q] = info.latency_us, info.batch
-            if (q + 1) % 10000 == 0:
-                print(f"[serving] {mode}: {q + 1} answered after {time.perf_counter() - start:.1f} s",
-                      file=sys.stderr, flush=True)
-        th.join()
-        wall = time.perf_counter() - start
-        bfs_lat = lat[~is_sssp]
-        results[mode] = {
-            "queries_per_s": round(args.queries / wall, 1),
-            "latency_ms_p50": round(float(np.percentile(lat, 50)) / 1e3, 3),
-            "latency_ms_p99": round(float(np.percentile(lat, 99)) / 1e3, 3),
-            "bfs_latency_ms_p99": round(float(np.percentile(bfs_lat, 99)) / 1e3, 3),
-            "mean_bfs_batch": round(float(batch[~is_sssp].mean()), 2),
-            "launches": int(sum(srv.stats(i)["launches"] for i in range(len(SERVING_MIX)))),
-            "graph_build_s": round(build_s, 2)}
-        print(f"[serving] {mode}: {results[mode]}", file=sys.stderr, flush=True)
+                while tickets[q] == 0:          # not submitted yet
+                    time.sleep(0.0005)
+                info = srv.wait(tickets[q])
+                lat[q], batch[q], joined[q] = info.latency_us, info.batch, info.joined_level
+                if (q + 1) % 10000 == 0:
+                    print(f"[serving] {mode} @ {qps:g} q/s: {q + 1} answered after "
+                          f"{time.perf_counter() - start:.1f} s", file=sys.stderr, flush=True)
+            th.join()
+            wall = time.perf_counter() - start
+            bfs = ~is_sssp
+            launches = sum(srv.stats(i)["launches"] for i in range(len(SERVING_MIX))) - launches0
+            r = {"answered_qps": round(args.queries / wall, 1),
+                 "latency_ms_p50": round(float(np.percentile(lat, 50)) / 1e3, 3),
+                 "latency_ms_p99": round(float(np.percentile(lat, 99)) / 1e3, 3),
+                 "bfs_latency_ms_p50": round(float(np.percentile(lat[bfs], 50)) / 1e3, 3),
+                 "bfs_latency_ms_p99": round(float(np.percentile(lat[bfs], 99)) / 1e3, 3),
+                 "sssp_latency_ms_p99": round(float(np.percentile(lat[~bfs], 99)) / 1e3, 3),
+                 "mean_bfs_batch": round(float(batch[bfs].mean()), 2),
+                 "bfs_joined_mid_traversal": round(float((joined[bfs] > 0).mean()), 3),
+                 "launches": int(launches), "graph_build_s": round(build_s, 2)}
+            results[f"{qps:g}"][mode] = r
+            print(f"[serving] {mode} @ {qps:g} q/s: {r}", file=sys.stderr, flush=True)
         srv.close()
-    out = {"metric": "serving mix: queries/s and p50/p99 latency, batched vs one-at-a-time",
-           "offered_qps": args.qps, "queries": args.queries, "bfs_share": 0.8,
+    out = {"metric": "serving mix: answered queries/s and p50/p99 latency per offered load, "
+                     "batched vs one-at-a-time",
+           "queries_per_load": args.queries, "bfs_share": 0.8,
            "graphs": [f"{w.name} n={n} m={m}" + (" weighted" if w.weight_seed else "")
                       for w, (n, m) in zip(SERVING_MIX, sizes)],
-           **results}
+           "loads": results}
     print(json.dumps(out))
     return 0
 
