@@ -497,11 +497,14 @@ void free_graph_device(gphi_graph *gr)
 {
 	DevGraph &G = gr->g;
 	free_traversal_state(gr);
-	void *in[] = {gr->in_off, gr->in_src, gr->pieces};
+	void *in[] = {gr->in_off, gr->in_src, gr->pieces, gr->pr_mem};
 	for (void *p : in)
 		if (p) (void)hipFree(p);
 	gr->in_off = gr->in_src = nullptr;
 	gr->pieces = nullptr;
+	gr->pr_mem = nullptr;
+	gr->pr_src = nullptr;
+	gr->pr_bands.clear();
 	if (gr->base) return;  // a view: the graph arrays are its base's
 	void *ps[] = {G.row_off, G.cols, G.csr_w, G.deg, G.heads, G.tails, G.coo_w, G.coo_w8, G.meta, G.pack, G.heads16, G.tails16};
 	for (void *p : ps)
@@ -1666,14 +1669,14 @@ __global__ __launch_bounds__(kBlock) void frontier_emit_kernel(
 // ---------------------------------------------------------------------------
 constexpr unsigned kBfsDenseGrid = 2048;  // BFS/SSSP sweeps: 8 blocks per CU, every block resident at once
 
-// blocks of kBlock threads of a kernel that fit the device at once, rounded
+// blocks of `block` threads of a kernel that fit the device at once, rounded
 // down to a multiple of the XCD count (kBfsDenseGrid if the query fails)
-static unsigned resident_grid(const void *kernel)
+static unsigned resident_grid(const void *kernel, unsigned block = kBlock)
 {
 	int dev = 0, ncu = 0, per = 0;
 	if (hipGetDevice(&dev) != hipSuccess ||
 	    hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-	    hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, kBlock, 0) != hipSuccess || ncu <= 0 || per <= 0)
+	    hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, block, 0) != hipSuccess || ncu <= 0 || per <= 0)
 		return kBfsDenseGrid;
 	return std::max<unsigned>((unsigned)(ncu * per) / kNumXcd * kNumXcd, kNumXcd);
 }
@@ -1915,49 +1918,78 @@ int run_sssp(gphi_graph *gr, uint32_t src, uint32_t *dist, gphi_stats *st)
 // PageRank (FP32): contrib = rank/deg; sum[t] += contrib[h] over the tiles;
 // rank = (1-d)/N + d*sum   (apps/pageRank_simd/page_rank.cpp:128-199,428-446)
 // ---------------------------------------------------------------------------
-__global__ void pr_prepare_kernel(uint32_t n, const float *__restrict__ rank,
-                                  const uint32_t *__restrict__ deg, float *__restrict__ contrib)
+__global__ void pr_prepare_kernel(uint32_t n, const float *__restrict__ rank, const uint32_t *__restrict__ deg,
+                                  float *__restrict__ contrib, float *__restrict__ sum)
 {
 	uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
 	if (v < n) {
 		uint32_t d = deg[v];
 		contrib[v] = d ? rank[v] / (float)d : 0.f;
+		sum[v] = 0.f;
 	}
 }
 
 // PageRank sum (apps/pageRank_simd/page_rank.cpp:128-199 gathers
-// rank[h]/deg[h] into sum[t] over the tiles) as a pull over the in-edges:
-// sum[t] = sum of contrib[h] over t's in-neighbours, in a fixed order, so an
-// iteration gives the same FP32 bits on every run.  Vertices with up to
-// kPullLight in-edges get kPullGroup lanes each: lane g loads in-edges g,
-// g + kPullGroup, ... (all of them before any is used), adds their
-// contributions in that order, and the group adds its lanes in a fixed
-// butterfly.  The others get a wavefront per piece of up to kPullPiece
-// in-edges, rounds of kPrUnroll * 64 edges with the same scheme; each piece
-// writes its partial sum, and pr_piece_sum_kernel adds a vertex's pieces in
-// piece order.
+// rank[h]/deg[h] into sum[t] over stripes of tile rows, each tile column
+// owned by one thread) over head bands (build_pr_bands), in band order, each
+// band adding its in-edges of every tail to sum[t].  A band's gathers all
+// fall in its window of contrib, which stays in L2, instead of spreading over
+// the whole vector.  The FP32 additions run in a fixed order, so an
+// iteration gives the same bits on every run: sum[t] = ((0 + band 0) +
+// band 1) + ...; a small pair is summed by one lane in edge order; within a
+// light pair kPullGroup lanes add in-edges g, g + kPullGroup, ... in turn and
+// the group adds its lanes in a fixed butterfly; a heavy pair's pieces (a
+// wavefront each, rounds of kPrUnroll * 64 in-edges) are summed the same way,
+// a one-piece pair's straight into sum, the others' into psum, which
+// pr_multi_kernel then adds in piece order (lane j takes pieces j, j + 64,
+// ..., then the butterfly).  Every tail appears at most once per band, so
+// the adds of one band never meet on a tail.  All loads of a lane are issued
+// before any gathered value is used.
 constexpr uint32_t kPullGroup = 8;
-constexpr uint32_t kPrSlots = kPullLight / kPullGroup;  // in-edges per lane of a light vertex
+constexpr uint32_t kPrSlots = kPullLight / kPullGroup;  // in-edges per lane of a light pair
 constexpr uint32_t kPrUnroll = 8;                       // loads in flight per lane of a heavy piece
+constexpr uint32_t kPrLightPerBlock = kBlock / kPullGroup;
+constexpr uint32_t kPrPiecesPerBlock = kBlock / 64;
 
-__global__ __launch_bounds__(kBlock) void pr_pull_light_kernel(uint32_t n, const uint32_t *__restrict__ in_off,
-                                                               const uint32_t *__restrict__ in_src,
-                                                               const float *__restrict__ contrib,
-                                                               float *__restrict__ sum)
+__global__ __launch_bounds__(kBlock) void pr_small_kernel(PrBand b, const uint32_t *__restrict__ lt,
+                                                          const uint32_t *__restrict__ llo,
+                                                          const uint8_t *__restrict__ lcnt,
+                                                          const uint32_t *__restrict__ in_src,
+                                                          const float *__restrict__ contrib, float *__restrict__ sum)
 {
-	const uint64_t t = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
-	const uint32_t v = (uint32_t)(t / kPullGroup), g = (uint32_t)(t % kPullGroup);
-	uint32_t e0 = 0, cnt = 0;
-	if (v < n) {
-		e0 = in_off[v];
-		cnt = in_off[v + 1] - e0;
+	const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+	if (i >= b.nsmall) return;
+	const uint32_t t = lt[b.small0 + i], lo = llo[b.small0 + i], cnt = lcnt[b.small0 + i];
+	uint32_t h[kPrSmall];
+#pragma unroll
+	for (uint32_t j = 0; j < kPrSmall; ++j) h[j] = j < cnt ? in_src[lo + j] : 0xFFFFFFFFu;
+	float c[kPrSmall];
+#pragma unroll
+	for (uint32_t j = 0; j < kPrSmall; ++j) c[j] = h[j] != 0xFFFFFFFFu ? contrib[h[j]] : 0.f;
+	float acc = 0.f;
+#pragma unroll
+	for (uint32_t j = 0; j < kPrSmall; ++j) acc += c[j];
+	sum[t] += acc;
+}
+
+__global__ __launch_bounds__(kBlock) void pr_light_kernel(PrBand b, const uint32_t *__restrict__ lt,
+                                                          const uint32_t *__restrict__ llo,
+                                                          const uint8_t *__restrict__ lcnt,
+                                                          const uint32_t *__restrict__ in_src,
+                                                          const float *__restrict__ contrib, float *__restrict__ sum)
+{
+	const uint32_t i = blockIdx.x * kPrLightPerBlock + threadIdx.x / kPullGroup, g = threadIdx.x % kPullGroup;
+	uint32_t t = 0, lo = 0, cnt = 0;
+	if (i < b.nlight) {
+		t = lt[b.light0 + i];
+		lo = llo[b.light0 + i];
+		cnt = lcnt[b.light0 + i];
 	}
-	const bool heavy = cnt > kPullLight;
 	uint32_t h[kPrSlots];
 #pragma unroll
 	for (uint32_t j = 0; j < kPrSlots; ++j) {
 		const uint32_t k = j * kPullGroup + g;
-		h[j] = !heavy && k < cnt ? in_src[e0 + k] : 0xFFFFFFFFu;
+		h[j] = k < cnt ? in_src[lo + k] : 0xFFFFFFFFu;
 	}
 	float c[kPrSlots];
 #pragma unroll
@@ -1967,63 +1999,119 @@ __global__ __launch_bounds__(kBlock) void pr_pull_light_kernel(uint32_t n, const
 	for (uint32_t j = 0; j < kPrSlots; ++j) acc += c[j];
 #pragma unroll
 	for (int o = kPullGroup / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, kPullGroup);
-	if (g == 0 && v < n && !heavy) sum[v] = acc;  // heavy vertices: pr_piece_sum_kernel
+	if (g == 0 && cnt) sum[t] += acc;
 }
 
-__global__ __launch_bounds__(kBlock) void pr_pull_heavy_kernel(uint32_t npieces, const uint2 *__restrict__ pieces,
-                                                               const uint32_t *__restrict__ in_off,
-                                                               const uint32_t *__restrict__ in_src,
-                                                               const float *__restrict__ contrib,
-                                                               float *__restrict__ psum)
+__device__ __forceinline__ float wave_sum_f32(float x)
 {
-	const uint32_t lane = threadIdx.x & 63, nwaves = (gridDim.x * kBlock) >> 6;
-	for (uint32_t p = (blockIdx.x * kBlock + threadIdx.x) >> 6; p < npieces; p += nwaves) {
-		const uint2 pc = pieces[p];
-		const uint32_t cnt = min(pc.y + kPullPiece, in_off[pc.x + 1]) - pc.y;
-		float acc = 0.f;
-		for (uint32_t off = 0; off < cnt; off += 64 * kPrUnroll) {
-			uint32_t h[kPrUnroll];
+	// xor butterfly: lanes l and l^o add the same two values, so every lane
+	// ends with the same bits
 #pragma unroll
-			for (uint32_t u = 0; u < kPrUnroll; ++u) {
-				const uint32_t j = off + u * 64 + lane;
-				h[u] = j < cnt ? in_src[pc.y + j] : 0xFFFFFFFFu;
-			}
-			float c[kPrUnroll];
+	for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+	return x;
+}
+
+// One piece p of a heavy pair, by one wavefront.  With kHot, heads below
+// `hot` read their contributions from s_hot (LDS) instead of contrib; the
+// values and their order are the same either way.
+template <bool kHot>
+__device__ __forceinline__ void pr_piece(uint32_t p, uint32_t lane, const uint4 *__restrict__ hp,
+                                         const uint32_t *__restrict__ pc, const uint32_t *__restrict__ src,
+                                         const float *__restrict__ contrib, const float *s_hot, uint32_t hot,
+                                         float *__restrict__ sum, float *__restrict__ psum)
+{
+	const uint32_t hi = pc[p];
+	const uint4 r = hp[hi];  // tail, first edge, count, first piece
+	const uint32_t k = p - r.w, e0 = r.y + k * kPullPiece, cnt = min(kPullPiece, r.z - k * kPullPiece);
+	float acc = 0.f;
+	for (uint32_t off = 0; off < cnt; off += 64 * kPrUnroll) {
+		uint32_t hh[kPrUnroll];
 #pragma unroll
-			for (uint32_t u = 0; u < kPrUnroll; ++u) c[u] = h[u] != 0xFFFFFFFFu ? contrib[h[u]] : 0.f;
+		for (uint32_t u = 0; u < kPrUnroll; ++u) {
+			const uint32_t j = off + u * 64 + lane;
+			hh[u] = j < cnt ? src[e0 + j] : 0xFFFFFFFFu;
+		}
+		float c[kPrUnroll];
 #pragma unroll
-			for (uint32_t u = 0; u < kPrUnroll; ++u) acc += c[u];
+		for (uint32_t u = 0; u < kPrUnroll; ++u) {
+			if (kHot && hh[u] < hot)
+				c[u] = s_hot[hh[u]];
+			else
+				c[u] = hh[u] != 0xFFFFFFFFu ? contrib[hh[u]] : 0.f;
 		}
 #pragma unroll
-		for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
-		if (lane == 0) psum[p] = acc;
+		for (uint32_t u = 0; u < kPrUnroll; ++u) acc += c[u];
+	}
+	acc = wave_sum_f32(acc);
+	if (lane == 0) {
+		if (r.z <= kPullPiece)
+			sum[r.x] += acc;  // the pair's only piece
+		else
+			psum[p] = acc;
 	}
 }
 
-// sum[v] of a heavy vertex: its pieces' partial sums in piece order (pieces
-// are sorted by vertex, then by first in-edge)
-__global__ __launch_bounds__(kBlock) void pr_piece_sum_kernel(uint32_t npieces, const uint2 *__restrict__ pieces,
-                                                              const float *__restrict__ psum,
-                                                              float *__restrict__ sum)
+__global__ __launch_bounds__(kBlock) void pr_piece_kernel(PrBand b, const uint4 *__restrict__ hp,
+                                                          const uint32_t *__restrict__ pc,
+                                                          const uint32_t *__restrict__ src,
+                                                          const float *__restrict__ contrib, float *__restrict__ sum,
+                                                          float *__restrict__ psum)
 {
-	const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
-	if (p >= npieces) return;
-	const uint32_t v = pieces[p].x;
-	if (p > 0 && pieces[p - 1].x == v) return;  // not the vertex's first piece
-	float s = 0.f;
-	for (uint32_t q = p; q < npieces && pieces[q].x == v; ++q) s += psum[q];
-	sum[v] = s;
+	const uint32_t q = blockIdx.x * kPrPiecesPerBlock + (threadIdx.x >> 6);
+	if (q >= b.npieces) return;  // uniform per wavefront
+	pr_piece<false>(b.piece0 + q, threadIdx.x & 63, hp, pc, src, contrib, nullptr, 0, sum, psum);
 }
 
-// rank = (1-d)/N + d*sum (page_rank.cpp:428-446), and the next iteration's
-// contributions rank/deg in the same pass
-__global__ void pr_apply_kernel(uint32_t n, float first, float damping, const float *__restrict__ sum,
+// The pieces of the first band (heads from 0), whose lowest heads, the
+// highest out-degrees after the degree reordering, carry a large share of
+// its in-edges: every block first copies the contributions of heads below
+// kPrHot into LDS (144 KiB, one block per CU), so those gathers skip L2,
+// then walks the pieces, a wavefront each.
+constexpr uint32_t kPrHot = 36864;
+constexpr uint32_t kPrHotBlock = 1024;
+
+__global__ __launch_bounds__(kPrHotBlock) void pr_piece_hot_kernel(PrBand b, const uint4 *__restrict__ hp,
+                                                                   const uint32_t *__restrict__ pc,
+                                                                   const uint32_t *__restrict__ src,
+                                                                   const float *__restrict__ contrib,
+                                                                   float *__restrict__ sum, float *__restrict__ psum)
+{
+	__shared__ float s_hot[kPrHot];
+	const uint32_t hot = min(kPrHot, b.head_hi);  // head_lo is 0
+	for (uint32_t i = threadIdx.x; i < hot; i += kPrHotBlock) s_hot[i] = contrib[i];
+	__syncthreads();
+	constexpr uint32_t kWaves = kPrHotBlock / 64;
+	for (uint32_t q = blockIdx.x * kWaves + (threadIdx.x >> 6); q < b.npieces; q += gridDim.x * kWaves)
+		pr_piece<true>(b.piece0 + q, threadIdx.x & 63, hp, pc, src, contrib, s_hot, hot, sum, psum);
+}
+
+// a wavefront per heavy pair of more than one piece: its pieces' sums in
+// piece order
+__global__ __launch_bounds__(kBlock) void pr_multi_kernel(PrBand b, const uint32_t *__restrict__ multi,
+                                                          const uint4 *__restrict__ hp,
+                                                          const float *__restrict__ psum, float *__restrict__ sum)
+{
+	const uint32_t lane = threadIdx.x & 63;
+	const uint32_t q = blockIdx.x * kPrPiecesPerBlock + (threadIdx.x >> 6);
+	if (q >= b.nmulti) return;  // uniform per wavefront
+	const uint4 r = hp[multi[b.multi0 + q]];
+	const uint32_t np = (r.z + kPullPiece - 1) / kPullPiece;
+	float s = 0.f;
+	for (uint32_t j = lane; j < np; j += 64) s += psum[r.w + j];
+	s = wave_sum_f32(s);
+	if (lane == 0) sum[r.x] += s;
+}
+
+// rank = (1-d)/N + d*sum (page_rank.cpp:428-446), the next iteration's
+// contributions rank/deg, and sum cleared for it, in one pass
+__global__ void pr_apply_kernel(uint32_t n, float first, float damping, float *__restrict__ sum,
                                 const uint32_t *__restrict__ deg, float *__restrict__ rank,
                                 float *__restrict__ contrib)
 {
 	uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
 	if (v < n) {
 		const float r = first + damping * sum[v];
+		sum[v] = 0.f;
 		rank[v] = r;
 		const uint32_t d = deg[v];
 		contrib[v] = d ? r / (float)d : 0.f;
@@ -2037,9 +2125,10 @@ int run_pagerank(gphi_graph *gr, uint32_t iters, float damping, float init, floa
 	hipStream_t s = ctx->stream;
 	int rc;
 	if ((rc = build_in_edges(gr))) return rc;
+	if ((rc = build_pr_bands(gr))) return rc;
 	if (!gr->rank) {
 		// rank, sum, contrib (n_pad each), then the partial sums of the pieces
-		const size_t bytes = 3 * (size_t)G.n_pad * 4 + ((size_t)gr->npieces + 1) * 4;
+		const size_t bytes = 3 * (size_t)G.n_pad * 4 + ((size_t)gr->pr_npieces + 1) * 4;
 		if ((rc = dmalloc(ctx, (void **)&gr->rank, bytes, nullptr))) return rc;
 		gr->sum = gr->rank + G.n_pad;
 		gr->contrib = gr->sum + G.n_pad;
@@ -2050,17 +2139,28 @@ int run_pagerank(gphi_graph *gr, uint32_t iters, float damping, float init, floa
 	fill_u32_kernel<<<grid_for(G.n, kBlock, 4096), kBlock, 0, s>>>((uint32_t *)gr->rank, G.n, bits);
 	const float first = (1.0f - damping) / (float)G.n;
 	unsigned vgrid = grid_for(G.n, kBlock, 1u << 30);
-	const unsigned lgrid = (unsigned)(((uint64_t)G.n * kPullGroup + kBlock - 1) / kBlock);
-	const unsigned hgrid = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>((gr->npieces + 3) / 4, 16384));
-	const unsigned pgrid = grid_for(gr->npieces, kBlock, 1u << 30);
+	static const unsigned hot_grid = resident_grid((const void *)pr_piece_hot_kernel, kPrHotBlock);
 	if (ctx->ev_ok) GPHI_HIP(ctx, hipEventRecord(ctx->ev[0], s));
-	pr_prepare_kernel<<<vgrid, kBlock, 0, s>>>(G.n, gr->rank, G.deg, gr->contrib);
+	pr_prepare_kernel<<<vgrid, kBlock, 0, s>>>(G.n, gr->rank, G.deg, gr->contrib, gr->sum);
 	for (uint32_t it = 0; it < iters; ++it) {
-		pr_pull_light_kernel<<<lgrid, kBlock, 0, s>>>(G.n, gr->in_off, gr->in_src, gr->contrib, gr->sum);
-		if (gr->npieces) {
-			pr_pull_heavy_kernel<<<hgrid, kBlock, 0, s>>>(gr->npieces, gr->pieces, gr->in_off, gr->in_src,
-			                                             gr->contrib, psum);
-			pr_piece_sum_kernel<<<pgrid, kBlock, 0, s>>>(gr->npieces, gr->pieces, psum, gr->sum);
+		for (const PrBand &b : gr->pr_bands) {
+			if (b.nsmall)
+				pr_small_kernel<<<(b.nsmall + kBlock - 1) / kBlock, kBlock, 0, s>>>(
+				    b, gr->pr_lt, gr->pr_llo, gr->pr_lcnt, gr->pr_src, gr->contrib, gr->sum);
+			if (b.nlight)
+				pr_light_kernel<<<(b.nlight + kPrLightPerBlock - 1) / kPrLightPerBlock, kBlock, 0, s>>>(
+				    b, gr->pr_lt, gr->pr_llo, gr->pr_lcnt, gr->pr_src, gr->contrib, gr->sum);
+			if (b.npieces && b.head_lo == 0)
+				pr_piece_hot_kernel<<<std::min<unsigned>(hot_grid, (b.npieces + kPrHotBlock / 64 - 1) /
+				                                                      (kPrHotBlock / 64)),
+				                      kPrHotBlock, 0, s>>>(b, gr->pr_hp, gr->pr_pc, gr->pr_src, gr->contrib,
+				                                           gr->sum, psum);
+			else if (b.npieces)
+				pr_piece_kernel<<<(b.npieces + kPrPiecesPerBlock - 1) / kPrPiecesPerBlock, kBlock, 0, s>>>(
+				    b, gr->pr_hp, gr->pr_pc, gr->pr_src, gr->contrib, gr->sum, psum);
+			if (b.nmulti)
+				pr_multi_kernel<<<(b.nmulti + kPrPiecesPerBlock - 1) / kPrPiecesPerBlock, kBlock, 0, s>>>(
+				    b, gr->pr_multi, gr->pr_hp, psum, gr->sum);
 		}
 		pr_apply_kernel<<<vgrid, kBlock, 0, s>>>(G.n, first, damping, gr->sum, G.deg, gr->rank, gr->contrib);
 	}

@@ -18,6 +18,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <array>
 #include <climits>
 #include <cstring>
 #include <vector>
@@ -737,10 +738,10 @@ __global__ void piece_emit_kernel(uint32_t n, const uint32_t *__restrict__ off, 
 }
 }  // namespace
 
-// In-edge arrays for the pull sweeps (multi-source BFS, PageRank, hop
-// relaxation): in_src holds the heads of every edge grouped by tail
-// (ascending heads within a tail, as a stable sort of the CSR order gives;
-// tile order within a tail for a graph without CSR), in_off[v] the first
+// In-edge arrays for the pull sweeps (multi-source BFS, PageRank): in_src
+// holds the heads of every edge grouped by tail
+// (ascending heads within a tail: a stable sort of the CSR order by tail, or
+// of the tiles by head then tail for a graph without CSR), in_off[v] the first
 // in-edge of v, and pieces the (vertex, first edge) of every kPullPiece-edge
 // piece of a vertex with more than kPullLight in-edges.
 int build_in_edges(gphi_graph *gr)
@@ -759,14 +760,19 @@ int build_in_edges(gphi_graph *gr)
 	OOM_IF(off && src && h && kout && pc && base);
 	if (m && !G.has_csr) {
 		// only the tile arrays (the row-major files PageRank and hop relaxation
-		// read): heads grouped by tail, in tile order within a tail
+		// read): sorted by head, then stably by tail, so heads ascend within a
+		// tail whatever the order inside the tiles
 		const int eb = bits_for(n ? n - 1 : 0);
+		uint32_t *ts = sc.alloc<uint32_t>(m);
+		OOM_IF(ts);
 		size_t tb = 0;
-		GPHI_HIP(ctx, hipcub::DeviceRadixSort::SortPairs(nullptr, tb, G.tails, kout, G.heads, src, m, 0, eb, s));
+		GPHI_HIP(ctx, hipcub::DeviceRadixSort::SortPairs(nullptr, tb, G.heads, h, G.tails, ts, m, 0, eb, s));
 		void *tmp = sc.alloc<uint8_t>(tb);
 		OOM_IF(tmp);
-		GPHI_HIP(ctx, hipcub::DeviceRadixSort::SortPairs(tmp, tb, G.tails, kout, G.heads, src, m, 0, eb, s));
+		GPHI_HIP(ctx, hipcub::DeviceRadixSort::SortPairs(tmp, tb, G.heads, h, G.tails, ts, m, 0, eb, s));
+		GPHI_HIP(ctx, hipcub::DeviceRadixSort::SortPairs(tmp, tb, ts, kout, h, src, m, 0, eb, s));
 		sc.release(tmp);
+		sc.release(ts);
 	} else if (m) {
 		GPHI_HIP(ctx, hipMemsetAsync(h, 0, m * 4, s));
 		row_start_kernel<<<grid1(n), kB, 0, s>>>(n, G.row_off, G.deg, h);
@@ -807,6 +813,207 @@ int build_in_edges(gphi_graph *gr)
 	gr->pieces = pieces;
 	gr->npieces = npieces;
 	gr->in_bytes = ((uint64_t)n + 1) * 4 + m * 4 + ((uint64_t)npieces + 1) * 8;
+	return GPHI_OK;
+}
+
+namespace {
+__device__ __forceinline__ uint32_t lower_bound_u32(const uint32_t *__restrict__ a, uint32_t lo, uint32_t hi,
+                                                    uint32_t x)
+{
+	while (lo < hi) {
+		const uint32_t mid = lo + ((hi - lo) >> 1);
+		if (a[mid] < x)
+			lo = mid + 1;
+		else
+			hi = mid;
+	}
+	return lo;
+}
+
+// t's in-edges with heads in [hlo, hhi): [*lo, *lo + cnt)
+__device__ __forceinline__ uint32_t band_range(const uint32_t *__restrict__ in_off, const uint32_t *__restrict__ in_src,
+                                               uint32_t t, uint32_t hlo, uint32_t hhi, uint32_t *lo)
+{
+	const uint32_t a = in_off[t], b = in_off[t + 1];
+	*lo = lower_bound_u32(in_src, a, b, hlo);
+	return lower_bound_u32(in_src, *lo, b, hhi) - *lo;
+}
+
+// the classes of a band's pairs, each counted per tail: small, light and
+// heavy pairs, pieces, multi-piece pairs, then the edges of the small, light
+// and heavy pairs
+constexpr int kBandClasses = 8;
+__device__ __forceinline__ void band_classes(uint32_t c, uint32_t (&f)[kBandClasses])
+{
+	const uint32_t np = c > kPullLight ? (c + kPullPiece - 1) / kPullPiece : 0;
+	f[0] = c >= 1 && c <= kPrSmall;
+	f[1] = c > kPrSmall && c <= kPullLight;
+	f[2] = c > kPullLight;
+	f[3] = np;
+	f[4] = np > 1;
+	f[5] = f[0] ? c : 0;
+	f[6] = f[1] ? c : 0;
+	f[7] = f[2] ? c : 0;
+}
+
+// per tail (and a zero entry at n, so that an exclusive scan of n+1 entries
+// ends with the totals): the pair counts of each class, class k at
+// f + k * (n + 1)
+__global__ void band_count_kernel(uint32_t n, const uint32_t *__restrict__ in_off,
+                                  const uint32_t *__restrict__ in_src, uint32_t hlo, uint32_t hhi,
+                                  uint32_t *__restrict__ f)
+{
+	GRID_STRIDE(t, (uint64_t)n + 1)
+	{
+		uint32_t lo, c = 0, cls[kBandClasses];
+		if (t < n) c = band_range(in_off, in_src, (uint32_t)t, hlo, hhi, &lo);
+		band_classes(c, cls);
+#pragma unroll
+		for (int k = 0; k < kBandClasses; ++k) f[(uint64_t)k * (n + 1) + t] = cls[k];
+	}
+}
+
+__global__ void band_emit_kernel(uint32_t n, const uint32_t *__restrict__ in_off, const uint32_t *__restrict__ in_src,
+                                  const uint32_t *__restrict__ pos, PrBand b, uint32_t *__restrict__ src,
+                                  uint32_t *__restrict__ lt, uint32_t *__restrict__ llo, uint8_t *__restrict__ lcnt,
+                                  uint4 *__restrict__ hp, uint32_t *__restrict__ pc, uint32_t *__restrict__ multi)
+{
+	GRID_STRIDE(t, n)
+	{
+		uint32_t lo;
+		const uint32_t c = band_range(in_off, in_src, (uint32_t)t, b.head_lo, b.head_hi, &lo);
+		const uint64_t stride = (uint64_t)n + 1;
+		uint32_t e = 0;  // first edge of the pair in src
+		if (c >= 1 && c <= kPullLight) {
+			const bool small = c <= kPrSmall;
+			const uint32_t i = small ? b.small0 + pos[t] : b.light0 + pos[stride + t];
+			e = small ? b.edge_small + pos[5 * stride + t] : b.edge_light + pos[6 * stride + t];
+			lt[i] = (uint32_t)t;
+			llo[i] = e;
+			lcnt[i] = (uint8_t)c;
+		} else if (c > kPullLight) {
+			const uint32_t i = b.heavy0 + pos[2 * stride + t], p0 = b.piece0 + pos[3 * stride + t];
+			e = b.edge_heavy + pos[7 * stride + t];
+			hp[i] = make_uint4((uint32_t)t, e, c, p0);
+			const uint32_t np = (c + kPullPiece - 1) / kPullPiece;
+			for (uint32_t k = 0; k < np; ++k) pc[p0 + k] = i;
+			if (np > 1) multi[b.multi0 + pos[4 * stride + t]] = i;
+		}
+		for (uint32_t k = 0; k < c; ++k) src[e + k] = in_src[lo + k];
+	}
+}
+}  // namespace
+
+// PageRank head bands.  The sum over in-edges gathers contrib[h] at random;
+// swept band by band, every gather of a band falls in one window of
+// contributions that stays in each XCD's L2.  Bands are cut from windows of
+// kBandHeads heads: a window holding more than M/64 edges (its out-degree
+// sum) is a band of its own, lighter windows merge with the next while the
+// band stays under M/64 edges (their few gathers miss either way, and each
+// band costs a launch).  Per band, a count pass and scans place the pairs in
+// tail order, then an emit pass writes them and copies their heads into the
+// band's run of pr_src (a sweep over the tail-major in_src would fetch each
+// tail's cache lines once per band); the counts of all bands are taken first
+// so that one allocation holds every band.
+int build_pr_bands(gphi_graph *gr)
+{
+	gphi_ctx *ctx = gr->ctx;
+	DevGraph &G = gr->g;
+	hipStream_t s = ctx->stream;
+	if (gr->pr_mem || !gr->pr_bands.empty()) return GPHI_OK;
+	if (!gr->in_off) return set_error(ctx, GPHI_E_STATE, "PageRank bands need the in-edge arrays");
+	const uint32_t n = G.n;
+	const uint64_t m = G.m;
+	// band cuts from the out-degree sums of the windows
+	std::vector<uint32_t> hdeg(n);
+	if (n) GPHI_HIP(ctx, hipMemcpyAsync(hdeg.data(), G.deg, (size_t)n * 4, hipMemcpyDeviceToHost, s));
+	GPHI_HIP(ctx, hipStreamSynchronize(s));
+	const uint32_t nwin = (uint32_t)(((uint64_t)n + kBandHeads - 1) / kBandHeads);
+	std::vector<uint64_t> wsum(nwin, 0);
+	for (uint32_t v = 0; v < n; ++v) wsum[v / kBandHeads] += hdeg[v];
+	std::vector<uint32_t> cuts{0};
+	const uint64_t cap = std::max<uint64_t>(m / 64, 1);
+	for (uint32_t w = 0; w < nwin;) {
+		uint64_t e = wsum[w++];
+		while (w < nwin && e + wsum[w] <= cap) e += wsum[w++];
+		cuts.push_back((uint32_t)std::min<uint64_t>((uint64_t)w * kBandHeads, n));
+	}
+	const uint64_t stride = (uint64_t)n + 1;
+	Scratch sc{ctx, {}};
+	uint32_t *f = sc.alloc<uint32_t>(kBandClasses * stride), *pos = sc.alloc<uint32_t>(kBandClasses * stride);
+	OOM_IF(f && pos);
+	size_t tb = 0;
+	GPHI_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(nullptr, tb, f, pos, stride, s));
+	void *tmp = sc.alloc<uint8_t>(tb);
+	OOM_IF(tmp);
+	// per-tail class counts of the band [hlo, hhi) and their exclusive scans
+	auto count_band = [&](uint32_t hlo, uint32_t hhi) -> int {
+		band_count_kernel<<<grid1(stride), kB, 0, s>>>(n, gr->in_off, gr->in_src, hlo, hhi, f);
+		for (int k = 0; k < kBandClasses; ++k) {
+			size_t t2 = tb;
+			GPHI_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(tmp, t2, f + k * stride, pos + k * stride, stride, s));
+		}
+		return GPHI_OK;
+	};
+	const size_t nb = cuts.size() - 1;
+	std::vector<std::array<uint32_t, kBandClasses>> tot(nb);
+	uint64_t all[kBandClasses] = {};
+	for (size_t b = 0; b < nb; ++b) {
+		int rc = count_band(cuts[b], cuts[b + 1]);
+		if (rc) return rc;
+		for (int k = 0; k < kBandClasses; ++k)
+			GPHI_HIP(ctx, hipMemcpyAsync(&tot[b][k], pos + k * stride + n, 4, hipMemcpyDeviceToHost, s));
+		GPHI_HIP(ctx, hipStreamSynchronize(s));
+		for (int k = 0; k < kBandClasses; ++k) all[k] += tot[b][k];
+	}
+	const uint64_t npairs = all[0] + all[1];  // small pairs of every band, then light pairs
+	if (npairs + all[2] + all[3] >= (1ull << 32)) return set_error(ctx, GPHI_E_RANGE, "PageRank bands: too many pairs");
+	std::vector<PrBand> bands(nb);
+	uint64_t off[5] = {0, all[0], 0, 0, 0}, e_off = 0;
+	for (size_t b = 0; b < nb; ++b) {
+		const uint64_t el = e_off + tot[b][5], eh = el + tot[b][6];
+		bands[b] = PrBand{cuts[b],         cuts[b + 1], (uint32_t)off[0], tot[b][0], (uint32_t)off[1],
+		                  tot[b][1],       (uint32_t)off[2], tot[b][2], (uint32_t)off[3], tot[b][3],
+		                  (uint32_t)off[4], tot[b][4], (uint32_t)e_off, (uint32_t)el, (uint32_t)eh};
+		for (int k = 0; k < 5; ++k) off[k] += tot[b][k];
+		e_off = eh + tot[b][7];
+	}
+	if (e_off != m)
+		return set_error(ctx, GPHI_E_STATE, "PageRank bands cover %llu of %llu in-edges", (unsigned long long)e_off,
+		                 (unsigned long long)m);
+	// heads band by band, pair tails and firsts, heavy pairs, piece owners,
+	// multi-piece pairs, pair counts
+	auto up16 = [](uint64_t x) { return (x + 15) & ~15ull; };
+	const uint64_t o_lt = up16(m * 4), o_llo = up16(o_lt + npairs * 4), o_hp = up16(o_llo + npairs * 4),
+	               o_pc = o_hp + all[2] * 16, o_mu = up16(o_pc + all[3] * 4), o_lc = up16(o_mu + all[4] * 4),
+	               bytes = up16(o_lc + npairs);
+	uint8_t *mem = sc.alloc<uint8_t>(bytes);
+	OOM_IF(mem);
+	uint32_t *src = (uint32_t *)mem;
+	uint32_t *lt = (uint32_t *)(mem + o_lt), *llo = (uint32_t *)(mem + o_llo), *pc = (uint32_t *)(mem + o_pc);
+	uint32_t *multi = (uint32_t *)(mem + o_mu);
+	uint4 *hp = (uint4 *)(mem + o_hp);
+	uint8_t *lcnt = mem + o_lc;
+	for (const PrBand &b : bands) {
+		int rc = count_band(b.head_lo, b.head_hi);
+		if (rc) return rc;
+		band_emit_kernel<<<grid1(n), kB, 0, s>>>(n, gr->in_off, gr->in_src, pos, b, src, lt, llo, lcnt, hp, pc,
+		                                         multi);
+	}
+	GPHI_HIP(ctx, hipGetLastError());
+	GPHI_HIP(ctx, hipStreamSynchronize(s));
+	sc.keep(mem);
+	gr->pr_mem = mem;
+	gr->pr_src = src;
+	gr->pr_lt = lt;
+	gr->pr_llo = llo;
+	gr->pr_lcnt = lcnt;
+	gr->pr_hp = hp;
+	gr->pr_pc = pc;
+	gr->pr_multi = multi;
+	gr->pr_npieces = (uint32_t)all[3];
+	gr->pr_bytes = bytes;
+	gr->pr_bands = std::move(bands);
 	return GPHI_OK;
 }
 

@@ -120,6 +120,28 @@ struct DevGraph {
 	uint64_t device_bytes = 0;
 };
 
+// PageRank head bands (build_pr_bands).  The heads are cut into bands
+// [head_lo, head_hi); within a band, the in-edges of a tail whose heads lie
+// in it form a pair (tail, first edge, count).  A small pair has 1..kPrSmall
+// in-edges, a light pair up to kPullLight, a heavy pair more, split into
+// pieces of kPullPiece; the heavy pairs of more than one piece are also
+// listed apart (multi).  Each class is in tail order; indices are global
+// over the band arrays.  The heads are copied band by band (pr_src): a
+// band's small pairs' heads from edge_small on, its light pairs' from
+// edge_light, its heavy pairs' from edge_heavy, each pair's ascending, so a
+// band's sweep streams one contiguous run of the copy.
+constexpr uint32_t kBandHeads = 1u << 19;  // heads per band window: 2 MiB of FP32 contributions
+constexpr uint32_t kPrSmall = 8;           // in-edges a lane sums alone
+struct PrBand {
+	uint32_t head_lo, head_hi;
+	uint32_t small0, nsmall;
+	uint32_t light0, nlight;
+	uint32_t heavy0, nheavy;
+	uint32_t piece0, npieces;
+	uint32_t multi0, nmulti;
+	uint32_t edge_small, edge_light, edge_heavy;
+};
+
 }  // namespace gphi
 
 struct gphi_ctx {
@@ -167,8 +189,22 @@ struct gphi_graph {
 	uint32_t *live_cnt = nullptr;
 	uint4 *trace = nullptr;    // per step: direction, |F|, sum deg(F), chunks swept
 	// PageRank: one allocation holding rank, sum and contrib (n_pad each) and
-	// the partial sums of the in-edge pieces
+	// the partial sums of the heavy pairs' pieces
 	float *rank = nullptr, *sum = nullptr, *contrib = nullptr;
+	// PageRank head bands (build_pr_bands; built once, independent of W and S):
+	// the heads band by band (pr_src), small and light pairs (tail, first
+	// edge in pr_src, count), heavy pairs {tail, first edge, count, first
+	// piece}, each piece's heavy pair, and the multi-piece heavy pairs; all in
+	// one allocation, pr_mem
+	std::vector<gphi::PrBand> pr_bands;
+	void *pr_mem = nullptr;
+	uint32_t *pr_src = nullptr;
+	uint32_t *pr_lt = nullptr, *pr_llo = nullptr;
+	uint8_t *pr_lcnt = nullptr;
+	uint4 *pr_hp = nullptr;
+	uint32_t *pr_pc = nullptr, *pr_multi = nullptr;
+	uint32_t pr_npieces = 0;
+	uint64_t pr_bytes = 0;
 	// multi-source BFS (batch.hip): per-vertex source masks and uint8 levels
 	uint32_t batch_words = 0;  // 64-bit words per vertex the buffers were sized for
 	// leading mask words that front / next (and the touched bitmap) are known
@@ -199,6 +235,7 @@ void free_graph_device(gphi_graph *gr);
 void free_traversal_state(gphi_graph *gr);
 int retile_graph(gphi_graph *gr, uint32_t W, uint32_t S, int tile_order);
 int build_in_edges(gphi_graph *gr);
+int build_pr_bands(gphi_graph *gr);  // after build_in_edges
 int run_bfs(gphi_graph *gr, uint32_t src, int32_t *levels, int32_t *parents, gphi_stats *st);
 int run_bfs_noalloc(gphi_graph *gr, uint32_t src, int32_t *levels_dev, int32_t *parents_dev, gphi_stats *st);
 void noalloc_setup(int device);  // with `device` current, once per device (gphi_open)

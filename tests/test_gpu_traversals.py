@@ -150,6 +150,63 @@ def test_pagerank_fixed_summation_order(ctx, c1):
     assert float(np.abs(a.astype(np.float64) - ref).sum()) < 1e-6
 
 
+def band_graph():
+    """N = 3 * 2^19 + 123 vertices, so the PageRank sweep cuts the heads into
+    four bands of 2^19 (every window carries far more than M/64 edges).
+    Tail 0 gets an in-edge from every third vertex (about 175K per band: heavy
+    pairs of many pieces in every band); tails 1-4 get 64, 65, 4096 and 4097
+    in-edges from heads 20.. (the light/heavy and one/two-piece boundaries);
+    every vertex has 4 random out-edges to tails >= 16."""
+    n = 3 * (1 << 19) + 123
+    rng = np.random.default_rng(7)
+    hs = [np.arange(3, n, 3, dtype=np.int64)]
+    ts = [np.zeros(len(hs[0]), np.int64)]
+    for t, cnt in ((1, 64), (2, 65), (3, 4096), (4, 4097)):
+        hs.append(np.arange(20, 20 + cnt, dtype=np.int64))
+        ts.append(np.full(cnt, t, np.int64))
+    rh = np.repeat(np.arange(n, dtype=np.int64), 4)
+    rt = rng.integers(16, n, len(rh))
+    keep = rh != rt
+    hs.append(rh[keep])
+    ts.append(rt[keep])
+    h, t = np.concatenate(hs), np.concatenate(ts)
+    order = np.lexsort((t, h))
+    h, t = h[order].astype(np.uint32), t[order].astype(np.uint32)
+    return n, h, t, np.bincount(h, minlength=n).astype(np.uint32)
+
+
+def test_pagerank_head_bands(ctx):
+    """PageRank over several head bands, with light pairs, one-piece and
+    many-piece heavy pairs in each: equal to the FP64 oracle, the same bits
+    on a second run (the per-pair piece counters are left at zero), and the
+    same bits from the row-major tiles alone with the edges shuffled inside
+    every tile (the in-edges are re-sorted by head within each tail)."""
+    print("case: pagerank head bands", flush=True)
+    n, h, t, deg = band_graph()
+    W = 4096
+    gg = ctx.graph_from_arrays(n, deg, W=W, S=16, csr_edges=t)
+    try:
+        a, _ = gg.pagerank(20, 0.85, 1.0 / n)
+        b, _ = gg.pagerank(20, 0.85, 1.0 / n)
+    finally:
+        gg.free()
+    ref = O.pagerank_f64(n, h, t, deg, 20, 0.85, 1.0 / n)
+    l1 = float(np.abs(a.astype(np.float64) - ref).sum())
+    print(f"pagerank head bands L1 vs fp64 = {l1:.3e}")
+    assert l1 < 1e-6
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    th, tt, _, off = O.coo_tile(n, W, h, t)
+    tile = np.searchsorted(off.astype(np.int64), np.arange(len(th)), side="right") - 1
+    shuf = np.lexsort((np.random.default_rng(3).random(len(th)), tile))
+    gt = ctx.graph_from_arrays(n, deg, W=W, S=16, tile_heads=th[shuf], tile_tails=tt[shuf], tile_offsets=off,
+                               tile_order=1)
+    try:
+        c, _ = gt.pagerank(20, 0.85, 1.0 / n)
+    finally:
+        gt.free()
+    assert np.array_equal(a.view(np.uint32), c.view(np.uint32))
+
+
 def test_bellman_ford_c1(ctx, c1):
     print("case: bellman-ford c1", flush=True)
     gg = ctx.graph_from_arrays(c1.n, c1.deg, W=c1.W, S=c1.S, tile_heads=c1.rh, tile_tails=c1.rt,
