@@ -3,7 +3,8 @@
 
 Workload (default): RMAT scale 25, edge factor 36 (2^25 vertices, 1,207,959,552
 samples, a=0.57 b=0.19 c=0.19, seed 4), self-loops/duplicates removed,
-degree-reordered, column-stripe tiles W=4096 S=16; the graph is generated and
+degree-reordered, column-stripe tiles W=2048 S=32 (the autotuner's choice);
+the graph is generated and
 laid out in HBM by libgphi before timing.  One step = one hybrid BFS from the
 next seeded source (out-degree > 0), results left in HBM.
 

@@ -45,8 +45,9 @@ WORKLOADS = {
     "c2": Workload("pokec-shape", 21, 1_632_803, 2, 1 << 27, target_unique=30_622_564, perm_seed=2),
     # c1 graph + uint32 weights uniform in [1,255], seed 3
     "c3": Workload("rmat16-ef16-weighted", 16, 1 << 16, 1, 16 << 16, weight_seed=3),
-    # scale 25, edge factor 36 (~1.2e9 samples), seed 4
-    "c4": Workload("rmat25-ef36", 25, 1 << 25, 4, 36 << 25, W=4096, S=16),
+    # scale 25, edge factor 36 (~1.2e9 samples), seed 4; W=2048 S=32, the
+    # autotuner's choice over W in 512..4096 and S in 8, 16, 32
+    "c4": Workload("rmat25-ef36", 25, 1 << 25, 4, 36 << 25, W=2048, S=32),
 }
 
 # serving mix: scales 18..24, edge factor 16, seeds 10..17, every other graph weighted

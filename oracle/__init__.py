@@ -290,7 +290,7 @@ CONFIGS = {
     "c1": dict(scale=16, nverts=1 << 16, seed=1, samples=16 << 16, target=0, perm=False, W=1024, S=16),
     "c2": dict(scale=21, nverts=1_632_803, seed=2, samples=1 << 27, target=30_622_564, perm=True,
                W=1024, S=16),
-    "c4": dict(scale=25, nverts=1 << 25, seed=4, samples=36 << 25, target=0, perm=False, W=4096, S=16),
+    "c4": dict(scale=25, nverts=1 << 25, seed=4, samples=36 << 25, target=0, perm=False, W=2048, S=32),
 }
 
 

@@ -16,9 +16,9 @@ def test_more_than_one_gpu_is_refused():
 
 def test_traffic_comes_from_the_committed_profile():
     import bench
-    b, src, mean_ms = bench.measured_traffic("c4", 4096, 16)
+    b, src, mean_ms = bench.measured_traffic("c4", 2048, 32)
     assert src == "profiles/bfs_dense_kernel_rmat25-ef36.md"
     # FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, in bytes: a few GB per launch
     assert 1e9 < b < 2e10
-    assert mean_ms is None or 0.1 < mean_ms < 10
-    assert bench.measured_traffic("c4", 1024, 16) == (None, None, None)   # another tiling: no figure
+    assert 0.1 < mean_ms < 10
+    assert bench.measured_traffic("c4", 4096, 16) == (None, None, None)   # another tiling: no figure

@@ -6,7 +6,7 @@
   20 iterations from 1/N, within 1e-6 L1 of the FP64 oracle;
 - weighted rmat16 (weights uniform in [1,255], seed 3): SSSP from 16 seeded
   sources, distances bit-exact against Dijkstra;
-- rmat25-ef36 (33,554,432 vertices, ~1.18e9 edges, W=4096 S=16): plain BFS
+- rmat25-ef36 (33,554,432 vertices, ~1.18e9 edges, W=2048 S=32): plain BFS
   from 2 sources against the oracle's OpenMP hybrid BFS (levels, trace,
   parents); a 64-source batched launch whose rows are checked against the
   oracle for 16 sources and against the plain BFS for all 64;
