@@ -93,3 +93,29 @@ def test_batch_rejects_bad_arguments(ctx, c1):
             assert ei.value.code == code
     finally:
         gg.free()
+
+
+@pytest.mark.parametrize("k", [64, 200])
+def test_batch_hub_and_piece_boundaries(ctx, k):
+    """A graph of 1.57M vertices with one tail of 524K in-edges (128 heavy
+    pull pieces) and tails at the light/heavy and one/two-piece boundaries:
+    every checked row equals that source's own levels."""
+    import sys
+    from pathlib import Path
+    sys.path.insert(0, str(Path(__file__).resolve().parent))
+    from test_gpu_traversals import band_graph
+    print(f"case: batch band graph k={k}", flush=True)
+    n, h, t, deg = band_graph()
+    W, S = 4096, 16
+    th, tt, _, off = O.coo_tile(n, W, h, t)
+    ch, ct, _, coff = O.col_major(n, W, S, th, tt, None, off)
+    gg = ctx.graph_from_arrays(n, deg, W=W, S=S, csr_edges=t, tile_heads=ch, tile_tails=ct, tile_offsets=coff)
+    srcs = pick_sources(deg, k, 77 + k)
+    try:
+        lv, _ = gg.bfs_batch(srcs)
+    finally:
+        gg.free()
+    for i in range(0, k, max(1, k // 8)):
+        ref = O.bfs_levels(n, deg, t, int(srcs[i]))
+        want = np.where(ref >= 0, ref, 255).astype(np.uint8)
+        assert np.array_equal(lv[i], want), f"row {i} (source {srcs[i]}) differs"
