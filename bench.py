@@ -111,6 +111,9 @@ def main() -> int:
                     help="timed launches per batch size (p50/p99 over them)")
     ap.add_argument("--other-paths", type=int, default=1,
                     help="also time PageRank, hop relaxation and weighted SSSP on the same graph (0 skips)")
+    ap.add_argument("--config-legs", type=int, default=1,
+                    help="also run the other configurations briefly: pokec-shape BFS and PageRank, "
+                         "weighted rmat16 SSSP, the serving mix at two loads (0 skips)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB five times (calibrates the "
                          "FETCH_SIZE / WRITE_SIZE counters against known bytes)")
@@ -343,9 +346,11 @@ def main() -> int:
     }
     if args.other_paths:
         out["other_paths"] = other_paths(G, ctx, g, wl, deg)
-    print(json.dumps(out))
     g.free()
     ctx.close()
+    if args.config_legs:
+        out["configs"] = config_legs(G)
+    print(json.dumps(out))
     return 0
 
 
@@ -398,7 +403,51 @@ def other_paths(G, ctx, g, wl, deg) -> dict:
     return res
 
 
-def serving(args) -> int:
+def config_legs(G) -> dict:
+    """The other benchmark configurations, each briefly, on the same GPU:
+    pokec shape (64 seeded sources one at a time, then 20 PageRank
+    iterations), weighted rmat16 (SSSP from its 16 seeded sources) and the
+    serving mix at two offered loads (6,000 queries each, batched vs
+    one-at-a-time; the full load sweep is `--serving`)."""
+    from bfs_simd_benchmarks_amd.workloads import WORKLOADS, pick_sources
+    res = {}
+    with G.Context(0) as ctx:
+        g = WORKLOADS["c2"].generate(ctx)
+        try:
+            deg = g.download(csr=False, tiles=False)["deg"]
+            srcs = pick_sources(deg, 64, WORKLOADS["c2"].seed)
+            g.bfs(int(srcs[0]), parents=False, levels=False)
+            ctx.set_level_timing(False)
+            t0 = time.perf_counter()
+            edges = sum(int(g.bfs(int(s), parents=False, levels=False)[2].edges_traversed) for s in srcs)
+            wall = time.perf_counter() - t0
+            ctx.set_level_timing(True)
+            g.pagerank(1, 0.85, 1.0 / g.n)
+            _, pr_ms = g.pagerank(20, 0.85, 1.0 / g.n)
+            res["pokec_shape"] = {"vertices": g.n, "edges": g.m, "bfs_sources": len(srcs),
+                                  "bfs_ms_per_query": round(wall * 1e3 / len(srcs), 3),
+                                  "bfs_gteps": round(edges / wall / 1e9, 2),
+                                  "pagerank_iterations": 20, "pagerank_ms_per_iteration": round(pr_ms / 20, 4)}
+        finally:
+            g.free()
+        g = WORKLOADS["c3"].generate(ctx, weighted=True)
+        try:
+            deg = g.download(csr=False, tiles=False)["deg"]
+            srcs = pick_sources(deg, 16, 3)
+            g.sssp(int(srcs[0]))
+            t0 = time.perf_counter()
+            for s in srcs:
+                g.sssp(int(s))
+            wall = time.perf_counter() - t0
+            res["rmat16_weighted_sssp"] = {"sources": len(srcs), "ms_per_query": round(wall * 1e3 / len(srcs), 3)}
+        finally:
+            g.free()
+    print(f"[bench] configs: {res}", file=sys.stderr, flush=True)
+    res["serving_mix"] = serving_loads(G, [1000.0, 3000.0], 6000)
+    return res
+
+
+def serving_loads(G, loads, queries) -> dict:
     """Serving mix: 8 resident graphs (RMAT scales 18,19,20,21,22,22,23,24, edge
     factor 16, seeds 10-17, every other graph with uint32 weights in [1,255]);
     at each offered load, a Poisson stream of `queries` queries (seed 5): 80%
@@ -410,11 +459,10 @@ def serving(args) -> int:
     stays below the offered one and latency grows with the backlog."""
     import threading
 
-    import bfs_simd_benchmarks_amd as G
     from bfs_simd_benchmarks_amd.workloads import SERVING_MIX, serving_stream
 
-    loads = [float(x) for x in str(args.qps).split(",") if x]
     results = {f"{q:g}": {} for q in loads}
+    sizes = []
     for mode, max_batch in (("batched", 256), ("one_at_a_time", 1)):
         srv = G.Server(max_batch=max_batch)
         sizes = []
@@ -426,14 +474,14 @@ def serving(args) -> int:
             srv.add_graph(g)
         build_s = time.perf_counter() - t0
         for qps in loads:
-            arrivals, is_sssp, gsel, sources = serving_stream(args.queries, qps, 5)
+            arrivals, is_sssp, gsel, sources = serving_stream(queries, qps, 5)
             srcs = sources([n for n, _ in sizes])
             launches0 = sum(srv.stats(i)["launches"] for i in range(len(SERVING_MIX)))
-            tickets = [0] * args.queries
+            tickets = [0] * queries
             start = time.perf_counter()
 
             def producer():
-                for q in range(args.queries):
+                for q in range(queries):
                     dt = arrivals[q] - (time.perf_counter() - start)
                     if dt > 0:
                         time.sleep(dt)
@@ -441,8 +489,8 @@ def serving(args) -> int:
                                             int(srcs[q]))
             th = threading.Thread(target=producer)
             th.start()
-            lat, batch, joined = np.empty(args.queries), np.empty(args.queries), np.empty(args.queries)
-            for q in range(args.queries):
+            lat, batch, joined = np.empty(queries), np.empty(queries), np.empty(queries)
+            for q in range(queries):
                 while tickets[q] == 0:          # not submitted yet
                     time.sleep(0.0005)
                 info = srv.wait(tickets[q])
@@ -454,7 +502,7 @@ def serving(args) -> int:
             wall = time.perf_counter() - start
             bfs = ~is_sssp
             launches = sum(srv.stats(i)["launches"] for i in range(len(SERVING_MIX))) - launches0
-            r = {"answered_qps": round(args.queries / wall, 1),
+            r = {"answered_qps": round(queries / wall, 1),
                  "latency_ms_p50": round(float(np.percentile(lat, 50)) / 1e3, 3),
                  "latency_ms_p99": round(float(np.percentile(lat, 99)) / 1e3, 3),
                  "bfs_latency_ms_p50": round(float(np.percentile(lat[bfs], 50)) / 1e3, 3),
@@ -466,15 +514,21 @@ def serving(args) -> int:
             results[f"{qps:g}"][mode] = r
             print(f"[serving] {mode} @ {qps:g} q/s: {r}", file=sys.stderr, flush=True)
         srv.close()
+    return {"queries_per_load": queries, "bfs_share": 0.8,
+            "graphs": [f"{w.name} n={n} m={m}" + (" weighted" if w.weight_seed else "")
+                       for w, (n, m) in zip(SERVING_MIX, sizes)],
+            "loads": results}
+
+
+def serving(args) -> int:
+    """The serving mix over the offered loads of --qps (serving_loads)."""
+    import bfs_simd_benchmarks_amd as G
+    loads = [float(x) for x in str(args.qps).split(",") if x]
     out = {"metric": "serving mix: answered queries/s and p50/p99 latency per offered load, "
                      "batched vs one-at-a-time",
-           "queries_per_load": args.queries, "bfs_share": 0.8,
-           "graphs": [f"{w.name} n={n} m={m}" + (" weighted" if w.weight_seed else "")
-                      for w, (n, m) in zip(SERVING_MIX, sizes)],
-           "loads": results}
+           **serving_loads(G, loads, args.queries)}
     print(json.dumps(out))
     return 0
-
 
 if __name__ == "__main__":
     sys.exit(main())

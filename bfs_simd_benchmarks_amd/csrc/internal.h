@@ -259,6 +259,9 @@ int bfs_level_loop(gphi_graph *gr, uint32_t src, gphi_stats *st);
 uint32_t live_list_capacity(uint64_t nchunks);  // entries of one XCD's live-chunk list
 void launch_frontier_update_driven(gphi_graph *gr, uint32_t threshold);
 int run_bfs_batch(gphi_graph *gr, const uint32_t *srcs, uint32_t k, uint8_t *levels, gphi_stats *st);
+// the multi-source state of gr for nw mask words: traversal state, in-edge
+// arrays, masks, levels (idempotent)
+int batch_prepare(gphi_graph *gr, uint32_t nw);
 
 // A multi-source traversal (batch.hip).  gphi_bfs_batch runs one to its end
 // on the device; the serving loop advances one level by level, admitting new

@@ -342,6 +342,21 @@ int gphi_server_add_graph(gphi_server *S, gphi_graph *g, uint32_t *graph_id)
 		std::lock_guard<std::mutex> lk(S->mu);
 		S->hbm_used -= need;
 	};
+	// the lane's state is built at admission, not by its first query: the
+	// in-edge sort alone takes tens of milliseconds on the larger graphs, and
+	// every query queued behind the first traversal would wait for it
+	{
+		std::lock_guard<std::mutex> ctx_lk(g->ctx->mu);
+		int rc = hipSetDevice(g->ctx->device) == hipSuccess ? GPHI_OK : GPHI_E_HIP;
+		if (rc == GPHI_OK)
+			rc = S->opts.max_batch > 1 && g->g.has_csr && g->g.has_tiles
+			         ? gphi::batch_prepare(g, S->opts.max_batch <= 64 ? 1 : S->opts.max_batch <= 128 ? 2 : 4)
+			         : gphi::ensure_traversal_state(g);
+		if (rc != GPHI_OK) {
+			unreserve();
+			return rc;
+		}
+	}
 	std::vector<gphi_graph *> views;
 	std::vector<gphi_ctx *> vctxs;
 	for (uint32_t i = 0; i < nv; ++i) {
