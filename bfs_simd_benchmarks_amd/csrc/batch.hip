@@ -433,6 +433,11 @@ __global__ __launch_bounds__(kBlock) void batch_update_kernel(
 	bool over = false;
 #pragma unroll
 	for (int j = 0; j < NW; ++j) wor[j] = 0;
+	// lane l holds the level sources j*64 + l reach at this step, relative to
+	// their start levels
+	uint32_t rel_lane[NW];
+#pragma unroll
+	for (int j = 0; j < NW; ++j) rel_lane[j] = next_level - slot_start[j * 64 + lane];
 	// blocks walk the vertices kBlock at a time (any grid size works)
 	for (uint32_t vb = blockIdx.x * kBlock; vb < n; vb += gridDim.x * kBlock) {
 		const uint32_t v = vb + threadIdx.x;
@@ -476,7 +481,8 @@ __global__ __launch_bounds__(kBlock) void batch_update_kernel(
 				const uint32_t b = (uint32_t)__ffsll((long long)u) - 1;
 				u &= u - 1;
 				const uint32_t s = j * 64 + b;
-				const uint32_t rel = next_level - slot_start[s];
+				// source s's level, from lane b's register (no load in the loop)
+				const uint32_t rel = (uint32_t)__builtin_amdgcn_readlane((int)rel_lane[j], (int)b);
 				over |= rel >= kUnreached;
 				if ((nw[j] >> b) & 1) level[(uint64_t)s * n_pitch + v] = (uint8_t)rel;
 			}
