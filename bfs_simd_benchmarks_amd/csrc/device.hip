@@ -1915,8 +1915,8 @@ int run_sssp(gphi_graph *gr, uint32_t src, uint32_t *dist, gphi_stats *st)
 }
 
 // ---------------------------------------------------------------------------
-// PageRank (FP32): contrib = rank/deg; sum[t] += contrib[h] over the tiles;
-// rank = (1-d)/N + d*sum   (apps/pageRank_simd/page_rank.cpp:128-199,428-446)
+// PageRank (FP32): contrib = rank/deg; sum[t] += contrib[h] over the
+// in-edges; rank = (1-d)/N + d*sum   (apps/pageRank_simd/page_rank.cpp:128-199,428-446)
 // ---------------------------------------------------------------------------
 __global__ void pr_prepare_kernel(uint32_t n, const float *__restrict__ rank, const uint32_t *__restrict__ deg,
                                   float *__restrict__ contrib, float *__restrict__ sum)
@@ -1954,7 +1954,7 @@ constexpr uint32_t kPrPiecesPerBlock = kBlock / 64;
 __global__ __launch_bounds__(kBlock) void pr_small_kernel(PrBand b, const uint32_t *__restrict__ lt,
                                                           const uint32_t *__restrict__ llo,
                                                           const uint8_t *__restrict__ lcnt,
-                                                          const uint32_t *__restrict__ in_src,
+                                                          const uint32_t *__restrict__ src,
                                                           const float *__restrict__ contrib, float *__restrict__ sum)
 {
 	const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
@@ -1962,7 +1962,7 @@ __global__ __launch_bounds__(kBlock) void pr_small_kernel(PrBand b, const uint32
 	const uint32_t t = lt[b.small0 + i], lo = llo[b.small0 + i], cnt = lcnt[b.small0 + i];
 	uint32_t h[kPrSmall];
 #pragma unroll
-	for (uint32_t j = 0; j < kPrSmall; ++j) h[j] = j < cnt ? in_src[lo + j] : 0xFFFFFFFFu;
+	for (uint32_t j = 0; j < kPrSmall; ++j) h[j] = j < cnt ? src[lo + j] : 0xFFFFFFFFu;
 	float c[kPrSmall];
 #pragma unroll
 	for (uint32_t j = 0; j < kPrSmall; ++j) c[j] = h[j] != 0xFFFFFFFFu ? contrib[h[j]] : 0.f;
@@ -1975,7 +1975,7 @@ __global__ __launch_bounds__(kBlock) void pr_small_kernel(PrBand b, const uint32
 __global__ __launch_bounds__(kBlock) void pr_light_kernel(PrBand b, const uint32_t *__restrict__ lt,
                                                           const uint32_t *__restrict__ llo,
                                                           const uint8_t *__restrict__ lcnt,
-                                                          const uint32_t *__restrict__ in_src,
+                                                          const uint32_t *__restrict__ src,
                                                           const float *__restrict__ contrib, float *__restrict__ sum)
 {
 	const uint32_t i = blockIdx.x * kPrLightPerBlock + threadIdx.x / kPullGroup, g = threadIdx.x % kPullGroup;
@@ -1989,7 +1989,7 @@ __global__ __launch_bounds__(kBlock) void pr_light_kernel(PrBand b, const uint32
 #pragma unroll
 	for (uint32_t j = 0; j < kPrSlots; ++j) {
 		const uint32_t k = j * kPullGroup + g;
-		h[j] = k < cnt ? in_src[lo + k] : 0xFFFFFFFFu;
+		h[j] = k < cnt ? src[lo + k] : 0xFFFFFFFFu;
 	}
 	float c[kPrSlots];
 #pragma unroll
