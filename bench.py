@@ -398,6 +398,21 @@ def other_paths(G, ctx, g, wl, deg) -> dict:
         res["sssp"] = {"ms_per_query": round(ms / (len(srcs) - 1), 3),
                        "gteps": round(edges / (ms * 1e-3) / 1e9, 1), "sources": len(srcs) - 1}
         print(f"[bench] sssp {res['sssp']['ms_per_query']} ms per query", file=sys.stderr, flush=True)
+        # 16 sources in one multi-source traversal, rows checked against the
+        # single-source runs of two of them
+        bsrcs = pick_sources(deg, 16, wl.seed + 78)
+        gw.sssp_batch(bsrcs, dist=False)                      # warm-up (state)
+        t0 = time.perf_counter()
+        _, bst = gw.sssp_batch(bsrcs, dist=False)
+        bwall = time.perf_counter() - t0
+        bd, _ = gw.sssp_batch(bsrcs)
+        exact = all(np.array_equal(bd[i], gw.sssp(int(bsrcs[i]))[0]) for i in (0, 15))
+        res["sssp_batch"] = {"sources": 16, "ms_per_launch": round(bwall * 1e3, 3),
+                             "queries_per_s": round(16 / bwall, 1),
+                             "vs_single": round(16 / bwall / (1e3 / res["sssp"]["ms_per_query"]), 2),
+                             "steps": int(bst.steps), "rows_equal_single_source": bool(exact)}
+        print(f"[bench] sssp batch 16: {res['sssp_batch']}", file=sys.stderr, flush=True)
+        del bd
     finally:
         gw.free()
     return res

@@ -131,6 +131,7 @@ def _declare(lib):
         "gphi_bfs": (i32, [_P, u32, _P, _P, C.POINTER(Stats)]),
         "gphi_sssp": (i32, [_P, u32, _P, C.POINTER(Stats)]),
         "gphi_bfs_batch": (i32, [_P, _P, u32, _P, C.POINTER(Stats)]),
+        "gphi_sssp_batch": (i32, [_P, _P, u32, _P, C.POINTER(Stats)]),
         "gphi_pagerank": (i32, [_P, u32, C.c_float, C.c_float, _P, _P]),
         "gphi_bellman_ford": (i32, [_P, u32, u32, _P, _P, _P]),
         "gphi_bfs_alg_bytes": (i32, [_P, _P, C.POINTER(Stats), _P]),
@@ -427,6 +428,16 @@ class Graph:
         d = np.empty(self.n, np.uint32)
         st = Stats()
         self.ctx.check(self.ctx.lib.gphi_sssp(self.h, src, _ptr(d), C.byref(st)))
+        return d, st
+
+    def sssp_batch(self, srcs, dist: bool = True):
+        """Weighted SSSP from up to 16 sources in one traversal.  Returns
+        (dist uint32[k, n] or None, Stats); 2147483647 = unreached."""
+        srcs = _u32(srcs)
+        k = len(srcs)
+        d = np.empty((k, self.n), np.uint32) if dist else None
+        st = Stats()
+        self.ctx.check(self.ctx.lib.gphi_sssp_batch(self.h, _ptr(srcs), k, _ptr(d), C.byref(st)))
         return d, st
 
     def pagerank(self, iters: int = 1, damping: float = 0.85, init: float = 1.0):

@@ -31,7 +31,7 @@ HOST_FLAGS = ["-O2", "-fPIC", "-std=c++17", "-fopenmp", "-Wall", "-Wextra",
               "-D__HIP_PLATFORM_AMD__", f"-I{ROCM / 'include'}"]
 LINK_FLAGS = [f"-L{ROCM / 'lib'}", "-lamdhip64", "-fopenmp", "-pthread", f"-Wl,-rpath,{ROCM / 'lib'}"]
 
-HIP_SOURCES = ["device.hip", "gen.hip", "batch.hip", "noalloc.hip"]
+HIP_SOURCES = ["device.hip", "gen.hip", "batch.hip", "noalloc.hip", "sssp_batch.hip"]
 HOST_SOURCES = ["api.cpp", "formats.cpp", "server.cpp", "cache.cpp"]
 CLIS = {"bfs": "bfs_main.cpp", "sssp": "sssp_main.cpp", "page_rank": "pagerank_main.cpp",
         "bellman-ford": "bellman_ford_main.cpp"}

@@ -282,6 +282,14 @@ int gphi_sssp(gphi_graph *g, uint32_t src, uint32_t *dist, gphi_stats *st)
 	return gphi::run_sssp(g, src, dist, st);
 }
 
+int gphi_sssp_batch(gphi_graph *g, const uint32_t *srcs, uint32_t k, uint32_t *dist, gphi_stats *st)
+{
+	GPHI_ENTER(g);
+	uint32_t *rows[GPHI_SSSP_BATCH_MAX] = {};
+	for (uint32_t i = 0; dist && i < k && i < GPHI_SSSP_BATCH_MAX; ++i) rows[i] = dist + (size_t)i * g->g.n;
+	return gphi::run_sssp_batch(g, srcs, k, dist ? rows : nullptr, st);
+}
+
 int gphi_pagerank(gphi_graph *g, uint32_t iters, float damping, float init, float *rank, float *ms)
 {
 	GPHI_ENTER(g);

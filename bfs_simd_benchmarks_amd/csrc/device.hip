@@ -483,7 +483,7 @@ void free_traversal_state(gphi_graph *gr)
 	               (void **)&gr->trace,
 	               (void **)&gr->rank, (void **)&gr->b_front,
 	               (void **)&gr->b_seen, (void **)&gr->b_next, (void **)&gr->b_level, (void **)&gr->b_sums,
-	               (void **)&gr->b_live, (void **)&gr->b_touch, (void **)&gr->b_aux};
+	               (void **)&gr->b_live, (void **)&gr->b_touch, (void **)&gr->b_aux, (void **)&gr->sb_dist};
 	for (void **p : ps) {
 		if (*p) (void)hipFree(*p);
 		*p = nullptr;
@@ -497,10 +497,12 @@ void free_graph_device(gphi_graph *gr)
 {
 	DevGraph &G = gr->g;
 	free_traversal_state(gr);
-	void *in[] = {gr->in_off, gr->in_src, gr->pieces, gr->pr_mem};
+	void *in[] = {gr->in_off, gr->in_src, gr->pieces, gr->pr_mem, gr->in_w8, gr->in_w32};
 	for (void *p : in)
 		if (p) (void)hipFree(p);
 	gr->in_off = gr->in_src = nullptr;
+	gr->in_w8 = nullptr;
+	gr->in_w32 = nullptr;
 	gr->pieces = nullptr;
 	gr->pr_mem = nullptr;
 	gr->pr_src = nullptr;

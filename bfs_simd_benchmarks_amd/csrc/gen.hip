@@ -817,6 +817,55 @@ int build_in_edges(gphi_graph *gr)
 }
 
 namespace {
+__global__ void narrow_w8_kernel(uint64_t m, const uint32_t *__restrict__ w, uint8_t *__restrict__ w8)
+{
+	GRID_STRIDE(e, m) w8[e] = (uint8_t)w[e];
+}
+}  // namespace
+
+// The weights of the in-edges, in in_src's order: the same stable sort of
+// the CSR by tail that build_in_edges ran, with the CSR weights as payload
+// (equal keys keep their CSR order in both sorts, so the two payloads line
+// up).  One byte per edge when the tile weights have a 1-byte copy (every
+// weight below 256).
+int build_in_weights(gphi_graph *gr)
+{
+	gphi_ctx *ctx = gr->ctx;
+	DevGraph &G = gr->g;
+	hipStream_t s = ctx->stream;
+	if (gr->in_w8 || gr->in_w32) return GPHI_OK;
+	if (!gr->in_off) return set_error(ctx, GPHI_E_STATE, "in-edge weights need the in-edge arrays");
+	if (!G.has_csr || !G.csr_w) return set_error(ctx, GPHI_E_STATE, "in-edge weights need CSR weights");
+	const uint64_t m = G.m;
+	Scratch sc{ctx, {}};
+	uint32_t *kout = sc.alloc<uint32_t>(m), *w = sc.alloc<uint32_t>(m);
+	OOM_IF(kout && w);
+	const int eb = bits_for(G.n ? G.n - 1 : 0);
+	size_t tb = 0;
+	if (m) {
+		GPHI_HIP(ctx, hipcub::DeviceRadixSort::SortPairs(nullptr, tb, G.cols, kout, G.csr_w, w, m, 0, eb, s));
+		void *tmp = sc.alloc<uint8_t>(tb);
+		OOM_IF(tmp);
+		GPHI_HIP(ctx, hipcub::DeviceRadixSort::SortPairs(tmp, tb, G.cols, kout, G.csr_w, w, m, 0, eb, s));
+		sc.release(tmp);
+	}
+	if (G.coo_w8) {
+		uint8_t *w8 = sc.alloc<uint8_t>(m);
+		OOM_IF(w8);
+		if (m) narrow_w8_kernel<<<grid1(m), kB, 0, s>>>(m, w, w8);
+		GPHI_HIP(ctx, hipGetLastError());
+		GPHI_HIP(ctx, hipStreamSynchronize(s));
+		sc.keep(w8);
+		gr->in_w8 = w8;
+	} else {
+		GPHI_HIP(ctx, hipStreamSynchronize(s));
+		sc.keep(w);
+		gr->in_w32 = w;
+	}
+	return GPHI_OK;
+}
+
+namespace {
 __device__ __forceinline__ uint32_t lower_bound_u32(const uint32_t *__restrict__ a, uint32_t lo, uint32_t hi,
                                                     uint32_t x)
 {

@@ -130,6 +130,7 @@ struct DevGraph {
 // band's small pairs' heads from edge_small on, its light pairs' from
 // edge_light, its heavy pairs' from edge_heavy, each pair's ascending, so a
 // band's sweep streams one contiguous run of the copy.
+constexpr uint32_t kSsspBatchMax = GPHI_SSSP_BATCH_MAX;  // distances per vertex of the SSSP batch
 constexpr uint32_t kBandHeads = 1u << 19;  // heads per band window: 2 MiB of FP32 contributions
 constexpr uint32_t kPrSmall = 8;           // in-edges a lane sums alone
 struct PrBand {
@@ -216,8 +217,15 @@ struct gphi_graph {
 	unsigned long long *b_live = nullptr;  // sources with a non-empty frontier, then the per-block slots
 	uint32_t *b_touch = nullptr;           // bitmap: vertices whose next mask got a bit this step
 	uint32_t *b_aux = nullptr;             // sources to seed, their slots, per-slot start levels
+	// multi-source SSSP (sssp_batch.hip): kSsspBatchMax distances per vertex
+	// (vertex-major, n_pad rows), then the sources
+	uint32_t *sb_dist = nullptr;
 	// in-edge arrays of the pull steps (built once, independent of W and S)
 	uint32_t *in_off = nullptr, *in_src = nullptr;
+	// the weights of in_src's edges (build_in_weights): 1 byte when every
+	// weight is below 256, else 4
+	uint8_t *in_w8 = nullptr;
+	uint32_t *in_w32 = nullptr;
 	uint2 *pieces = nullptr;
 	uint32_t npieces = 0;
 	uint64_t in_bytes = 0;
@@ -236,6 +244,7 @@ void free_traversal_state(gphi_graph *gr);
 int retile_graph(gphi_graph *gr, uint32_t W, uint32_t S, int tile_order);
 int build_in_edges(gphi_graph *gr);
 int build_pr_bands(gphi_graph *gr);  // after build_in_edges
+int build_in_weights(gphi_graph *gr);  // after build_in_edges; weighted graphs with CSR
 int run_bfs(gphi_graph *gr, uint32_t src, int32_t *levels, int32_t *parents, gphi_stats *st);
 int run_bfs_noalloc(gphi_graph *gr, uint32_t src, int32_t *levels_dev, int32_t *parents_dev, gphi_stats *st);
 void noalloc_setup(int device);  // with `device` current, once per device (gphi_open)
@@ -262,6 +271,11 @@ int run_bfs_batch(gphi_graph *gr, const uint32_t *srcs, uint32_t k, uint8_t *lev
 // the multi-source state of gr for nw mask words: traversal state, in-edge
 // arrays, masks, levels (idempotent)
 int batch_prepare(gphi_graph *gr, uint32_t nw);
+// weighted SSSP from k <= kSsspBatchMax sources in one traversal
+// (sssp_batch.hip); rows[i]: n distances of source i to copy out, or null
+// (rows itself may be null)
+int run_sssp_batch(gphi_graph *gr, const uint32_t *srcs, uint32_t k, uint32_t *const *rows, gphi_stats *st);
+int sssp_batch_prepare(gphi_graph *gr);  // its state, in-edges and weights (idempotent)
 
 // A multi-source traversal (batch.hip).  gphi_bfs_batch runs one to its end
 // on the device; the serving loop advances one level by level, admitting new

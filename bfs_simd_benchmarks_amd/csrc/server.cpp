@@ -12,8 +12,10 @@
 // completes as soon as its own frontier is empty: its levels are copied out
 // while the traversal goes on for the others.  A weighted graph also gets SSSP
 // lanes -- views of the graph (gphi_graph_view) on streams of their own --
-// that run SSSP queries one at a time beside the BFS lane.  Graphs are
-// admitted while their HBM bytes plus traversal state fit the budget.
+// that run SSSP queries beside the BFS lane: the SSSP queries pending on a
+// lane share one multi-source traversal (up to GPHI_SSSP_BATCH_MAX) when a
+// query's share of one is expected to beat a run alone.  Graphs are admitted
+// while their HBM bytes plus traversal state fit the budget.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -62,6 +64,10 @@ struct Lane {
 	// query's share of a multi-source traversal, per mask width (1, 2, 4
 	// words); 0 = not measured yet
 	double t_plain = 0, t_share[3] = {0, 0, 0};
+	// the same for SSSP: one query alone, and a query's share of a
+	// multi-source SSSP traversal
+	double t_sssp = 0, t_sssp_share = 0;
+	bool sssp_batch_ok = false;  // the lane's graph was prepared for SSSP batches
 	bool busy = false;  // a launch is running (guarded by the server mutex)
 };
 
@@ -77,6 +83,15 @@ bool use_batch(const Lane &L, size_t k, uint32_t max_batch)
 	const int w = width_index(k);
 	if (L.t_plain > 0 && L.t_share[w] > 0) return L.t_share[w] < L.t_plain;
 	return k >= 4;  // no estimate yet: batch only clearly large groups
+}
+
+// k pending SSSP queries share one traversal when a query's share of one is
+// expected to beat a run alone (before any estimate: when 4 or more wait)
+bool use_sssp_batch(const Lane &L, size_t k)
+{
+	if (!L.sssp_batch_ok || k <= 1) return false;
+	if (L.t_sssp > 0 && L.t_sssp_share > 0) return L.t_sssp_share < L.t_sssp;
+	return k >= 4;
 }
 
 void ema(double &x, double v) { x = x > 0 ? 0.8 * x + 0.2 * v : v; }
@@ -107,6 +122,14 @@ uint64_t lane_state_bytes(const DevGraph &G, uint32_t max_batch)
 	const uint64_t nw = max_batch <= 64 ? 1 : max_batch <= 128 ? 2 : 4;
 	const uint64_t in_edges = max_batch > 1 ? 4 * G.m + 12 * (uint64_t)G.n_pad + G.m / 256 : 0;
 	return (uint64_t)G.n_pad * (42 + 3 * 8 * nw + 64 * nw + 1) + G.m / 256 + in_edges;
+}
+
+// what an SSSP lane adds for multi-source SSSP: 16 distances per vertex, the
+// in-edge arrays and their weights (1 byte each below 256, else 4)
+uint64_t sssp_batch_bytes(const DevGraph &G)
+{
+	return (uint64_t)G.n_pad * 4 * GPHI_SSSP_BATCH_MAX + 4 * G.m + 12 * (uint64_t)G.n_pad + G.m / 256 +
+	       (G.coo_w8 ? 1 : 4) * G.m;
 }
 
 // BFS levels of a finished slot into the query's int32 buffer (-1 unreached)
@@ -223,10 +246,22 @@ void lane_loop(gphi_server *S, Lane *L)
 			std::unique_lock<std::mutex> lk(S->mu);
 			S->cv_work.wait(lk, [&] { return S->stopping || !L->pending.empty(); });
 			if (L->pending.empty()) return;  // stopping and drained
-			// one SSSP query alone, or every pending BFS query (up to max_batch)
+			// the pending SSSP queries (up to GPHI_SSSP_BATCH_MAX) when they
+			// are to share a traversal, else one alone; or every pending BFS
+			// query (up to max_batch)
 			if (L->pending.front()->kind == GPHI_QUERY_SSSP) {
-				take.push_back(L->pending.front());
-				L->pending.pop_front();
+				size_t k = 0;
+				for (auto &q : L->pending) k += q->kind == GPHI_QUERY_SSSP;
+				const size_t lim = use_sssp_batch(*L, std::min<size_t>(k, GPHI_SSSP_BATCH_MAX))
+				                       ? GPHI_SSSP_BATCH_MAX : 1;
+				for (auto it = L->pending.begin(); it != L->pending.end() && take.size() < lim;) {
+					if ((*it)->kind == GPHI_QUERY_SSSP) {
+						take.push_back(*it);
+						it = L->pending.erase(it);
+					} else {
+						++it;
+					}
+				}
 			} else {
 				for (auto it = L->pending.begin(); it != L->pending.end() && take.size() < S->opts.max_batch;) {
 					if ((*it)->kind == GPHI_QUERY_BFS) {
@@ -240,20 +275,43 @@ void lane_loop(gphi_server *S, Lane *L)
 			L->busy = true;
 		}
 		if (take[0]->kind == GPHI_QUERY_SSSP) {
-			Query &q = *take[0];
 			uint64_t launch;
 			{
 				std::lock_guard<std::mutex> lk(S->mu);
 				launch = S->next_launch++;
 			}
-			q.t_start = now_us();
-			q.launch = launch;
-			q.batch = 1;
-			const int rc = gphi_sssp(L->g, q.src, (uint32_t *)q.out, nullptr);
-			complete(S, take, rc, now_us());
+			const double t0 = now_us();
+			for (auto &q : take) {
+				q->t_start = t0;
+				q->launch = launch;
+				q->batch = (uint32_t)take.size();
+			}
+			int rc;
+			if (take.size() == 1) {
+				rc = gphi_sssp(L->g, take[0]->src, (uint32_t *)take[0]->out, nullptr);
+			} else {
+				uint32_t srcs[GPHI_SSSP_BATCH_MAX];
+				uint32_t *rows[GPHI_SSSP_BATCH_MAX];
+				for (size_t i = 0; i < take.size(); ++i) {
+					srcs[i] = take[i]->src;
+					rows[i] = (uint32_t *)take[i]->out;
+				}
+				std::lock_guard<std::mutex> ctx_lk(L->ctx->mu);
+				rc = hipSetDevice(L->ctx->device) == hipSuccess
+				         ? run_sssp_batch(L->g, srcs, (uint32_t)take.size(), rows, nullptr)
+				         : GPHI_E_HIP;
+			}
+			const double t1 = now_us();
+			complete(S, take, rc, t1);
 			std::lock_guard<std::mutex> lk(S->mu);
+			if (rc == GPHI_OK) {
+				if (take.size() == 1)
+					ema(L->t_sssp, t1 - t0);
+				else
+					ema(L->t_sssp_share, (t1 - t0) / (double)take.size());
+			}
 			L->batches += 1;
-			L->served += 1;
+			L->served += take.size();
 		} else if (use_batch(*L, take.size(), S->opts.max_batch)) {
 			std::lock_guard<std::mutex> ctx_lk(L->ctx->mu);  // the traversal's kernels on this context
 			if (hipSetDevice(L->ctx->device) != hipSuccess) {
@@ -325,8 +383,11 @@ int gphi_server_add_graph(gphi_server *S, gphi_graph *g, uint32_t *graph_id)
 	if (g->base) return set_error(nullptr, GPHI_E_ARG, "add the graph itself, not a view of it");
 	const bool weighted = g->g.weighted;
 	const uint32_t nv = weighted ? S->opts.sssp_lanes : 0;
+	// SSSP queries share traversals when the server batches and the graph has
+	// the CSR and tile arrays the multi-source SSSP reads
+	const bool sssp_batch = weighted && S->opts.max_batch > 1 && g->g.has_csr && g->g.has_tiles;
 	const uint64_t need = g->g.device_bytes + gphi::lane_state_bytes(g->g, S->opts.max_batch) +
-	                      nv * gphi::lane_state_bytes(g->g, 1);
+	                      nv * (gphi::lane_state_bytes(g->g, 1) + (sssp_batch ? gphi::sssp_batch_bytes(g->g) : 0));
 	{
 		// the check and the reservation in one critical section, so two
 		// concurrent admissions cannot both fit into the same free bytes
@@ -372,6 +433,21 @@ int gphi_server_add_graph(gphi_server *S, gphi_graph *g, uint32_t *graph_id)
 			unreserve();
 			return rc;
 		}
+		if (sssp_batch) {
+			{
+				// built now, not by the first batch of SSSP queries
+				std::lock_guard<std::mutex> ctx_lk(vc->mu);
+				rc = hipSetDevice(vc->device) == hipSuccess ? gphi::sssp_batch_prepare(v) : GPHI_E_HIP;
+			}
+			if (rc != GPHI_OK) {
+				gphi_graph_free(v);
+				gphi_close(vc);
+				for (gphi_graph *x : views) gphi_graph_free(x);
+				for (gphi_ctx *x : vctxs) gphi_close(x);
+				unreserve();
+				return rc;
+			}
+		}
 		views.push_back(v);
 		vctxs.push_back(vc);
 	}
@@ -389,6 +465,7 @@ int gphi_server_add_graph(gphi_server *S, gphi_graph *g, uint32_t *graph_id)
 		V->g = views[i];
 		V->ctx = vctxs[i];
 		V->n = g->g.n;
+		V->sssp_batch_ok = sssp_batch;
 		sl.push_back(V.get());
 		S->lanes.push_back(std::move(V));
 	}

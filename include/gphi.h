@@ -155,6 +155,15 @@ int gphi_bellman_ford(gphi_graph *g, uint32_t src, uint32_t max_sweeps, uint32_t
  * st->edges_traversed are summed over the k sources. */
 int gphi_bfs_batch(gphi_graph *g, const uint32_t *srcs, uint32_t k, uint8_t *levels,
                    gphi_stats *st);
+/* Multi-source weighted SSSP: k (1..GPHI_SSSP_BATCH_MAX) sources in one
+ * label-correcting traversal (no reference counterpart: sssp_weighted,
+ * apps/sssp_simd/sssp.cpp:969-1112, with the 16 sources' distances of a
+ * vertex side by side, relaxed together along every edge of the union
+ * frontier).  dist: k*n uint32, source-major (dist[i*n + v]), 2147483647 when
+ * unreached; NULL keeps them in HBM.  Each row equals gphi_sssp's distances
+ * of that source.  st->steps and st->total_ms are set. */
+#define GPHI_SSSP_BATCH_MAX 16
+int gphi_sssp_batch(gphi_graph *g, const uint32_t *srcs, uint32_t k, uint32_t *dist, gphi_stats *st);
 
 /* ---- serving: resident graphs, continuous batching (no reference counterpart) ----
  * Each added graph gets a BFS lane with the graph's own context (HIP stream)

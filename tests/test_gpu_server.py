@@ -164,3 +164,23 @@ def test_query_joins_running_traversal(c1):
     print(f"joined the running launch: {len(joined)} of {len(info2)}, at levels "
           f"{sorted(i.joined_level for i in joined)}", flush=True)
     assert joined, [(i.launch, i.joined_level) for i in info2]
+
+
+def test_sssp_queries_share_a_traversal(c3):
+    """SSSP queries that pile up on a lane run as one multi-source SSSP
+    traversal; every answer equals Dijkstra's."""
+    print("case: server SSSP batches", flush=True)
+    srv = G.Server(max_batch=64, sssp_lanes=1)
+    try:
+        gid = srv.add_graph(lane(c3, weighted=True))
+        srcs = [int(s) for s in pick_sources(c3.deg, 40, 91)]
+        outs = [np.empty(c3.n, np.uint32) for _ in srcs]
+        tickets = [srv.submit(gid, G.QUERY_SSSP, s, o) for s, o in zip(srcs, outs)]
+        infos = [srv.wait(t) for t in tickets]
+        for s, o in zip(srcs, outs):
+            assert np.array_equal(o, O.sssp_dijkstra(c3.n, c3.deg, c3.cols, c3.cw, s)), s
+        assert max(i.batch for i in infos) > 1
+        st = srv.stats(gid)
+        assert st["served"] == len(srcs) and st["launches"] < len(srcs)
+    finally:
+        srv.close()
