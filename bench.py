@@ -535,12 +535,52 @@ def serving_loads(G, loads, queries) -> dict:
             "loads": results}
 
 
+def sssp_group_costs(G) -> dict:
+    """What sharing costs on the serving mix's weighted graphs: wall ms of one
+    SSSP alone (mean and median over 24 uniform random sources, seed 3) and of
+    one multi-source SSSP traversal of k = 2, 4, 8, 16 of those sources (mean
+    over 4 groups)."""
+    from bfs_simd_benchmarks_amd.workloads import SERVING_MIX
+    rng = np.random.default_rng(3)
+    out = {}
+    for w in SERVING_MIX:
+        if not w.weight_seed:
+            continue
+        ctx = G.Context(0)
+        g = w.generate(ctx, W=1024, S=16, weighted=True)
+        try:
+            srcs = rng.integers(0, g.n, 64)
+            g.sssp(int(srcs[0]))
+            g.sssp_batch(srcs[:16], dist=False)           # warm-up (state)
+            alone = []
+            for s_ in srcs[:24]:
+                t0 = time.perf_counter()
+                g.sssp(int(s_))
+                alone.append((time.perf_counter() - t0) * 1e3)
+            r = {"alone_mean": round(float(np.mean(alone)), 3), "alone_median": round(float(np.median(alone)), 3)}
+            for k in (2, 4, 8, 16):
+                tk = []
+                for i in range(4):
+                    t0 = time.perf_counter()
+                    g.sssp_batch(srcs[16 * i:16 * i + k], dist=False)
+                    tk.append((time.perf_counter() - t0) * 1e3)
+                r[f"group_{k}"] = round(float(np.mean(tk)), 3)
+            out[w.name] = r
+            print(f"[serving] sssp {w.name}: {r}", file=sys.stderr, flush=True)
+        finally:
+            g.free()
+    return out
+
+
 def serving(args) -> int:
-    """The serving mix over the offered loads of --qps (serving_loads)."""
+    """The serving mix over the offered loads of --qps (serving_loads), after
+    the SSSP group costs of its weighted graphs (sssp_group_costs)."""
     import bfs_simd_benchmarks_amd as G
     loads = [float(x) for x in str(args.qps).split(",") if x]
+    costs = sssp_group_costs(G)
     out = {"metric": "serving mix: answered queries/s and p50/p99 latency per offered load, "
                      "batched vs one-at-a-time",
+           "sssp_ms_alone_and_per_group": costs,
            **serving_loads(G, loads, args.queries)}
     print(json.dumps(out))
     return 0

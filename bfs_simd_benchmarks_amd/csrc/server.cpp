@@ -13,8 +13,8 @@
 // while the traversal goes on for the others.  A weighted graph also gets SSSP
 // lanes -- views of the graph (gphi_graph_view) on streams of their own --
 // that run SSSP queries beside the BFS lane: the SSSP queries pending on a
-// lane share one multi-source traversal (up to GPHI_SSSP_BATCH_MAX) when a
-// query's share of one is expected to beat a run alone.  Graphs are admitted
+// lane share one multi-source traversal (up to GPHI_SSSP_BATCH_MAX) when it
+// is expected to take less time than running them alone.  Graphs are admitted
 // while their HBM bytes plus traversal state fit the budget.
 #include <hip/hip_runtime.h>
 
@@ -52,6 +52,30 @@ struct Query {
 	bool done = false;
 };
 
+// A multi-source traversal costs a fixed amount per level (the update and
+// frontier passes, a host round trip; for SSSP, 16 distances per vertex
+// whatever the number of sources) plus a little per source, so a query's
+// share of one falls as the group grows.  RunCosts keeps the time of a whole
+// traversal per size class, k in [2^c, 2^(c+1)), the class of a BFS traversal
+// counting the queries that joined it.
+struct RunCosts {
+	double t[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+	static int size_class(size_t k)
+	{
+		int c = 0;
+		while (c < 8 && (size_t(2) << c) <= k) ++c;
+		return c;
+	}
+	// the estimate for k's class, else that of the nearest larger class
+	// measured (an upper bound for k); 0 when none is
+	double upper(size_t k) const
+	{
+		for (int c = size_class(k); c < 9; ++c)
+			if (t[c] > 0) return t[c];
+		return 0;
+	}
+};
+
 struct Lane {
 	gphi_graph *g = nullptr;
 	gphi_ctx *ctx = nullptr;
@@ -60,41 +84,35 @@ struct Lane {
 	std::thread worker;
 	uint64_t device_bytes = 0;
 	uint64_t batches = 0, served = 0;
-	// running estimates (microseconds) of one plain BFS query and of a BFS
-	// query's share of a multi-source traversal, per mask width (1, 2, 4
-	// words); 0 = not measured yet
-	double t_plain = 0, t_share[3] = {0, 0, 0};
-	// the same for SSSP: one query alone, and a query's share of a
-	// multi-source SSSP traversal
-	double t_sssp = 0, t_sssp_share = 0;
+	// running estimates (microseconds) of one query run alone, and of a whole
+	// multi-source traversal per size class (RunCosts); 0 = not measured yet
+	double t_plain = 0, t_sssp = 0;
+	RunCosts bfs_runs, sssp_runs;
 	bool sssp_batch_ok = false;  // the lane's graph was prepared for SSSP batches
 	bool busy = false;  // a launch is running (guarded by the server mutex)
 };
 
-int width_index(size_t slots) { return slots <= 64 ? 0 : slots <= 128 ? 1 : 2; }
+void ema(double &x, double v) { x = x > 0 ? 0.8 * x + 0.2 * v : v; }
 
-// The multi-source traversal has a fixed cost per level (the update and
-// frontier passes, a host round trip) that small groups do not repay: start
-// one for k pending BFS queries only when a query's expected share of it
-// beats a plain run.
+// k pending queries share one traversal when its expected time beats k runs
+// alone; before both estimates exist, when at least `first` are pending
+bool share_pays(const RunCosts &runs, double t_alone, size_t k, size_t first)
+{
+	if (k <= 1) return false;
+	const double run = runs.upper(k);
+	if (t_alone > 0 && run > 0) return run < (double)k * t_alone;
+	return k >= first;
+}
+
 bool use_batch(const Lane &L, size_t k, uint32_t max_batch)
 {
-	if (max_batch <= 1 || k <= 1) return false;
-	const int w = width_index(k);
-	if (L.t_plain > 0 && L.t_share[w] > 0) return L.t_share[w] < L.t_plain;
-	return k >= 4;  // no estimate yet: batch only clearly large groups
+	return max_batch > 1 && share_pays(L.bfs_runs, L.t_plain, k, 4);
 }
 
-// k pending SSSP queries share one traversal when a query's share of one is
-// expected to beat a run alone (before any estimate: when 4 or more wait)
 bool use_sssp_batch(const Lane &L, size_t k)
 {
-	if (!L.sssp_batch_ok || k <= 1) return false;
-	if (L.t_sssp > 0 && L.t_sssp_share > 0) return L.t_sssp_share < L.t_sssp;
-	return k >= 4;
+	return L.sssp_batch_ok && share_pays(L.sssp_runs, L.t_sssp, k, 8);
 }
-
-void ema(double &x, double v) { x = x > 0 ? 0.8 * x + 0.2 * v : v; }
 
 }  // namespace
 
@@ -231,7 +249,7 @@ void run_continuous(gphi_server *S, Lane *L, std::vector<std::shared_ptr<Query>>
 	}
 	const double t1 = now_us();
 	if (!rest.empty()) complete(S, rest, rc, t1);
-	if (rc == GPHI_OK && !slot_q.empty()) ema(L->t_share[width_index(run.cap)], (t1 - t0) / (double)slot_q.size());
+	if (rc == GPHI_OK && !slot_q.empty()) ema(L->bfs_runs.t[RunCosts::size_class(slot_q.size())], t1 - t0);
 	std::lock_guard<std::mutex> lk(S->mu);
 	L->batches += 1;
 	L->served += slot_q.size();
@@ -308,7 +326,7 @@ void lane_loop(gphi_server *S, Lane *L)
 				if (take.size() == 1)
 					ema(L->t_sssp, t1 - t0);
 				else
-					ema(L->t_sssp_share, (t1 - t0) / (double)take.size());
+					ema(L->sssp_runs.t[RunCosts::size_class(take.size())], t1 - t0);
 			}
 			L->batches += 1;
 			L->served += take.size();

@@ -176,7 +176,10 @@ int gphi_sssp_batch(gphi_graph *g, const uint32_t *srcs, uint32_t k, uint32_t *d
  * (up to the launch's mask width: 64, 128 or 256 sources), and every query
  * completes as soon as its own frontier is empty.  A weighted graph also
  * gets sssp_lanes SSSP lanes, each a gphi_graph_view on a context of its own,
- * running SSSP queries one at a time beside the BFS lane.  A graph is
+ * running SSSP queries beside the BFS lane: the SSSP queries pending on a
+ * lane (up to GPHI_SSSP_BATCH_MAX) share one multi-source SSSP traversal
+ * when that is expected to beat running them alone (with max_batch 1, every
+ * query runs alone).  A graph is
  * admitted only while its HBM bytes plus every lane's traversal state fit
  * hbm_budget (GPHI_E_OOM otherwise).  The server takes ownership of each
  * added graph and of its context (create one context per graph). */
