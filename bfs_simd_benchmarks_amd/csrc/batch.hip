@@ -25,6 +25,15 @@
 // whose next mask it set in a bitmap, and the per-level update visits only
 // those (and the previous frontier), not every vertex.
 //
+// The union direction rule can make a dense step for sources whose own
+// frontiers are still small.  A pull finds such a source only far into a
+// vertex's in-edges, so wanting it keeps the heavy pieces scanning to their
+// end.  The update counts each source's new vertices; a source whose
+// frontier is estimated to have fewer than M / kSparseRatio out-edges is
+// left out of the next dense step's pulls and pushed instead, from a queue of
+// its frontier vertices.  Which sources go which way changes only the work,
+// not the levels.
+//
 // Sources may also join a running traversal between two levels (the serving
 // loop's continuous batching): a source that joins at level J gets a slot of
 // its own, its vertex enters the union frontier, and the levels stored for
@@ -48,6 +57,16 @@ typedef unsigned long long u64;
 // b_aux layout (uint32 words): sources to seed, their slots, per-slot start level
 constexpr uint32_t kAuxSrc = 0, kAuxSlot = kBatchMaxSources, kAuxStart = 2 * kBatchMaxSources;
 constexpr uint32_t kAuxWords = 3 * kBatchMaxSources;
+// the sources of a dense step whose frontiers have few out-edges (sparse
+// sources, batch_live_kernel); their mask sits beside the live mask
+constexpr uint32_t kSparseWord = 4;     // live[4..7]: the sparse sources of the next frontier
+constexpr uint32_t kSparseCount = 8;    // live[8]: entries of the sparse-source queue
+constexpr uint32_t kSparseOverflow = 9; // live[9]: the queue overflowed (the pulls take the sparse sources)
+// sparse: fewer than M / 1536 frontier out-edges (estimated).  Measured at
+// scale 25 over ratios 640..2560: 960..1920 is best, past it the pull keeps
+// scanning for the sources left to it, below it the push takes too many
+constexpr uint32_t kSparseRatio = 1536;
+constexpr uint32_t kCountSample = 16;   // after a dense step, every 16th block counts new vertices per source
 
 __device__ __forceinline__ bool test_bit32(const uint32_t *__restrict__ bm, uint32_t v)
 {
@@ -283,7 +302,10 @@ __global__ __launch_bounds__(kBlock) void batch_pull_light_kernel(
 
 // Pull for vertices with more than kPullLight in-edges: a wavefront per piece
 // of up to kPullPiece in-edges, 64 · kUnroll edges between checks of the stop rule.
-// Pieces of one vertex combine with one atomicOr per word.
+// Pieces of one vertex combine with one atomicOr per word.  The sparse
+// sources (few frontier out-edges, batch_live_kernel) are left to
+// batch_sparse_src_push_kernel: a pull finds them only far into a vertex's
+// in-edges, so wanting them would keep most pieces scanning to their end.
 template <int NW, bool kFilter>
 __global__ __launch_bounds__(kBlock) void batch_pull_heavy_kernel(
     uint32_t n, const StepCtrl *__restrict__ ctrl, uint32_t npieces, const uint2 *__restrict__ pieces,
@@ -292,6 +314,12 @@ __global__ __launch_bounds__(kBlock) void batch_pull_heavy_kernel(
     uint32_t *touched)
 {
 	if (!pull_runs<kFilter>(ctrl, n)) return;
+	// the sources to pull: live ones, less the sparse ones unless their
+	// queue overflowed
+	const bool pushed = live[kSparseOverflow] == 0;
+	u64 pull[NW];
+#pragma unroll
+	for (int i = 0; i < NW; ++i) pull[i] = live[i] & (pushed ? ~live[kSparseWord + i] : ~0ull);
 	// loads in flight per lane between checks of the stop rule: a vertex stops
 	// as soon as every wanted source is found, so short rounds (64 or 128
 	// edges) beat deeper unrolling here
@@ -307,7 +335,7 @@ __global__ __launch_bounds__(kBlock) void batch_pull_heavy_kernel(
 		u64 want[NW], acc[NW], red[NW], any = 0;
 #pragma unroll
 		for (int i = 0; i < NW; ++i) {
-			want[i] = uniform64(~seen[(uint64_t)v * NW + i] & live[i]);
+			want[i] = uniform64(~seen[(uint64_t)v * NW + i] & pull[i]);
 			any |= want[i];
 			acc[i] = 0;
 			red[i] = 0;
@@ -413,21 +441,39 @@ __global__ __launch_bounds__(kBlock) void batch_sparse_kernel(
 // vertices, one coalesced byte store).  Blocks publish the sources they saw
 // advance into one of kLiveSlots slots and add the new pairs (and their
 // out-degrees) to the totals.
+//
+// Two launch shapes, each returning at once on the other's steps: after a
+// dense step (kSpan = false) a block per kBlock vertices, since the pulls
+// touch most vertices and independent blocks hide their latency best; after a
+// sparse step (kSpan = true) a block per kUpdateSpan vertices, which reads
+// their touched and frontier words in one round into LDS and then walks
+// them, so that a step touching few vertices costs one load round per block,
+// not one per kBlock vertices.
+constexpr uint32_t kUpdateSpan = 4096;  // vertices per block after a sparse step (divides N_pad)
 constexpr uint32_t kLiveSlots = 64;
 constexpr uint32_t kLiveStride = 16;  // u64 words per slot (one 128-byte line)
 constexpr uint32_t kSlotPairs = 8;    // slot words 8, 9: new pairs and their out-degrees
 
-template <int NW>
+template <int NW, bool kSpan>
 __global__ __launch_bounds__(kBlock) void batch_update_kernel(
     uint32_t n, uint32_t n_pitch, uint32_t next_level, StepCtrl *ctrl, const uint32_t *__restrict__ cur,
     uint32_t *__restrict__ touched, u64 *__restrict__ front, u64 *__restrict__ seen, u64 *__restrict__ next,
     uint8_t *__restrict__ level, uint8_t *__restrict__ flag, const uint32_t *__restrict__ slot_start,
-    const uint32_t *__restrict__ deg, u64 *slots, u64 *sums)
+    const uint32_t *__restrict__ deg, u64 *slots, u64 *sums, uint32_t *scount)
 {
-	if (ctrl->done) return;
+	if (ctrl->done || (ctrl->dir == 0) != kSpan) return;
+	__shared__ uint32_t s_words[kSpan ? 2 * kUpdateSpan / 32 : 1];  // touched words, then frontier words
 	__shared__ u64 s_or[kBlock / 64][NW];
 	__shared__ u64 s_cnt[kBlock / 64][2];
+	__shared__ uint32_t s_src[kBlock / 64][NW * 64];  // new vertices per source, per wavefront
 	const uint32_t lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+	// every block after a sparse step, every kCountSample-th after a dense
+	// one, counts the new vertices of each source (scaled up accordingly)
+	const bool counting = kSpan || blockIdx.x % kCountSample == 0;
+	if (counting) {
+#pragma unroll
+		for (int j = 0; j < NW; ++j) s_src[wid][j * 64 + lane] = 0;
+	}
 	u64 wor[NW];
 	u64 pairs = 0, pdeg = 0;
 	bool over = false;
@@ -438,15 +484,45 @@ __global__ __launch_bounds__(kBlock) void batch_update_kernel(
 	uint32_t rel_lane[NW];
 #pragma unroll
 	for (int j = 0; j < NW; ++j) rel_lane[j] = next_level - slot_start[j * 64 + lane];
-	// blocks walk the vertices kBlock at a time (any grid size works)
-	for (uint32_t vb = blockIdx.x * kBlock; vb < n; vb += gridDim.x * kBlock) {
+	uint32_t v_lo, v_hi, v_step;
+	if (kSpan) {
+		// the span's touched words (cleared for the next step as they are
+		// read) and frontier words, one load each per thread
+		v_lo = blockIdx.x * kUpdateSpan;
+		v_hi = min(n, v_lo + kUpdateSpan);
+		v_step = kBlock;
+		static_assert(2 * kUpdateSpan / 32 == kBlock, "one word per thread");
+		const uint32_t i = threadIdx.x, w = v_lo / 32 + (i % (kUpdateSpan / 32));
+		uint32_t x = 0;
+		if (i < kUpdateSpan / 32) {
+			x = touched[w];
+			if (x) touched[w] = 0u;
+		} else {
+			x = cur[w];
+		}
+		s_words[i] = x;
+		__syncthreads();
+	} else {
+		// blocks walk the vertices kBlock at a time (any grid size works)
+		v_lo = blockIdx.x * kBlock;
+		v_hi = n;
+		v_step = gridDim.x * kBlock;
+	}
+	for (uint32_t vb = v_lo; vb < v_hi; vb += v_step) {
 		const uint32_t v = vb + threadIdx.x;
-		const bool in = v < n;
-		const uint32_t tw = in ? touched[v >> 5] : 0u;
-		const uint32_t cw = in ? cur[v >> 5] : 0u;
+		const bool in = v < v_hi;
+		uint32_t tw, cw;
+		if (kSpan) {
+			const uint32_t k = (v - v_lo) / 32;
+			tw = in ? s_words[k] : 0u;
+			cw = in ? s_words[kUpdateSpan / 32 + k] : 0u;
+		} else {
+			tw = in ? touched[v >> 5] : 0u;
+			cw = in ? cur[v >> 5] : 0u;
+		}
 		const bool t = (tw >> (v & 31)) & 1u, c = (cw >> (v & 31)) & 1u;
 		if (!__ballot(t || c)) continue;  // uniform
-		if ((v & 31) == 0 && tw) touched[v >> 5] = 0u;  // clear for the next step
+		if (!kSpan && (v & 31) == 0 && tw) touched[v >> 5] = 0u;  // clear for the next step
 		u64 nw[NW];
 		u64 anyn = 0;
 #pragma unroll
@@ -484,7 +560,12 @@ __global__ __launch_bounds__(kBlock) void batch_update_kernel(
 				// source s's level, from lane b's register (no load in the loop)
 				const uint32_t rel = (uint32_t)__builtin_amdgcn_readlane((int)rel_lane[j], (int)b);
 				over |= rel >= kUnreached;
-				if ((nw[j] >> b) & 1) level[(uint64_t)s * n_pitch + v] = (uint8_t)rel;
+				const bool mine = (nw[j] >> b) & 1;
+				if (mine) level[(uint64_t)s * n_pitch + v] = (uint8_t)rel;
+				if (counting) {  // block-uniform; every lane takes part in the ballot
+					const uint32_t c = (uint32_t)__popcll(__ballot(mine));
+					if (lane == 0) s_src[wid][s] += c;
+				}
 			}
 		}
 	}
@@ -495,11 +576,10 @@ __global__ __launch_bounds__(kBlock) void batch_update_kernel(
 		for (int j = 0; j < NW; ++j) s_or[wid][j] = wor[j];
 		s_cnt[wid][0] = pairs;
 		s_cnt[wid][1] = pdeg;
-		if (over) {
-			// a level of 255 or more would read as unreached: stop here
-			ctrl->err = 1;
-			ctrl->done = 1;
-		}
+		// a level of 255 or more would read as unreached: the live kernel
+		// stops the traversal (no block of this launch may see done change,
+		// since every block of it reads done to decide whether to run)
+		if (over) ctrl->err = 1;
 	}
 	__syncthreads();
 	if (threadIdx.x < NW) {
@@ -527,21 +607,39 @@ __global__ __launch_bounds__(kBlock) void batch_update_kernel(
 			atomicAdd(slot + 1, d);
 		}
 	}
+	if (counting && threadIdx.x < NW * 64) {
+		uint32_t c = 0;
+#pragma unroll
+		for (uint32_t k = 0; k < kBlock / 64; ++k) c += s_src[k][threadIdx.x];
+		if (c)
+			atomicAdd(scount + (blockIdx.x % kLiveSlots) * kBatchMaxSources + threadIdx.x,
+			          kSpan ? c : c * kCountSample);
+	}
 }
 
 // live = OR of the slots (the sources whose new frontier is non-empty); the
-// slots are cleared for the next step.  One wavefront.
+// slots are cleared for the next step.  Then the sparse sources of the next
+// frontier (live[kSparseWord + i]): those whose new vertices, times the mean
+// out-degree of this step's new (source, vertex) pairs, come to less than
+// M / kSparseRatio edges.  A pull finds such a source only after scanning
+// about kSparseRatio in-edges per vertex, so on a dense step the heavy pull
+// leaves them to a push from their frontier.  One wavefront.
 template <int NW>
-__global__ void batch_live_kernel(const StepCtrl *__restrict__ ctrl, u64 *live, u64 *slots, u64 *sums)
+__global__ void batch_live_kernel(StepCtrl *ctrl, u64 *live, u64 *slots, u64 *sums, uint32_t *scount, u64 m)
 {
 	if (ctrl->done) return;
+	if (ctrl->err) {  // the update met a level past 254: stop here
+		if (threadIdx.x == 0) ctrl->done = 1;
+		return;
+	}
 	const uint32_t lane = threadIdx.x;
+	u64 lv[NW];
 #pragma unroll
 	for (int i = 0; i < NW; ++i) {
 		u64 *p = slots + (uint64_t)lane * kLiveStride + i;
-		const u64 x = wave_or(*p);
+		lv[i] = wave_or(*p);
 		*p = 0;
-		if (lane == 0) live[i] = x;
+		if (lane == 0) live[i] = lv[i];
 	}
 	// the step's new pairs and their out-degrees, into the totals
 	u64 *t = slots + (uint64_t)lane * kLiveStride + kSlotPairs;
@@ -551,6 +649,126 @@ __global__ void batch_live_kernel(const StepCtrl *__restrict__ ctrl, u64 *live, 
 	if (lane == 0) {
 		sums[0] += p;
 		sums[1] += d;
+	}
+	// lane l: source i * 64 + l; its count summed over the slots (and cleared)
+	const double mean_deg = p ? (double)d / (double)p : 0.0;
+#pragma unroll
+	for (int i = 0; i < NW; ++i) {
+		const uint32_t src = i * 64 + lane;
+		uint32_t c = 0;
+		for (uint32_t k = 0; k < kLiveSlots; ++k) {
+			uint32_t *q = scount + k * kBatchMaxSources + src;
+			c += *q;
+			*q = 0;
+		}
+		const bool sparse = ((lv[i] >> lane) & 1ull) && (double)c * mean_deg * kSparseRatio < (double)m;
+		const u64 sp = __ballot(sparse);
+		if (lane == 0) live[kSparseWord + i] = sp;
+	}
+	if (lane == 0) {
+		live[kSparseCount] = 0;
+		live[kSparseOverflow] = 0;
+	}
+}
+
+// inclusive prefix sum over the wavefront
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x)
+{
+	const uint32_t lane = threadIdx.x & 63;
+#pragma unroll
+	for (int o = 1; o < 64; o <<= 1) {
+		const uint32_t y = (uint32_t)__shfl_up((int)x, o, 64);
+		if (lane >= (uint32_t)o) x += y;
+	}
+	return x;
+}
+
+// The queue of a dense step's sparse sources: an entry (vertex, chunk) per
+// kSparseTile out-edges of each vertex whose front mask holds a sparse
+// source, in the queue half of the step's parity (a dense step's frontier
+// update leaves that half unused).  Runs after the frontier update has set
+// the step's direction; nothing to do unless the step is dense and has
+// sparse sources.  A lane per vertex, a block-stride walk; live[kSparseCount]
+// counts the entries (cleared by the live kernel), and an entry past the
+// half's capacity sets live[kSparseOverflow], which hands the sparse sources
+// back to the pulls.
+template <int NW>
+__global__ __launch_bounds__(kBlock) void batch_sparse_queue_kernel(uint32_t n, uint32_t n_pad,
+                                                                    const StepCtrl *__restrict__ ctrl, u64 *live,
+                                                                    const u64 *__restrict__ front,
+                                                                    const uint32_t *__restrict__ row_off,
+                                                                    QueueBufs qb)
+{
+	if (ctrl->done || ctrl->dir != 1) return;
+	u64 sp[NW], any = 0;
+#pragma unroll
+	for (int i = 0; i < NW; ++i) {
+		sp[i] = live[kSparseWord + i];
+		any |= sp[i];
+	}
+	if (!any) return;
+	const uint32_t par = ctrl->level & 1u;
+	uint32_t *__restrict__ queue = qb.queue + par * qb.qstride;
+	uint32_t *__restrict__ qchunk = qb.qoff + par * qb.qstride;
+	const uint32_t lane = threadIdx.x & 63;
+	for (uint32_t v = blockIdx.x * kBlock + threadIdx.x; v < n_pad; v += gridDim.x * kBlock) {
+		bool hit = false;
+		if (v < n) {
+#pragma unroll
+			for (int i = 0; i < NW; ++i) hit |= (front[(uint64_t)v * NW + i] & sp[i]) != 0;
+		}
+		const uint32_t nc = hit ? (row_off[v + 1] - row_off[v] + kSparseTile - 1) / kSparseTile : 0u;
+		const uint32_t incl = wave_incl_scan(nc);
+		const uint32_t tot = (uint32_t)__shfl((int)incl, 63, 64);
+		if (!tot) continue;  // uniform
+		u64 base = 0;
+		if (lane == 0) base = atomicAdd(live + kSparseCount, (u64)tot);
+		base = (u64)__shfl((long long)base, 0, 64);
+		if (base + tot > qb.qstride) {
+			if (lane == 0) live[kSparseOverflow] = 1;
+			continue;
+		}
+		for (uint32_t c = 0; c < nc; ++c) {
+			const uint32_t i = (uint32_t)base + incl - nc + c;
+			queue[i] = v;
+			qchunk[i] = c;
+		}
+	}
+}
+
+// The sparse sources of a dense step, pushed along the out-edges of their
+// frontier (the queue above): next[t] |= (front[h] & sparse) & ~seen[t].  A
+// wavefront per entry, after the pulls (the light pull stores next and the
+// touched words outright).
+template <int NW>
+__global__ __launch_bounds__(kBlock) void batch_sparse_src_push_kernel(
+    const StepCtrl *__restrict__ ctrl, const u64 *__restrict__ live, QueueBufs qb,
+    const uint32_t *__restrict__ row_off, const uint32_t *__restrict__ cols, const u64 *__restrict__ front,
+    const u64 *__restrict__ seen, u64 *next, uint32_t *touched)
+{
+	if (ctrl->done || ctrl->dir != 1 || live[kSparseOverflow]) return;
+	Mask<NW> sp;
+	u64 any = 0;
+#pragma unroll
+	for (int i = 0; i < NW; ++i) {
+		sp.w[i] = live[kSparseWord + i];
+		any |= sp.w[i];
+	}
+	if (!any) return;
+	const uint32_t cnt = (uint32_t)live[kSparseCount];
+	const uint32_t par = ctrl->level & 1u;
+	const uint32_t *__restrict__ queue = qb.queue + par * qb.qstride;
+	const uint32_t *__restrict__ qchunk = qb.qoff + par * qb.qstride;
+	const uint32_t lane = threadIdx.x & 63;
+	const uint32_t nwaves = (gridDim.x * kBlock) >> 6;
+	for (uint32_t q = (blockIdx.x * kBlock + threadIdx.x) >> 6; q < cnt; q += nwaves) {
+		const uint32_t h = queue[q];
+		const uint32_t e0 = row_off[h] + qchunk[q] * kSparseTile;
+		const uint32_t e1 = min(e0 + kSparseTile, row_off[h + 1]);
+		Mask<NW> f = load_mask<NW>(front, h);
+#pragma unroll
+		for (int i = 0; i < NW; ++i) f.w[i] &= sp.w[i];
+		for (uint32_t e = e0 + lane; e < e1; e += 64) push_mask<NW>(f, cols[e], seen, next, touched);
 	}
 }
 
@@ -583,7 +801,7 @@ __global__ __launch_bounds__(kBlock) void batch_clear_kernel(uint64_t mask_words
                                                              u64 *front, u64 *seen, u64 *next, uint8_t *flag,
                                                              uint32_t *cur, uint32_t *touched, u64 *live_all,
                                                              uint32_t live_words, uint32_t *slot_start,
-                                                             u64 *sums, StepCtrl *ctrl, bool clean)
+                                                             u64 *sums, uint32_t *scount, StepCtrl *ctrl, bool clean)
 {
 	const uint64_t stride = (uint64_t)gridDim.x * kBlock;
 	const uint64_t i0 = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
@@ -603,6 +821,7 @@ __global__ __launch_bounds__(kBlock) void batch_clear_kernel(uint64_t mask_words
 	}
 	for (uint64_t i = i0; i < live_words; i += stride) live_all[i] = 0;
 	for (uint64_t i = i0; i < kBatchMaxSources; i += stride) slot_start[i] = 0;
+	for (uint64_t i = i0; i < kLiveSlots * kBatchMaxSources; i += stride) scount[i] = 0;
 	if (i0 < 2) sums[i0] = 0;
 	if (i0 == 0) {
 		ctrl->level = UINT32_MAX;  // the first frontier update opens level 0
@@ -676,6 +895,9 @@ void enqueue_expand(gphi_graph *gr, uint32_t next_level)
 		                                                        gr->in_src, gr->cur, gr->b_front, gr->b_seen, live,
 		                                                        gr->b_next, gr->b_touch);
 	}
+	batch_sparse_src_push_kernel<NW><<<kBatchSparseGrid, kBlock, 0, s>>>(gr->ctrl, live, gr->queue_bufs(), G.row_off,
+	                                                                   G.cols, gr->b_front, gr->b_seen, gr->b_next,
+	                                                                   gr->b_touch);
 }
 
 template <int NW>
@@ -683,13 +905,27 @@ void enqueue_update(gphi_graph *gr, uint32_t next_level)
 {
 	const DevGraph &G = gr->g;
 	hipStream_t s = gr->ctx->stream;
-	// a block per kBlock vertices: the dense steps' updates touch most
-	// vertices, and independent blocks hide their latency better than a
-	// resident grid walking the vertices in turn
-	batch_update_kernel<NW><<<grid_n(G.n), kBlock, 0, s>>>(
+	// after a dense step, a block per kBlock vertices; after a sparse one, a
+	// block per kUpdateSpan vertices (each launch returns at once on the
+	// other's steps)
+	batch_update_kernel<NW, false><<<grid_n(G.n), kBlock, 0, s>>>(
 	    G.n, G.n_pad, next_level, gr->ctrl, gr->cur, gr->b_touch, gr->b_front, gr->b_seen, gr->b_next, gr->b_level,
-	    gr->flag, slot_starts(gr), G.deg, live_slots(gr), gr->b_sums);
-	batch_live_kernel<NW><<<1, 64, 0, s>>>(gr->ctrl, live_words(gr), live_slots(gr), gr->b_sums);
+	    gr->flag, slot_starts(gr), G.deg, live_slots(gr), gr->b_sums, gr->b_scount);
+	batch_update_kernel<NW, true><<<G.n_pad / kUpdateSpan, kBlock, 0, s>>>(
+	    G.n, G.n_pad, next_level, gr->ctrl, gr->cur, gr->b_touch, gr->b_front, gr->b_seen, gr->b_next, gr->b_level,
+	    gr->flag, slot_starts(gr), G.deg, live_slots(gr), gr->b_sums, gr->b_scount);
+	batch_live_kernel<NW><<<1, 64, 0, s>>>(gr->ctrl, live_words(gr), live_slots(gr), gr->b_sums, gr->b_scount,
+	                                       (u64)G.m);
+}
+
+// the sparse-source queue of the next step (after the frontier update has
+// set its direction)
+template <int NW>
+void enqueue_sparse_queue(gphi_graph *gr)
+{
+	const DevGraph &G = gr->g;
+	batch_sparse_queue_kernel<NW><<<std::min<unsigned>(grid_n(G.n_pad), kBatchWalkGrid), kBlock, 0, gr->ctx->stream>>>(
+	    G.n, G.n_pad, gr->ctrl, live_words(gr), gr->b_front, G.row_off, gr->queue_bufs());
 }
 
 template <int NW>
@@ -723,8 +959,15 @@ void enqueue_level(gphi_graph *gr, uint32_t nw, uint32_t next_level, bool expand
 }
 
 // the frontier update of the union frontier (mark / scan / emit of the
-// single-source loop), in its device-driven form: it opens the next level
-void enqueue_frontier(gphi_graph *gr) { launch_frontier_update_driven(gr, (uint32_t)gr->g.m / gr->ctx->t_ratio); }
+// single-source loop), in its device-driven form: it opens the next level;
+// then the next step's sparse-source queue
+void enqueue_frontier(gphi_graph *gr, uint32_t nw)
+{
+	launch_frontier_update_driven(gr, (uint32_t)gr->g.m / gr->ctx->t_ratio);
+	if (nw == 1) enqueue_sparse_queue<1>(gr);
+	else if (nw == 2) enqueue_sparse_queue<2>(gr);
+	else enqueue_sparse_queue<4>(gr);
+}
 
 }  // namespace
 
@@ -746,7 +989,7 @@ int batch_prepare(gphi_graph *gr, uint32_t nw)
 		if (p) (void)hipFree(p);
 	gr->b_front = gr->b_seen = gr->b_next = gr->b_sums = gr->b_live = nullptr;
 	gr->b_level = nullptr;
-	gr->b_touch = gr->b_aux = nullptr;
+	gr->b_touch = gr->b_aux = gr->b_scount = nullptr;
 	gr->batch_words = 0;
 	const size_t mb = (size_t)G.n_pad * nw * 8;
 	hipError_t e = hipMalloc((void **)&gr->b_front, mb);
@@ -755,11 +998,14 @@ int batch_prepare(gphi_graph *gr, uint32_t nw)
 	if (e == hipSuccess) e = hipMalloc((void **)&gr->b_level, (size_t)nw * 64 * G.n_pad);
 	if (e == hipSuccess) e = hipMalloc((void **)&gr->b_sums, 16);
 	if (e == hipSuccess) e = hipMalloc((void **)&gr->b_live, (size_t)(1 + kLiveSlots) * kLiveStride * 8);
-	if (e == hipSuccess) e = hipMalloc((void **)&gr->b_touch, (size_t)G.n_words * 4);
+	// the touched bitmap, then the per-source counts
+	if (e == hipSuccess)
+		e = hipMalloc((void **)&gr->b_touch, ((size_t)G.n_words + (size_t)kLiveSlots * kBatchMaxSources) * 4);
 	if (e == hipSuccess) e = hipMalloc((void **)&gr->b_aux, (size_t)kAuxWords * 4);
 	if (e != hipSuccess)
 		return set_error(ctx, GPHI_E_OOM, "batched BFS buffers (%zu MB): %s",
 		                 (3 * mb + (size_t)nw * 64 * G.n_pad) >> 20, hipGetErrorString(e));
+	gr->b_scount = gr->b_touch + G.n_words;
 	gr->batch_words = nw;
 	gr->batch_clean_words = 0;
 	return GPHI_OK;
@@ -800,11 +1046,11 @@ int BatchRun::begin(gphi_graph *g, uint32_t capacity, const uint32_t *srcs, uint
 	gr->batch_clean_words = 0;  // until this traversal ends
 	batch_clear_kernel<<<(unsigned)std::min<uint64_t>(grid_n(items), 16384), kBlock, 0, s>>>(
 	    mask_words, G.n_pad, G.n_words, gr->b_front, gr->b_seen, gr->b_next, gr->flag, gr->cur, gr->b_touch,
-	    gr->b_live, (1 + kLiveSlots) * kLiveStride, slot_starts(gr), gr->b_sums, gr->ctrl, clean);
+	    gr->b_live, (1 + kLiveSlots) * kLiveStride, slot_starts(gr), gr->b_sums, gr->b_scount, gr->ctrl, clean);
 	GPHI_HIP(ctx, hipGetLastError());
 	if ((rc = seed(srcs, k, 0))) return rc;
 	// the first frontier update: the sources form the union frontier of level 0
-	enqueue_frontier(gr);
+	enqueue_frontier(gr, nw);
 	GPHI_HIP(ctx, hipGetLastError());
 	return GPHI_OK;
 }
@@ -854,7 +1100,7 @@ int BatchRun::run_to_end(gphi_stats *st)
 			enqueue_level(gr, nw, enq + 1, true);
 			if (ev_step) GPHI_HIP(ctx, hipEventRecord(ev[2 + 2 * enq], s));
 			enqueue_level(gr, nw, enq + 1, false);
-			enqueue_frontier(gr);
+			enqueue_frontier(gr, nw);
 			if (ev_step) GPHI_HIP(ctx, hipEventRecord(ev[3 + 2 * enq], s));
 		}
 		GPHI_HIP(ctx, hipGetLastError());
@@ -923,7 +1169,7 @@ int BatchRun::step(const uint32_t *srcs, uint32_t k)
 	enqueue_level(gr, nw, next_level, false);
 	int rc = seed(srcs, k, next_level);
 	if (rc) return rc;
-	enqueue_frontier(gr);
+	enqueue_frontier(gr, nw);
 	GPHI_HIP(ctx, hipGetLastError());
 	GPHI_HIP(ctx, hipMemcpyAsync(hc, gr->ctrl, sizeof(StepCtrl), hipMemcpyDeviceToHost, s));
 	GPHI_HIP(ctx, hipMemcpyAsync(live, live_words(gr), (size_t)nw * 8, hipMemcpyDeviceToHost, s));

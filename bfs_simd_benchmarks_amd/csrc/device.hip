@@ -489,6 +489,7 @@ void free_traversal_state(gphi_graph *gr)
 		*p = nullptr;
 	}
 	gr->sum = gr->contrib = nullptr;  // inside rank's allocation
+	gr->b_scount = nullptr;  // inside b_touch's allocation
 	gr->batch_words = 0;
 	gr->batch_clean_words = 0;
 }

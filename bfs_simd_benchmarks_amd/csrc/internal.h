@@ -216,6 +216,7 @@ struct gphi_graph {
 	unsigned long long *b_sums = nullptr;
 	unsigned long long *b_live = nullptr;  // sources with a non-empty frontier, then the per-block slots
 	uint32_t *b_touch = nullptr;           // bitmap: vertices whose next mask got a bit this step
+	uint32_t *b_scount = nullptr;          // in b_touch's allocation: new vertices per source, per slot
 	uint32_t *b_aux = nullptr;             // sources to seed, their slots, per-slot start levels
 	// multi-source SSSP (sssp_batch.hip): kSsspBatchMax distances per vertex
 	// (vertex-major, n_pad rows), then the sources

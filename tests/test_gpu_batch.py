@@ -119,3 +119,32 @@ def test_batch_hub_and_piece_boundaries(ctx, k):
         ref = O.bfs_levels(n, deg, t, int(srcs[i]))
         want = np.where(ref >= 0, ref, 255).astype(np.uint8)
         assert np.array_equal(lv[i], want), f"row {i} (source {srcs[i]}) differs"
+
+
+def test_batch_sparse_sources_pushed(ctx, c1):
+    """Sources on a chain hanging off the graph keep one-vertex frontiers
+    while the others' union frontier makes the steps dense: they are pushed
+    from their frontier, the rest pulled.  Every row equals that source's own
+    levels, before and after the chain reaches the graph."""
+    chain = 12
+    n = c1.n + chain
+    ids = np.arange(c1.n, n, dtype=np.uint32)
+    # c1.n -> c1.n+1 -> ... -> last -> vertex 0 (the largest out-degree)
+    h = np.concatenate([c1.csr_heads, ids])
+    t = np.concatenate([c1.cols, np.append(ids[1:], np.uint32(0))])
+    uh, ut, _, deg = O.untiled(n, h, t)
+    W, S = c1.W, c1.S
+    th, tt, _, off = O.coo_tile(n, W, uh, ut)
+    ch, ct, _, coff = O.col_major(n, W, S, th, tt, None, off)
+    srcs = np.concatenate([pick_sources(c1.deg, 60, 4242), ids[[0, 3, 7, 11]]]).astype(np.uint32)
+    print("case: batch c1+chain k=64 sparse sources", flush=True)
+    gg = ctx.graph_from_arrays(n, deg, W=W, S=S, csr_edges=ut, tile_heads=ch, tile_tails=ct, tile_offsets=coff)
+    try:
+        lv, st = gg.bfs_batch(srcs)
+    finally:
+        gg.free()
+    assert any(st.dir[i] for i in range(st.steps)), "no dense step"
+    for i, s in enumerate(srcs):
+        ref = O.bfs_levels(n, deg, ut, int(s))
+        want = np.where(ref >= 0, ref, 255).astype(np.uint8)
+        assert np.array_equal(lv[i], want), f"row {i} (source {s}) differs at {np.flatnonzero(lv[i] != want)[:8]}"
