@@ -175,6 +175,16 @@ __device__ __forceinline__ bool pull_runs(const StepCtrl *ctrl, uint32_t n)
 	return !ctrl->done && ctrl->dir == 1 && (ctrl->fsize < n / 8) == kFilter;
 }
 
+// The sources a dense step pulls: the live ones, less the sparse ones, which
+// batch_sparse_src_push_kernel pushes (unless their queue overflowed).
+template <int NW>
+__device__ __forceinline__ void pulled_sources(const u64 *__restrict__ live, u64 (&pull)[NW])
+{
+	const bool pushed = live[kSparseOverflow] == 0;
+#pragma unroll
+	for (int i = 0; i < NW; ++i) pull[i] = live[i] & (pushed ? ~live[kSparseWord + i] : ~0ull);
+}
+
 // Pull for vertices with 1..kPullLight in-edges.  Each lane first reads its
 // own vertex (in-edge range, want = sources not yet seen there whose frontier
 // is non-empty).  A vertex with at most kLightSlots in-edges is served by its
@@ -200,6 +210,8 @@ __global__ __launch_bounds__(kBlock) void batch_pull_light_kernel(
     uint32_t *__restrict__ touched)
 {
 	if (!pull_runs<kFilter>(ctrl, n)) return;
+	u64 pull[NW];
+	pulled_sources<NW>(live, pull);
 	const uint32_t lane = threadIdx.x & 63, g = lane % kLightGroup, grp = lane / kLightGroup;
 	// blocks walk the vertices kBlock at a time: a resident-size grid, so a
 	// launch on a step it has nothing to do in returns at once
@@ -215,7 +227,7 @@ __global__ __launch_bounds__(kBlock) void batch_pull_light_kernel(
 	}
 #pragma unroll
 	for (int i = 0; i < NW; ++i) {
-		want[i] = cnt ? ~seen[(uint64_t)v * NW + i] & live[i] : 0;
+		want[i] = cnt ? ~seen[(uint64_t)v * NW + i] & pull[i] : 0;
 		any |= want[i];
 	}
 	u64 got = 0;  // the wavefront's vertices (bit = lane) that got a source
@@ -314,12 +326,8 @@ __global__ __launch_bounds__(kBlock) void batch_pull_heavy_kernel(
     uint32_t *touched)
 {
 	if (!pull_runs<kFilter>(ctrl, n)) return;
-	// the sources to pull: live ones, less the sparse ones unless their
-	// queue overflowed
-	const bool pushed = live[kSparseOverflow] == 0;
 	u64 pull[NW];
-#pragma unroll
-	for (int i = 0; i < NW; ++i) pull[i] = live[i] & (pushed ? ~live[kSparseWord + i] : ~0ull);
+	pulled_sources<NW>(live, pull);
 	// loads in flight per lane between checks of the stop rule: a vertex stops
 	// as soon as every wanted source is found, so short rounds (64 or 128
 	// edges) beat deeper unrolling here
