@@ -147,8 +147,9 @@ int gphi_bellman_ford(gphi_graph *g, uint32_t src, uint32_t max_sweeps, uint32_t
 
 /* Multi-source BFS: k (1..256) sources in one traversal, bit-packed per
  * vertex; one edge stream serves all of them (no reference counterpart: the
- * batched form of graph_prepare, apps/bfs_simd/bfs.cpp:753-963, with the
- * same tile sweep, CSR push and direction rule applied to the union frontier).
+ * batched form of graph_prepare, apps/bfs_simd/bfs.cpp:753-963, with its
+ * CSR push and direction rule applied to the union frontier; dense steps pull
+ * over the in-edges, and push the sources whose own frontiers are small).
  * levels: k*n uint8, source-major (levels[i*n + v]); 255 = unreached; NULL
  * keeps them in HBM.  Each row equals gphi_bfs's levels of that source; a
  * traversal deeper than 254 levels returns GPHI_E_RANGE.  st->reached and
