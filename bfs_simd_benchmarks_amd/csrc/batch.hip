@@ -467,7 +467,7 @@ __global__ __launch_bounds__(kBlock) void batch_update_kernel(
     uint32_t n, uint32_t n_pitch, uint32_t next_level, StepCtrl *ctrl, const uint32_t *__restrict__ cur,
     uint32_t *__restrict__ touched, u64 *__restrict__ front, u64 *__restrict__ seen, u64 *__restrict__ next,
     uint8_t *__restrict__ level, uint8_t *__restrict__ flag, const uint32_t *__restrict__ slot_start,
-    const uint32_t *__restrict__ deg, u64 *slots, u64 *sums, uint32_t *scount)
+    const uint32_t *__restrict__ deg, u64 *slots, uint32_t *scount)
 {
 	if (ctrl->done || (ctrl->dir == 0) != kSpan) return;
 	__shared__ uint32_t s_words[kSpan ? 2 * kUpdateSpan / 32 : 1];  // touched words, then frontier words
@@ -882,7 +882,7 @@ uint32_t *slot_starts(gphi_graph *gr) { return gr->b_aux + kAuxStart; }
 // block says it is its step -- then the update of the vertices the step
 // touched, then the live-source mask.
 template <int NW>
-void enqueue_expand(gphi_graph *gr, uint32_t next_level)
+void enqueue_expand(gphi_graph *gr)
 {
 	const DevGraph &G = gr->g;
 	hipStream_t s = gr->ctx->stream;
@@ -918,10 +918,10 @@ void enqueue_update(gphi_graph *gr, uint32_t next_level)
 	// other's steps)
 	batch_update_kernel<NW, false><<<grid_n(G.n), kBlock, 0, s>>>(
 	    G.n, G.n_pad, next_level, gr->ctrl, gr->cur, gr->b_touch, gr->b_front, gr->b_seen, gr->b_next, gr->b_level,
-	    gr->flag, slot_starts(gr), G.deg, live_slots(gr), gr->b_sums, gr->b_scount);
+	    gr->flag, slot_starts(gr), G.deg, live_slots(gr), gr->b_scount);
 	batch_update_kernel<NW, true><<<G.n_pad / kUpdateSpan, kBlock, 0, s>>>(
 	    G.n, G.n_pad, next_level, gr->ctrl, gr->cur, gr->b_touch, gr->b_front, gr->b_seen, gr->b_next, gr->b_level,
-	    gr->flag, slot_starts(gr), G.deg, live_slots(gr), gr->b_sums, gr->b_scount);
+	    gr->flag, slot_starts(gr), G.deg, live_slots(gr), gr->b_scount);
 	batch_live_kernel<NW><<<1, 64, 0, s>>>(gr->ctrl, live_words(gr), live_slots(gr), gr->b_sums, gr->b_scount,
 	                                       (u64)G.m);
 }
@@ -956,9 +956,9 @@ int finalize_rows(gphi_graph *gr, uint32_t nw, const u64 *which)
 void enqueue_level(gphi_graph *gr, uint32_t nw, uint32_t next_level, bool expand_half)
 {
 	if (expand_half) {
-		if (nw == 1) enqueue_expand<1>(gr, next_level);
-		else if (nw == 2) enqueue_expand<2>(gr, next_level);
-		else enqueue_expand<4>(gr, next_level);
+		if (nw == 1) enqueue_expand<1>(gr);
+		else if (nw == 2) enqueue_expand<2>(gr);
+		else enqueue_expand<4>(gr);
 	} else {
 		if (nw == 1) enqueue_update<1>(gr, next_level);
 		else if (nw == 2) enqueue_update<2>(gr, next_level);

@@ -94,7 +94,7 @@ __global__ __launch_bounds__(kBlock) void chunk_meta_kernel(const uint32_t *__re
 		}
 		__syncthreads();
 		if (threadIdx.x == 0) {
-			for (int k = 1; k < kBlock / 64; ++k) {
+			for (uint32_t k = 1; k < kBlock / 64; ++k) {
 				rlo = min(rlo, s[0][k]);
 				rhi = max(rhi, s[1][k]);
 				clo = min(clo, s[2][k]);
@@ -1153,7 +1153,7 @@ __device__ __forceinline__ void sparse_push_block(
     uint32_t bid, uint32_t nbid, QueueBufs qb, StepCtrl *ctrl, const uint32_t *__restrict__ row_off,
     const uint32_t *__restrict__ cols, const uint32_t *__restrict__ csr_w, const uint32_t *__restrict__ deg,
     uint32_t *visited, uint8_t *__restrict__ flag, int32_t *__restrict__ parent, int32_t *__restrict__ level,
-    uint32_t *dist, uint32_t W, uint32_t *__restrict__ row_stamp, uint32_t *col_unvis)
+    uint32_t *dist, uint32_t W, uint32_t *__restrict__ row_stamp)
 {
 	__shared__ uint32_t s_off[kSparseTile + 2];
 	__shared__ uint32_t s_u[kSparseTile + 2];
@@ -1353,7 +1353,7 @@ __global__ __launch_bounds__(kBlock) void level_front_kernel(
 	if (ctrl->dir == 0) {
 		if (blockIdx.x < npush)
 			sparse_push_block<kBfs>(blockIdx.x, npush, qb, ctrl, row_off, cols, csr_w, deg, visited, flag, parent,
-			                        level, dist, W, row_stamp, col_unvis);
+			                        level, dist, W, row_stamp);
 		return;
 	}
 	if (kBfs && ctrl->queued) {

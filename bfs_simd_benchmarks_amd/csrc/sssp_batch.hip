@@ -39,7 +39,7 @@ __device__ __forceinline__ void load_row(const uint32_t *__restrict__ dist, uint
 {
 	const uint4 *p = reinterpret_cast<const uint4 *>(dist + (uint64_t)v * kSsspBatchMax);
 #pragma unroll
-	for (int i = 0; i < kSsspBatchMax / 4; ++i) {
+	for (uint32_t i = 0; i < kSsspBatchMax / 4; ++i) {
 		const uint4 x = p[i];
 		d[4 * i] = x.x;
 		d[4 * i + 1] = x.y;
