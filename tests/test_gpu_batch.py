@@ -121,11 +121,13 @@ def test_batch_hub_and_piece_boundaries(ctx, k):
         assert np.array_equal(lv[i], want), f"row {i} (source {srcs[i]}) differs"
 
 
-def test_batch_sparse_sources_pushed(ctx, c1):
+@pytest.mark.parametrize("k", [64, 100, 256])
+def test_batch_sparse_sources_pushed(ctx, c1, k):
     """Sources on a chain hanging off the graph keep one-vertex frontiers
     while the others' union frontier makes the steps dense: they are pushed
     from their frontier, the rest pulled.  Every row equals that source's own
-    levels, before and after the chain reaches the graph."""
+    levels, before and after the chain reaches the graph (one, two and four
+    mask words per vertex)."""
     chain = 12
     n = c1.n + chain
     ids = np.arange(c1.n, n, dtype=np.uint32)
@@ -136,8 +138,8 @@ def test_batch_sparse_sources_pushed(ctx, c1):
     W, S = c1.W, c1.S
     th, tt, _, off = O.coo_tile(n, W, uh, ut)
     ch, ct, _, coff = O.col_major(n, W, S, th, tt, None, off)
-    srcs = np.concatenate([pick_sources(c1.deg, 60, 4242), ids[[0, 3, 7, 11]]]).astype(np.uint32)
-    print("case: batch c1+chain k=64 sparse sources", flush=True)
+    srcs = np.concatenate([pick_sources(c1.deg, k - 4, 4242), ids[[0, 3, 7, 11]]]).astype(np.uint32)
+    print(f"case: batch c1+chain k={k} sparse sources", flush=True)
     gg = ctx.graph_from_arrays(n, deg, W=W, S=S, csr_edges=ut, tile_heads=ch, tile_tails=ct, tile_offsets=coff)
     try:
         lv, st = gg.bfs_batch(srcs)
