@@ -26,9 +26,13 @@ import time
 from pathlib import Path
 
 # CPU baseline placement (read by the OpenMP runtime when it loads, so before
-# any import that loads it): threads on consecutive physical cores
+# any import that loads it): threads on consecutive physical cores.  With a
+# binding policy the runtime pins the calling thread to one core at its first
+# parallel region, and threads created later inherit that mask: the serving
+# legs put their thread back on the cores the process started with.
 os.environ.setdefault("OMP_PROC_BIND", "close")
 os.environ.setdefault("OMP_PLACES", "cores")
+START_CPUS = os.sched_getaffinity(0)
 
 import numpy as np
 
@@ -475,6 +479,13 @@ def serving_loads(G, loads, queries) -> dict:
     import threading
 
     from bfs_simd_benchmarks_amd.workloads import SERVING_MIX, serving_stream
+
+    # the server's lane threads (and the producer below) inherit this thread's
+    # CPU mask: not the one core an OpenMP region may have pinned it to
+    try:
+        os.sched_setaffinity(0, START_CPUS)
+    except OSError:
+        pass
 
     results = {f"{q:g}": {} for q in loads}
     sizes = []

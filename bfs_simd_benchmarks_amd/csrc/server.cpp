@@ -94,13 +94,17 @@ struct Lane {
 
 void ema(double &x, double v) { x = x > 0 ? 0.8 * x + 0.2 * v : v; }
 
-// k pending queries share one traversal when its expected time beats k runs
-// alone; before both estimates exist, when at least `first` are pending
+// k pending queries share one traversal when that lowers their mean latency:
+// run one after another, the i-th of them completes after i runs alone, so
+// their mean is (k + 1) / 2 runs; sharing, each completes with the traversal.
+// (Comparing with all k runs instead maximises throughput but makes queries
+// wait on wide traversals at loads the runs alone still keep up with.)
+// Before both estimates exist: when at least `first` are pending.
 bool share_pays(const RunCosts &runs, double t_alone, size_t k, size_t first)
 {
 	if (k <= 1) return false;
 	const double run = runs.upper(k);
-	if (t_alone > 0 && run > 0) return run < (double)k * t_alone;
+	if (t_alone > 0 && run > 0) return run < 0.5 * (double)(k + 1) * t_alone;
 	return k >= first;
 }
 
