@@ -1146,7 +1146,7 @@ __global__ __launch_bounds__(kBlock) void sssp_dense_kernel(
 // queue -- position and degree prefix reserved with one atomic per block and
 // tile -- and the first-entry index of every next-step tile its edges start.
 // So the level needs no pass over all vertices.  Column counts are left to a
-// recount before the next dense step (col_recount_kernel).  A large step
+// recount before the next dense step (col_recount).  A large step
 // claims through the state byte and leaves the next frontier to that pass.
 template <bool kBfs>
 __device__ __forceinline__ void sparse_push_block(
@@ -1587,24 +1587,22 @@ __global__ __launch_bounds__(kScanThreads) void frontier_scan_kernel(uint32_t nb
 // before a BFS dense step when queue-building sparse steps came before it
 // (their pushes leave the counts alone: many winners share a column, and one
 // atomic each on it would queue); the dense step's chunk selection skips
-// columns with none left.  A thread per column.
-__global__ __launch_bounds__(kBlock) void col_recount_kernel(const StepCtrl *__restrict__ ctrl, uint32_t n,
-                                                             uint32_t W, uint32_t side,
-                                                             const uint32_t *__restrict__ visited,
-                                                             uint32_t *__restrict__ col_unvis)
+// columns with none left.  A thread per column (a grid-stride walk), run by
+// the emit kernel's threads: the two passes never both have work at a level.
+__device__ __forceinline__ void col_recount(uint32_t n, uint32_t W, uint32_t side,
+                                            const uint32_t *__restrict__ visited, uint32_t *__restrict__ col_unvis)
 {
-	if (ctrl->done || !ctrl->col_stale || ctrl->dir != 1) return;
-	const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
-	if (c >= side) return;
-	const uint32_t lo = c * W, hi = min(lo + W, n);  // vertices [lo, hi)
-	uint32_t seen = 0;
-	for (uint32_t w = lo >> 5; w <= (hi - 1) >> 5; ++w) {
-		uint32_t bits = visited[w];
-		if (w == (lo >> 5) && (lo & 31)) bits &= ~0u << (lo & 31);
-		if (w == ((hi - 1) >> 5) && (hi & 31)) bits &= ~0u >> (32 - (hi & 31));
-		seen += (uint32_t)__popc(bits);
+	for (uint32_t c = blockIdx.x * kBlock + threadIdx.x; c < side; c += gridDim.x * kBlock) {
+		const uint32_t lo = c * W, hi = min(lo + W, n);  // vertices [lo, hi)
+		uint32_t seen = 0;
+		for (uint32_t w = lo >> 5; w <= (hi - 1) >> 5; ++w) {
+			uint32_t bits = visited[w];
+			if (w == (lo >> 5) && (lo & 31)) bits &= ~0u << (lo & 31);
+			if (w == ((hi - 1) >> 5) && (hi & 31)) bits &= ~0u >> (32 - (hi & 31));
+			seen += (uint32_t)__popc(bits);
+		}
+		col_unvis[c] = (hi - lo) - seen;
 	}
-	col_unvis[c] = (hi - lo) - seen;
 }
 
 // The queue goes to the half of the next level's parity.  queue_path: after a
@@ -1615,9 +1613,12 @@ __global__ __launch_bounds__(kBlock) void col_recount_kernel(const StepCtrl *__r
 __global__ __launch_bounds__(kBlock) void frontier_emit_kernel(
     const StepCtrl *__restrict__ ctrl, const uint32_t *__restrict__ deg, uint32_t *__restrict__ cur,
     const uint4 *__restrict__ blk_cnt, const uint2 *__restrict__ blk_base, const uint2 *__restrict__ wave_cnt,
-    QueueBufs qb, bool queue_path)
+    QueueBufs qb, bool queue_path, uint32_t n, uint32_t W, uint32_t side, const uint32_t *__restrict__ visited,
+    uint32_t *__restrict__ col_unvis)
 {
 	if (ctrl->done) return;
+	// BFS, before a dense step that follows queue-building steps: the column counts
+	if (col_unvis && ctrl->col_stale && ctrl->dir == 1) col_recount(n, W, side, visited, col_unvis);
 	const uint32_t w = blockIdx.x * kBlock + threadIdx.x;
 	if (queue_path && ctrl->queued) {
 		if (ctrl->dir == 1) cur[w] = 0u;
@@ -1779,10 +1780,8 @@ static int frontier_loop(gphi_graph *gr, uint32_t src, gphi_stats *st)
 			frontier_scan_kernel<<<1, kScanThreads, 0, s>>>(nblk, threshold, gr->blk_cnt, gr->blk_base,
 			                                              gr->live_cnt, gr->ctrl, gr->trace, kBfs);
 			frontier_emit_kernel<<<nblk, kBlock, 0, s>>>(gr->ctrl, G.deg, gr->cur, gr->blk_cnt, gr->blk_base,
-			                                            gr->wave_cnt, qb, kBfs);
-			if (kBfs)
-				col_recount_kernel<<<grid_for(G.side, kBlock, 1u << 30), kBlock, 0, s>>>(gr->ctrl, G.n, G.W, G.side,
-				                                                                        gr->visited, gr->col_unvis);
+			                                            gr->wave_cnt, qb, kBfs, G.n, G.W, G.side, gr->visited,
+			                                            kBfs ? gr->col_unvis : nullptr);
 			if (ev_step) GPHI_HIP(ctx, hipEventRecord(ev[3 + 2 * enq], s));
 		}
 		GPHI_HIP(ctx, hipGetLastError());
@@ -1855,7 +1854,8 @@ void launch_frontier_update_driven(gphi_graph *gr, uint32_t threshold)
 	frontier_scan_kernel<<<1, kScanThreads, 0, s>>>(nblk, threshold, gr->blk_cnt, gr->blk_base, gr->live_cnt,
 	                                              gr->ctrl, gr->trace, false);
 	frontier_emit_kernel<<<nblk, kBlock, 0, s>>>(gr->ctrl, G.deg, gr->cur, gr->blk_cnt, gr->blk_base,
-	                                            gr->wave_cnt, gr->queue_bufs(), false);
+	                                            gr->wave_cnt, gr->queue_bufs(), false, G.n, G.W, G.side, gr->visited,
+	                                            nullptr);
 }
 
 // level and parent -1 for the vertices a BFS did not reach (state 0; every
