@@ -13,8 +13,8 @@
 // distances whatever the order of the relaxations, so each row equals that
 // source's single-source result (and Dijkstra's).
 //
-// The steps reuse the single-source machinery: the direction rule on the
-// union frontier (|F| + sum deg > M / T_RATIO), the frontier update in its
+// The steps reuse the single-source machinery: a direction rule on the
+// union frontier (|F| + sum deg > M / 2, see kSsspBatchDenseDiv), the frontier update in its
 // device-driven form (state byte 1 = some distance dropped) and the CSR push
 // of a sparse step over the union queue.  A dense step is a pull over the
 // in-edges and their weights (build_in_weights), skipping heads outside the
@@ -34,6 +34,13 @@ namespace {
 constexpr uint32_t kInf = 2147483647u;  // unreached, as sssp.cpp:994-998
 constexpr uint32_t kSsspLevelGroup = 8;  // levels enqueued per read-back of the control block
 constexpr unsigned kSsspGrid = 8192;  // blocks of the push and pull launches (grid-stride loops)
+// A step is dense when the union frontier's |F| + sum deg(F) exceeds M / 2,
+// not M / T_RATIO as in the single-source rule: a push step relaxes 16
+// distances per frontier edge with one row gather, while a pull step reads
+// every in-edge.  Measured over divisors 20, 5, 4, 2 and 1: 2 is fastest on
+// rmat25-ef36 (106 against 112 ms for 16 sources) and on the serving mix's
+// rmat22 and rmat24 graphs (6.9 against 7.7 ms, 23.0 against 24.5 ms).
+constexpr uint64_t kSsspBatchDenseDiv = 2;
 
 __device__ __forceinline__ void load_row(const uint32_t *__restrict__ dist, uint32_t v, uint32_t (&d)[kSsspBatchMax])
 {
@@ -321,7 +328,7 @@ int run_sssp_batch(gphi_graph *gr, const uint32_t *srcs, uint32_t k, uint32_t *c
 	    G.n_pad, G.n_words, reinterpret_cast<uint4 *>(gr->sb_dist), gr->flag, gr->cur, gr->ctrl);
 	GPHI_HIP(ctx, hipMemcpyAsync(d_srcs, srcs, (size_t)k * 4, hipMemcpyHostToDevice, s));
 	ssspb_seed_kernel<<<1, 64, 0, s>>>(d_srcs, k, gr->sb_dist, gr->flag);
-	const uint32_t threshold = (uint32_t)G.m / ctx->t_ratio;
+	const uint32_t threshold = (uint32_t)(G.m / kSsspBatchDenseDiv);
 	launch_frontier_update_driven(gr, threshold);  // opens level 0: the sources
 	GPHI_HIP(ctx, hipGetLastError());
 	const unsigned lgrid = std::min<unsigned>((G.n + kBlock - 1) / kBlock, kSsspGrid);
