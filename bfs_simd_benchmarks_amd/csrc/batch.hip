@@ -191,7 +191,9 @@ __device__ __forceinline__ void pulled_sources(const u64 *__restrict__ live, u64
 // own lane; the wavefront then serves its other vertices that want something,
 // kLightGroup at a time, with kLightGroup lanes per vertex taking every
 // kLightGroup-th in-edge (at most kLightSlots each) and an OR across the
-// group.  next[v] is zero on entry and written only by v's lane or group; the
+// group: first the vertex's first 2 * kLightGroup in-edges, then the rest only
+// if a wanted source is still missing (after the first dense step most
+// vertices find every source they want among their first in-edges).  next[v] is zero on entry and written only by v's lane or group; the
 // wavefront's 64 vertices are two whole touched-bitmap words, which it
 // writes with plain stores (the heavy pull, a later launch, ORs into them).
 constexpr uint32_t kLightGroup = 8;
@@ -278,17 +280,42 @@ __global__ __launch_bounds__(kBlock) void batch_pull_light_kernel(
 			gwant[i] = __shfl(want[i], (int)from, 64);
 			acc[i] = 0;
 		}
-		uint32_t h[kLightSlots];
+		// the vertex's first 2 * kLightGroup in-edges, then the rest only for
+		// a vertex that still misses a wanted source (its group decides; the
+		// whole wavefront runs both rounds, so every lane joins the shuffles)
+		constexpr uint32_t kFirst = 2;
+		uint32_t h1[kFirst];
 #pragma unroll
-		for (uint32_t j = 0; j < kLightSlots; ++j) {
+		for (uint32_t j = 0; j < kFirst; ++j) {
 			const uint32_t k = j * kLightGroup + g;
-			h[j] = k < gcnt ? in_src[ge0 + k] : 0xFFFFFFFFu;
+			h1[j] = k < gcnt ? in_src[ge0 + k] : 0xFFFFFFFFu;
 		}
-		gather_fronts<NW, kFilter, kLightSlots>(h, cur, front, acc);
+		gather_fronts<NW, kFilter, kFirst>(h1, cur, front, acc);
 #pragma unroll
 		for (int i = 0; i < NW; ++i)
 #pragma unroll
 			for (int o = kLightGroup / 2; o > 0; o >>= 1) acc[i] |= __shfl_xor(acc[i], o, kLightGroup);
+		bool miss = false;
+#pragma unroll
+		for (int i = 0; i < NW; ++i) miss |= (gwant[i] & ~acc[i]) != 0;
+		if (__ballot(miss && gcnt > kFirst * kLightGroup)) {  // uniform
+			uint32_t h2[kLightSlots - kFirst];
+#pragma unroll
+			for (uint32_t j = kFirst; j < kLightSlots; ++j) {
+				const uint32_t k = j * kLightGroup + g;
+				h2[j - kFirst] = miss && k < gcnt ? in_src[ge0 + k] : 0xFFFFFFFFu;
+			}
+			u64 acc2[NW];
+#pragma unroll
+			for (int i = 0; i < NW; ++i) acc2[i] = 0;
+			gather_fronts<NW, kFilter, kLightSlots - kFirst>(h2, cur, front, acc2);
+#pragma unroll
+			for (int i = 0; i < NW; ++i) {
+#pragma unroll
+				for (int o = kLightGroup / 2; o > 0; o >>= 1) acc2[i] |= __shfl_xor(acc2[i], o, kLightGroup);
+				acc[i] |= acc2[i];
+			}
+		}
 		if (g == 0 && gcnt) {
 			const uint64_t gv = v0 + src;
 			bool hit = false;
