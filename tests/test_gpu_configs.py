@@ -136,6 +136,23 @@ def test_c4_batched_64_sources(c4):
     assert st.reached == reached
 
 
+def test_c4_batched_256_sources(c4):
+    """Four mask words per vertex at full scale: every 16th row against the
+    plain GPU BFS of its source, two against the oracle."""
+    g, h = c4
+    srcs = pick_sources(h.deg, 256, WORKLOADS["c4"].seed + 1000 + 256)
+    print("case: c4 batch of 256", flush=True)
+    lv, _ = g.bfs_batch(srcs)
+    for i in (0, 255):
+        print(f"case: c4 batch-256 row {i} (source {srcs[i]}) vs oracle", flush=True)
+        ref, _, _ = oracle_levels(h, srcs[i])
+        assert np.array_equal(lv[i], np.where(ref >= 0, ref, 255).astype(np.uint8)), i
+    for i in range(0, 256, 16):
+        one, _, _ = g.bfs(int(srcs[i]), parents=False)
+        assert np.array_equal(lv[i], np.where(one >= 0, one, 255).astype(np.uint8)), i
+    del lv
+
+
 # ------------------------------------------------------------ serving mix
 def test_c5_serving_mix_against_oracle():
     queries = 240
