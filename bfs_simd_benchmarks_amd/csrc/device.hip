@@ -1428,24 +1428,31 @@ __global__ __launch_bounds__(kBlock) void frontier_mark_kernel(
 	// 2048 vertices: in iteration i lane l takes vertex 64i + l, whose bit is in
 	// word 2i + l/32 of the wave, so every level store and degree load of a
 	// iteration covers 64 consecutive vertices (a lane per word would scatter
-	// them 128 bytes apart).  Iterations without a new vertex are skipped
-	// (uniform).
+	// them 128 bytes apart).  The 32 degree loads are all issued before the
+	// first is summed (one memory latency per wave, not 32); the words come
+	// from their lanes by readlane.
 	{
 		const uint32_t lane = lane_id();
 		const uint32_t wv0 = (w - lane) * 32;
-		const unsigned long long any = __ballot(bits != 0);
-		if (any) {
+		if (__ballot(bits != 0)) {
+			uint32_t dv[32];
+#pragma unroll
 			for (uint32_t i = 0; i < 32; ++i) {
-				const uint32_t wb = (uint32_t)__shfl((int)bits, (int)(2 * i + (lane >> 5)), 64);
-				if (!((any >> (2 * i)) & 3ull)) continue;
-				if ((wb >> (lane & 31)) & 1u) {
-					const uint32_t v = wv0 + 64 * i + lane;
+				const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)bits, (int)(2 * i));
+				const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)bits, (int)(2 * i + 1));
+				const uint32_t wb = lane < 32 ? lo : hi;
+				const uint32_t v = wv0 + 64 * i + lane;
+				const bool mine = (wb >> (lane & 31)) & 1u;
+				dv[i] = mine ? deg[v] : 0u;
+				if (mine) {
 					if (kBfs) level[v] = (int32_t)next_level;
 					if (!row_uniform) row_stamp[v / W] = next_level;
-					const uint32_t d = deg[v];
-					D += d;
-					Q += d > 0;
 				}
+			}
+#pragma unroll
+			for (uint32_t i = 0; i < 32; ++i) {
+				D += dv[i];
+				Q += dv[i] > 0;
 			}
 		}
 	}
@@ -1642,16 +1649,25 @@ __global__ __launch_bounds__(kBlock) void frontier_emit_kernel(
 	}
 	// vertex-major over the wave's 2048 vertices, as in the mark kernel: in
 	// iteration i lane l takes vertex 64i + l, so the entries come out in
-	// ascending vertex order and each store covers consecutive entries
+	// ascending vertex order and each store covers consecutive entries.  The
+	// 32 degree loads are issued before the first scan needs one.
 	const uint32_t wv0 = (w - lane) * 32;
 	const unsigned long long any = __ballot(bits != 0);
 	const unsigned long long below = (1ull << lane) - 1ull;
-	uint32_t pos = base.x, off = base.y;
+	uint32_t dd[32];
+#pragma unroll
 	for (uint32_t i = 0; i < 32; ++i) {
-		const uint32_t wb = (uint32_t)__shfl((int)bits, (int)(2 * i + (lane >> 5)), 64);
+		const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)bits, (int)(2 * i));
+		const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)bits, (int)(2 * i + 1));
+		const uint32_t wb = lane < 32 ? lo : hi;
+		dd[i] = ((wb >> (lane & 31)) & 1u) ? deg[wv0 + 64 * i + lane] : 0u;
+	}
+	uint32_t pos = base.x, off = base.y;
+#pragma unroll
+	for (uint32_t i = 0; i < 32; ++i) {
 		if (!((any >> (2 * i)) & 3ull)) continue;
 		const uint32_t v = wv0 + 64 * i + lane;
-		const uint32_t dv = ((wb >> (lane & 31)) & 1u) ? deg[v] : 0u;
+		const uint32_t dv = dd[i];
 		const unsigned long long qm = __ballot(dv > 0);
 		uint32_t dtot;
 		const uint32_t dx = wave_excl_scan(dv, &dtot);
