@@ -253,13 +253,27 @@ def test_sssp_wide_weights(ctx, c3, src):
 
 
 def test_bfs_level_timing_off(ctx, c1):
-    # per-level event timing off: same levels and trace, only the total time
+    """Per-level event timing off: same levels and trace, only the total time.
+    The loop then enqueues its levels back to back with no events between
+    them, so it is also run from other sources, after a retile, and on a
+    path deeper than one 8-level group, each against the oracle."""
     gg = upload(ctx, c1)
     try:
         lv_on, _, st_on = gg.bfs(0)
         ctx.set_level_timing(False)
         try:
             lv_off, _, st_off = gg.bfs(0)
+            for src in (0, 5, 1000, 0):
+                print(f"case: timing off, source {src}", flush=True)
+                lv, pa, _ = gg.bfs(src)
+                ref = O.bfs_levels(c1.n, c1.deg, c1.cols, src)
+                assert np.array_equal(lv, ref), src
+                assert O.check_parents(c1.n, c1.deg, c1.cols, src, lv, pa) == 0, src
+            print("case: timing off after a retile to W=512 S=8", flush=True)
+            gg.retile(512, 8)
+            lv, _, st = gg.bfs(5)
+            assert np.array_equal(lv, O.bfs_levels(c1.n, c1.deg, c1.cols, 5))
+            assert st.total_ms > 0
         finally:
             ctx.set_level_timing(True)
         assert np.array_equal(lv_on, lv_off)
@@ -269,6 +283,19 @@ def test_bfs_level_timing_off(ctx, c1):
         assert st_off.total_ms > 0 and any(st_on.sweep_times())
     finally:
         gg.free()
+    n = 40  # a path 0 -> 1 -> ... -> 39: 39 levels, past the first 8
+    path = O.tile_edgelist(n, np.arange(n - 1, dtype=np.uint32), np.arange(1, n, dtype=np.uint32), 4, 2)
+    pg = upload(ctx, path)
+    ctx.set_level_timing(False)
+    try:
+        for src in (0, 30):
+            print(f"case: timing off, path of {n}, source {src}", flush=True)
+            lv, _, st = pg.bfs(src)
+            assert np.array_equal(lv, O.bfs_levels(n, path.deg, path.cols, src)), src
+            assert st.steps == n - src
+    finally:
+        ctx.set_level_timing(True)
+        pg.free()
 
 
 def brute_force_step_bytes(g, lv, trace):
