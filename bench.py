@@ -22,6 +22,7 @@ import argparse
 import json
 import os
 import sys
+import threading
 import time
 from pathlib import Path
 
@@ -99,6 +100,43 @@ def measured_traffic(config: str, W: int, S: int):
     return round(b), str(f.relative_to(f.parents[1])), mean_ms
 
 
+def host_link_probe(mib: int, reps: int = 20) -> int:
+    """Device-to-host copy bandwidth, GB/s: pinned and pageable destinations,
+    1 and 8 threads each copying its own buffer on its own stream."""
+    import torch
+    nbytes = mib * (1 << 20)
+    res = {}
+    for pinned in (True, False):
+        for nthreads in (1, 8):
+            srcs = [torch.full((nbytes,), i + 1, dtype=torch.uint8, device="cuda") for i in range(nthreads)]
+            dsts = [torch.empty(nbytes, dtype=torch.uint8, pin_memory=pinned) for _ in range(nthreads)]
+            streams = [torch.cuda.Stream() for _ in range(nthreads)]
+            torch.cuda.synchronize()
+
+            def work(i, count):
+                with torch.cuda.stream(streams[i]):
+                    for _ in range(count):
+                        dsts[i].copy_(srcs[i], non_blocking=pinned)
+                    streams[i].synchronize()
+            for i in range(nthreads):
+                work(i, 1)
+            t0 = time.perf_counter()
+            th = [threading.Thread(target=work, args=(i, reps)) for i in range(nthreads)]
+            for t in th:
+                t.start()
+            for t in th:
+                t.join()
+            dt = time.perf_counter() - t0
+            ok = all(int(dsts[i][0]) == i + 1 and int(dsts[i][-1]) == i + 1 for i in range(nthreads))
+            key = f"{'pinned' if pinned else 'pageable'}_{nthreads}_threads_GBps"
+            res[key] = round(nthreads * reps * nbytes / dt / 1e9, 1)
+            res["ok"] = res.get("ok", True) and ok
+            print(f"[bench] host link {key}: {res[key]}", file=sys.stderr, flush=True)
+            del srcs, dsts
+    print(json.dumps({"host_link_probe_MiB": mib, "copies_per_thread": reps, **res}))
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -121,6 +159,10 @@ def main() -> int:
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB five times (calibrates the "
                          "FETCH_SIZE / WRITE_SIZE counters against known bytes)")
+    ap.add_argument("--host-link-probe", type=int, default=0, metavar="MIB",
+                    help="only time device-to-host copies of MIB MiB into pinned and pageable host "
+                         "memory, from 1 and 8 threads with a stream each (the host link that "
+                         "answers copied out of HBM cross)")
     ap.add_argument("--serving", action="store_true",
                     help="run the serving mix instead: 8 resident RMAT graphs (scales 18-24), a "
                          "Poisson stream of BFS/SSSP queries, batched vs one-at-a-time")
@@ -141,6 +183,9 @@ def main() -> int:
         print(json.dumps({"copy_probe_bytes_read": n * 4, "copies": 5,
                           "ok": bool(torch.equal(src, dst))}))
         return 0
+
+    if args.host_link_probe:
+        return host_link_probe(args.host_link_probe)
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -477,8 +522,6 @@ def serving_loads(G, loads, queries) -> dict:
     per traversal, later arrivals join it between levels) and against one that
     runs every query alone (max_batch 1).  Past saturation the answered rate
     stays below the offered one and latency grows with the backlog."""
-    import threading
-
     from bfs_simd_benchmarks_amd.workloads import SERVING_MIX, serving_stream
 
     # the server's lane threads (and the producer below) inherit this thread's
